@@ -1,0 +1,196 @@
+/* C ABI of the gfx950 SPAI-GMRES-IR library (libbspai_hip.so).
+ *
+ * Plain C types only: pointers, sizes, int status codes; no C++ or torch types
+ * cross this boundary and no exception escapes it.  Every call is synchronous
+ * on the context's stream when it returns (as the reference's CUDA objects
+ * are, include/bspai/cuda_gmres.hpp:18-20) and a context is not thread-safe.
+ *
+ * Vectors and matrices may live on the host (where = BSPAI_HIP_HOST: copied
+ * in/out by the library) or on the device (BSPAI_HIP_DEVICE: used in place;
+ * e.g. torch tensor data pointers).  Indices are int32 on the device side.
+ *
+ * Reference interfaces replaced (reference tree, file:line):
+ *   bspai_hip_matrix_create / bspai_hip_spmv    CudaCsrSpmv      include/bspai/cuda_spmv.hpp:7-19
+ *   bspai_hip_spai_build (+ _info/_columns/...) cuda_construct_spai include/bspai/cuda_spai.hpp:47
+ *                                               construct_spai    include/bspai/spai.hpp:69
+ *   bspai_hip_score                             cuda_evaluate_spai_score include/bspai/cuda_spai.hpp:48-49
+ *   bspai_hip_gmres                             cuda_left_preconditioned_gmres include/bspai/cuda_gmres.hpp:40-45
+ *                                               left_preconditioned_gmres include/bspai/gmres.hpp:55-57
+ *   bspai_hip_ir_solve                          cuda_refine_with_preconditioner include/bspai/cuda_ir.hpp:19-24
+ *   bspai_hip_spai_gmres_ir                     cuda_gpu_spai_gmres_ir include/bspai/cuda_ir.hpp:26-28
+ *                                               spai_gmres_ir     include/bspai/ir.hpp:65-66
+ *   bspai_hip_adjoint_solve, bspai_hip_sddmm    no reference counterpart (differentiable solve)
+ */
+#ifndef BSPAI_HIP_H
+#define BSPAI_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BSPAI_HIP_ABI_VERSION 1
+
+/* return codes */
+#define BSPAI_HIP_OK 0
+#define BSPAI_HIP_EINVAL 1  /* invalid argument (message in bspai_hip_last_error) */
+#define BSPAI_HIP_EDEVICE 2 /* HIP runtime / device failure */
+#define BSPAI_HIP_ENOMEM 3  /* device allocation failed */
+
+/* memory kinds */
+#define BSPAI_HIP_HOST 0
+#define BSPAI_HIP_DEVICE 1
+
+/* precision of the SPAI build and of the inner GMRES */
+#define BSPAI_HIP_FP64 0  /* plain: FP64 SPAI, GMRES and IR */
+#define BSPAI_HIP_MIXED 1 /* FP32 SPAI + FP32 GMRES inner, FP64 residual/update/Givens */
+
+/* status values mirror the reference enums:
+ *   SpaiStatus  include/bspai/spai.hpp:10-21   (success=0 ... iteration_limit=9)
+ *   GmresStatus include/bspai/gmres.hpp:10-20  (success=0 ... numerical_failure=8)
+ *   IrStatus    include/bspai/ir.hpp:9-17      (success=0 ... numerical_failure=6) */
+
+#define BSPAI_HIP_MAX_OUTER 24
+
+typedef struct bspai_hip_ctx bspai_hip_ctx;
+typedef struct bspai_hip_matrix bspai_hip_matrix;
+typedef struct bspai_hip_spai bspai_hip_spai;
+
+typedef struct {
+    double epsilon; /* accept a column when ||B(I,J) m - e_k|| <= epsilon */
+    int64_t alpha;  /* growth budget per column */
+    int64_t beta;   /* max indices added per growth */
+    int precision;  /* BSPAI_HIP_FP64 or BSPAI_HIP_MIXED */
+} bspai_hip_spai_options;
+
+typedef struct {
+    int status;              /* SpaiStatus */
+    int64_t failing_column;  /* first failing column, -1 if none */
+    int64_t n, p_nnz, pattern_nnz;
+    int64_t fits, growths, clamps;
+    int passes;              /* capacity buckets used */
+    double build_ms;         /* device time of the construction */
+    char message[256];
+} bspai_hip_spai_info;
+
+typedef struct {
+    double tolerance;        /* tau on ||P(b-Ax)||/||Pb|| */
+    int64_t max_iterations;  /* 0 = n */
+} bspai_hip_gmres_options;
+
+typedef struct {
+    int status, converged;
+    int64_t iterations;
+    double initial_original, initial_relative, final_original, final_relative;
+} bspai_hip_gmres_info;
+
+typedef struct {
+    double spai_epsilon;
+    int64_t spai_alpha;      /* 0 = ceil(n/8), as the solver CLI */
+    int64_t spai_beta;
+    double gmres_tolerance;  /* inner tau */
+    int64_t gmres_max_iterations; /* 0 = n */
+    int64_t max_outer_steps; /* <= 20 */
+    int precision;           /* BSPAI_HIP_FP64 or BSPAI_HIP_MIXED */
+    double relres_target;    /* > 0: also require ||b-Ax||_2 <= target*||b||_2 */
+} bspai_hip_ir_options;
+
+typedef struct {
+    int status, converged, construction_performed, spai_status;
+    int64_t outer_steps, inner_attempts;
+    int inner_status[BSPAI_HIP_MAX_OUTER];
+    int inner_iterations[BSPAI_HIP_MAX_OUTER];
+    int history_len;
+    double history[BSPAI_HIP_MAX_OUTER][4]; /* residual_inf, residual_2, nbe, cbe */
+    double residual_inf, residual_2, normwise_backward_error, componentwise_backward_error;
+    double spai_ms, solve_ms;
+    char message[256];
+} bspai_hip_ir_info;
+
+int bspai_hip_abi_version(void);
+
+/* context: device ordinal; stream = hipStream_t to run on, or NULL for a
+ * library-owned stream (created at default priority). */
+int bspai_hip_ctx_create(int device, void *stream, bspai_hip_ctx **out);
+void bspai_hip_ctx_destroy(bspai_hip_ctx *ctx);
+const char *bspai_hip_last_error(const bspai_hip_ctx *ctx);
+int bspai_hip_synchronize(bspai_hip_ctx *ctx);
+
+/* n x n CSR (canonical: sorted columns, no duplicates).  nsys > 1 declares a
+ * block-diagonal batch of independent systems: system s owns rows
+ * [sys_offsets[s], sys_offsets[s+1]) (host array, nsys+1 entries). */
+int bspai_hip_matrix_create(bspai_hip_ctx *ctx, int64_t n, int64_t nnz, const int32_t *ptr,
+                            const int32_t *col, const double *val, int where, int nsys,
+                            const int64_t *sys_offsets, bspai_hip_matrix **out);
+void bspai_hip_matrix_destroy(bspai_hip_matrix *m);
+int bspai_hip_matrix_info(const bspai_hip_matrix *m, int64_t *n, int64_t *nnz, int *nsys);
+int bspai_hip_matrix_set_values(bspai_hip_ctx *ctx, bspai_hip_matrix *m, const double *val,
+                                int where);
+int bspai_hip_matrix_download(bspai_hip_ctx *ctx, const bspai_hip_matrix *m, int32_t *ptr,
+                              int32_t *col, double *val);
+int bspai_hip_matrix_transpose(bspai_hip_ctx *ctx, const bspai_hip_matrix *m,
+                               bspai_hip_matrix **out);
+
+/* y = A x in FP64 (lanes: 0 = library choice, 1 = sequential row order,
+ * bit-identical to the reference's CsrMatrix::multiply). */
+int bspai_hip_spmv(bspai_hip_ctx *ctx, const bspai_hip_matrix *a, const double *x, double *y,
+                   int where, int lanes);
+/* y = A x with FP32 values and vectors (the mixed path's inner product). */
+int bspai_hip_spmv_f32(bspai_hip_ctx *ctx, const bspai_hip_matrix *a, const float *x, float *y,
+                       int where, int lanes);
+
+/* SPAI construction (device resident), P = N^T D. */
+int bspai_hip_spai_build(bspai_hip_ctx *ctx, const bspai_hip_matrix *a,
+                         const bspai_hip_spai_options *opt, bspai_hip_spai **out);
+void bspai_hip_spai_destroy(bspai_hip_spai *p);
+int bspai_hip_spai_get_info(const bspai_hip_spai *p, bspai_hip_spai_info *info);
+/* canonical P (int32 structure, values widened to FP64), host arrays:
+ * ptr[n+1], col[p_nnz], val[p_nnz] */
+int bspai_hip_spai_preconditioner(bspai_hip_ctx *ctx, const bspai_hip_spai *p, int32_t *ptr,
+                                  int32_t *col, double *val);
+/* per-column diagnostics, host arrays of n */
+int bspai_hip_spai_columns(bspai_hip_ctx *ctx, const bspai_hip_spai *p, int32_t *status,
+                           int32_t *fits, int32_t *growths, int32_t *clamps, double *residual,
+                           double *pivot_ratio);
+/* final patterns J_k and coefficients (exact zeros kept): ptr[n+1] (int64),
+ * idx[pattern_nnz], coef[pattern_nnz] */
+int bspai_hip_spai_patterns(bspai_hip_ctx *ctx, const bspai_hip_spai *p, int64_t *ptr,
+                            int32_t *idx, double *coef);
+int bspai_hip_spai_scaling(bspai_hip_ctx *ctx, const bspai_hip_spai *p, double *d);
+/* restricted score rho of one candidate on the device (FP64). */
+int bspai_hip_score(bspai_hip_ctx *ctx, double residual_squared, double dot,
+                    double denominator_squared, int *status, int *clamped, double *score);
+
+/* One left-preconditioned GMRES solve (FP64) per system; info[nsys]. */
+int bspai_hip_gmres(bspai_hip_ctx *ctx, const bspai_hip_matrix *a, const bspai_hip_matrix *p,
+                    const double *rhs, const double *x0, const bspai_hip_gmres_options *opt,
+                    double *x, int where, bspai_hip_gmres_info *info);
+
+/* Iterative refinement with a built preconditioner; info[nsys]. */
+int bspai_hip_ir_solve(bspai_hip_ctx *ctx, const bspai_hip_matrix *a, const bspai_hip_spai *p,
+                       const double *b, const bspai_hip_ir_options *opt, double *x, int where,
+                       bspai_hip_ir_info *info);
+/* SPAI construction + refinement (the solver CLI's call); info[nsys]. */
+int bspai_hip_spai_gmres_ir(bspai_hip_ctx *ctx, const bspai_hip_matrix *a, const double *b,
+                            const bspai_hip_ir_options *opt, double *x, int where,
+                            bspai_hip_ir_info *info);
+
+/* Adjoint solve A^T lambda = g with the transposed preconditioner (right
+ * preconditioning by P^T) under the same IR/precision options; at = A^T and
+ * pt = P^T as matrices; info[nsys]. */
+int bspai_hip_adjoint_solve(bspai_hip_ctx *ctx, const bspai_hip_matrix *at,
+                            const bspai_hip_matrix *pt, const double *g,
+                            const bspai_hip_ir_options *opt, double *lambda, int where,
+                            bspai_hip_ir_info *info);
+/* P (or P^T) of a build as a matrix handle (FP64 values). */
+int bspai_hip_spai_matrix(bspai_hip_ctx *ctx, const bspai_hip_spai *p, int transpose,
+                          bspai_hip_matrix **out);
+/* out[q] = alpha * u[row(q)] * v[col(q)] on A's stored pattern (SDDMM). */
+int bspai_hip_sddmm(bspai_hip_ctx *ctx, const bspai_hip_matrix *a, double alpha, const double *u,
+                    const double *v, double *out, int where);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,698 @@
+// Host orchestration of the SPAI / GMRES / IR kernels.
+#include "engine.hpp"
+
+#include <algorithm>
+#include <climits>
+#include <cmath>
+#include <cstddef>
+
+namespace bspai_hip {
+
+namespace {
+template <typename T> void upload(const Context &c, T *dst, const T *src, std::size_t n) {
+    if (n)
+        HIP_CHECK(hipMemcpyAsync(dst, src, n * sizeof(T), hipMemcpyHostToDevice, c.stream));
+}
+template <typename T> void download(const Context &c, T *dst, const T *src, std::size_t n) {
+    if (n)
+        HIP_CHECK(hipMemcpyAsync(dst, src, n * sizeof(T), hipMemcpyDeviceToHost, c.stream));
+}
+void sync(const Context &c) {
+    HIP_CHECK(hipStreamSynchronize(c.stream));
+}
+} // namespace
+
+// ---------------------------------------------------------------- layout
+void Layout::build(const Context &c, std::int64_t n_, const std::vector<std::int64_t> &offs) {
+    n = n_;
+    offsets = offs;
+    nsys = (int)offs.size() - 1;
+    std::vector<int> ts, r0, r1, st0(nsys), st1(nsys), rows(nsys);
+    for (int s = 0; s < nsys; ++s) {
+        st0[s] = (int)ts.size();
+        rows[s] = (int)(offs[s + 1] - offs[s]);
+        for (std::int64_t r = offs[s]; r < offs[s + 1]; r += kTileRows) {
+            ts.push_back(s);
+            r0.push_back((int)r);
+            r1.push_back((int)std::min<std::int64_t>(r + kTileRows, offs[s + 1]));
+        }
+        st1[s] = (int)ts.size();
+    }
+    ntiles = (int)ts.size();
+    t_sys.resize(ntiles);
+    t_row0.resize(ntiles);
+    t_row1.resize(ntiles);
+    s_tile0.resize(nsys);
+    s_tile1.resize(nsys);
+    s_rows.resize(nsys);
+    upload(c, t_sys.data(), ts.data(), ntiles);
+    upload(c, t_row0.data(), r0.data(), ntiles);
+    upload(c, t_row1.data(), r1.data(), ntiles);
+    upload(c, s_tile0.data(), st0.data(), nsys);
+    upload(c, s_tile1.data(), st1.data(), nsys);
+    upload(c, s_rows.data(), rows.data(), nsys);
+    sync(c);
+}
+
+int choose_lanes(std::int64_t n, std::int64_t nnz) {
+    const double avg = n ? (double)nnz / (double)n : 0;
+    if (avg <= 10)
+        return 1;
+    if (avg <= 24)
+        return 4;
+    if (avg <= 64)
+        return 8;
+    return 32;
+}
+
+void DevCsr::ensure32(const Context &c) {
+    if (has32)
+        return;
+    val32.resize(nnz);
+    launch_convert<double, float>(c.stream, nnz, val.data(), val32.data());
+    has32 = true;
+}
+
+void spmv_any(Context &c, const DevCsr &a, const double *x, double *y, int lanes) {
+    launch_spmv<double, double, double, double>(c.stream, lanes ? lanes : a.lanes,
+                                                a.layout->tiles(), nullptr, 0, a.ptr.data(),
+                                                a.col.data(), a.val.data(), x, y);
+}
+
+// ---------------------------------------------------------------- SPAI
+namespace {
+
+__global__ void iota_kernel(int n, int *out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+        out[i] = i;
+}
+__global__ void fill_int_kernel(std::int64_t n, int *out, int v) {
+    const std::int64_t i = (std::int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+        out[i] = v;
+}
+// Append columns with status == kSpaiOverflow to list (order irrelevant).
+__global__ void collect_overflow_kernel(int nwork, const int *work, const int *status, int *list,
+                                        int *count) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nwork)
+        return;
+    const int k = work[i];
+    if (status[k] == kSpaiOverflow)
+        list[atomicAdd(count, 1)] = k;
+}
+// Reduce per-column outcomes: first failing column, totals.
+__global__ void column_summary_kernel(int n, const int *status, const int *fits, const int *growths,
+                                      const int *clamps, const int *q, int *first_fail,
+                                      unsigned long long *tot) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n)
+        return;
+    if (status[k] != kSpaiSuccess)
+        atomicMin(first_fail, k);
+    atomicAdd(tot + 0, (unsigned long long)fits[k]);
+    atomicAdd(tot + 1, (unsigned long long)growths[k]);
+    atomicAdd(tot + 2, (unsigned long long)clamps[k]);
+    atomicAdd(tot + 3, (unsigned long long)q[k]);
+}
+// Column pool pointers: per column (pass, offset) -> the pools' base pointers.
+struct PoolTable {
+    const int *j[8];
+    const void *c[8];
+};
+template <typename T>
+__global__ void p_count_pools_kernel(int n, const int *q, const int *pass,
+                                     const std::int64_t *off, PoolTable pt, const T *d, int *counts,
+                                     int *asm_fail) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n)
+        return;
+    const int *pj = pt.j[pass[k]] + off[k];
+    const T *pc = static_cast<const T *>(pt.c[pass[k]]) + off[k];
+    int cnt = 0;
+    for (int t = 0; t < q[k]; ++t) {
+        const T cf = pc[t];
+        const T v = cf * d[pj[t]];
+        if (!isfinite(v) || (cf != 0 && v == 0))
+            atomicMin(asm_fail, k);
+        cnt += v != 0;
+    }
+    counts[k + 1] = cnt;
+}
+template <typename T>
+__global__ void p_fill_pools_kernel(int n, const int *q, const int *pass, const std::int64_t *off,
+                                    PoolTable pt, const T *d, const int *pptr, int *pcol,
+                                    T *pval) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n)
+        return;
+    const int *pj = pt.j[pass[k]] + off[k];
+    const T *pc = static_cast<const T *>(pt.c[pass[k]]) + off[k];
+    int w = pptr[k];
+    for (int t = 0; t < q[k]; ++t) {
+        const T v = pc[t] * d[pj[t]];
+        if (v != 0) {
+            pcol[w] = pj[t];
+            pval[w] = v;
+            ++w;
+        }
+    }
+}
+template <typename T>
+__global__ void patterns_kernel(int n, const int *q, const int *pass, const std::int64_t *off,
+                                PoolTable pt, const std::int64_t *optr, int *oidx, double *ocoef) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n)
+        return;
+    const int *pj = pt.j[pass[k]] + off[k];
+    const T *pc = static_cast<const T *>(pt.c[pass[k]]) + off[k];
+    const std::int64_t o = optr[k];
+    for (int t = 0; t < q[k]; ++t) {
+        oidx[o + t] = pj[t];
+        ocoef[o + t] = (double)pc[t];
+    }
+}
+
+const char *spai_column_message(int status) {
+    switch (status) {
+    case kSpaiRankDeficient:
+    case kSpaiNumericalFailure:
+        return "Householder QR/triangular fit failed";
+    case kSpaiEmptyCandidates:
+        return "No candidate remains before convergence";
+    case kSpaiNoGrowth:
+        return "No acceptable growth permitted before convergence";
+    case kSpaiIterationLimit:
+        return "Requested accuracy not reached within growth/refit budget";
+    case kSpaiZeroDenominator:
+    case kSpaiInvalidScore:
+        return "Candidate scoring failed";
+    default:
+        return "Column construction failed";
+    }
+}
+
+template <typename T>
+void spai_build_typed(Context &c, const DevCsr &a, const T *aval, T *d, double eps,
+                      long long alpha, int beta, SpaiBuild &out) {
+    const int n = (int)a.n;
+    hipStream_t s = c.stream;
+    // ---- scaling and validity (src/spai.cpp:322-348)
+    DeviceArray<int> bad(2);
+    {
+        const int init[2] = {INT_MAX, INT_MAX};
+        upload(c, bad.data(), init, 2);
+    }
+    launch_scale_rows<T>(s, n, a.ptr.data(), a.col.data(), aval, d, bad.data());
+    int hb[2];
+    download(c, hb, bad.data(), 2);
+    sync(c);
+    if (hb[0] != INT_MAX) {
+        const int row = hb[0] / 4, kind = hb[0] % 4;
+        out.status = kSpaiInvalidInput;
+        out.message = kind == 1 ? "Zero diagonal at row " + std::to_string(row)
+                                : std::string("Reciprocal row scaling outside binary64 range");
+        return;
+    }
+    if (hb[1] != INT_MAX) {
+        out.status = kSpaiInvalidInput;
+        out.message = "Scaling changes nonzero structure/range";
+        return;
+    }
+    // ---- B structure = transpose of A's pattern (row i lists rows j with a_ji != 0)
+    DeviceArray<int> bptr(n + 1), bcol(a.nnz > 0 ? a.nnz : 1), cursor(n + 1);
+    DeviceArray<int> scan_tmp(scan_tmp_ints(n + 1));
+    HIP_CHECK(hipMemsetAsync(bptr.data(), 0, (n + 1) * sizeof(int), s));
+    launch_count_cols(s, a.nnz, a.col.data(), bptr.data());
+    scan_inclusive_int(s, bptr.data(), n + 1, scan_tmp.data());
+    HIP_CHECK(hipMemcpyAsync(cursor.data(), bptr.data(), (n + 1) * sizeof(int),
+                             hipMemcpyDeviceToDevice, s));
+    launch_transpose_fill<T>(s, n, a.ptr.data(), a.col.data(), (const T *)nullptr, bptr.data(),
+                             cursor.data(), bcol.data(), (T *)nullptr);
+    // ---- columns, in capacity passes
+    out.c_status.resize(n);
+    out.c_fits.resize(n);
+    out.c_growths.resize(n);
+    out.c_clamps.resize(n);
+    out.c_q.resize(n);
+    out.c_res.resize(n);
+    out.c_piv.resize(n);
+    out.c_off.resize(n);
+    out.c_pass.resize(n);
+    DeviceArray<int> work(n), next(n), counter(2);
+    hipLaunchKernelGGL(iota_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, s, n, work.data());
+    int nwork = n;
+    int mcap = 64, qcap = 16, ccap = 128;
+    hipDeviceProp_t prop;
+    HIP_CHECK(hipGetDeviceProperties(&prop, c.device));
+    const int max_waves = prop.multiProcessorCount * 8; // lanes in flight
+    DeviceBuffer dwork, iwork;
+    while (nwork > 0) {
+        if (out.passes >= 8)
+            throw std::runtime_error("SPAI column exceeds the largest capacity bucket");
+        const int pass = out.passes++;
+        const std::int64_t ld = spai_lane_doubles(mcap, qcap, ccap);
+        const std::int64_t li = spai_lane_ints(mcap, qcap, ccap);
+        // Lanes in flight: bounded by the device and by a 2 GiB work area.
+        const int threads = 256, waves_per_block = threads / kWave;
+        std::int64_t waves = std::min<std::int64_t>(max_waves, (nwork + kWave - 1) / kWave);
+        const std::int64_t per_wave = (ld * (std::int64_t)sizeof(T) + li * 4) * kWave;
+        waves = std::max<std::int64_t>(1, std::min<std::int64_t>(waves, (2LL << 30) / per_wave));
+        // Every launched lane may take a column, so the work area covers whole blocks.
+        const int blocks = (int)((waves + waves_per_block - 1) / waves_per_block);
+        waves = (std::int64_t)blocks * waves_per_block;
+        dwork.resize((std::size_t)(waves * ld * kWave * sizeof(T)));
+        iwork.resize((std::size_t)(waves * li * kWave * 4));
+        out.pool_j.emplace_back((std::size_t)nwork * qcap);
+        out.pool_c.emplace_back((std::size_t)nwork * qcap * sizeof(T));
+        HIP_CHECK(hipMemsetAsync(counter.data(), 0, 2 * sizeof(int), s));
+        SpaiColumnOut o{out.c_status.data(), out.c_fits.data(), out.c_growths.data(),
+                        out.c_clamps.data(), out.c_q.data(), out.c_res.data(),
+                        out.c_piv.data(), out.c_off.data(), out.c_pass.data(), pass};
+        launch_spai_columns<T>(s, blocks, threads, n, a.ptr.data(), a.col.data(), aval, d,
+                               bptr.data(), bcol.data(), (T)eps, alpha, beta, work.data(), nwork,
+                               counter.data(), mcap, qcap, ccap, dwork.as<T>(), iwork.as<int>(),
+                               o, out.pool_j.back().data(), out.pool_c.back().as<T>(), qcap);
+        hipLaunchKernelGGL(collect_overflow_kernel, dim3(ceil_div(nwork, 256)), dim3(256), 0, s,
+                           nwork, work.data(), out.c_status.data(), next.data(),
+                           counter.data() + 1);
+        int nover = 0;
+        download(c, &nover, counter.data() + 1, 1);
+        sync(c);
+        std::swap(work, next);
+        nwork = nover;
+        mcap *= 4;
+        qcap *= 3;
+        ccap *= 4;
+    }
+    // ---- summary and P assembly (src/spai.cpp:352-372)
+    DeviceArray<int> ff(2);
+    DeviceArray<unsigned long long> tot(4);
+    {
+        const int init[2] = {INT_MAX, INT_MAX};
+        upload(c, ff.data(), init, 2);
+        HIP_CHECK(hipMemsetAsync(tot.data(), 0, 4 * sizeof(unsigned long long), s));
+    }
+    hipLaunchKernelGGL(column_summary_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, s, n,
+                       out.c_status.data(), out.c_fits.data(), out.c_growths.data(),
+                       out.c_clamps.data(), out.c_q.data(), ff.data(), tot.data());
+    PoolTable pt{};
+    for (int p = 0; p < out.passes; ++p) {
+        pt.j[p] = out.pool_j[p].data();
+        pt.c[p] = out.pool_c[p].as<T>();
+    }
+    out.p.n = n;
+    out.p.ptr.resize(n + 1);
+    HIP_CHECK(hipMemsetAsync(out.p.ptr.data(), 0, sizeof(int), s));
+    hipLaunchKernelGGL(p_count_pools_kernel<T>, dim3(ceil_div(n, 256)), dim3(256), 0, s, n,
+                       out.c_q.data(), out.c_pass.data(), out.c_off.data(), pt, (const T *)d,
+                       out.p.ptr.data(), ff.data() + 1);
+    scan_inclusive_int(s, out.p.ptr.data(), n + 1, scan_tmp.data());
+    int hff[2];
+    unsigned long long htot[4];
+    int pnnz = 0;
+    download(c, hff, ff.data(), 2);
+    download(c, htot, tot.data(), 4);
+    download(c, &pnnz, out.p.ptr.data() + n, 1);
+    sync(c);
+    out.fits = (std::int64_t)htot[0];
+    out.growths = (std::int64_t)htot[1];
+    out.clamps = (std::int64_t)htot[2];
+    out.pattern_nnz = (std::int64_t)htot[3];
+    out.p_nnz = pnnz;
+    out.p.nnz = pnnz;
+    out.p.col.resize(pnnz > 0 ? pnnz : 1);
+    if (sizeof(T) == 8) {
+        out.p.val.resize(pnnz > 0 ? pnnz : 1);
+        out.p.has64 = true;
+        out.p.has32 = false;
+    } else {
+        out.p.val32.resize(pnnz > 0 ? pnnz : 1);
+        out.p.has32 = true;
+        out.p.has64 = false;
+    }
+    T *pval = sizeof(T) == 8 ? (T *)(void *)out.p.val.data() : (T *)(void *)out.p.val32.data();
+    hipLaunchKernelGGL(p_fill_pools_kernel<T>, dim3(ceil_div(n, 256)), dim3(256), 0, s, n,
+                       out.c_q.data(), out.c_pass.data(), out.c_off.data(), pt, (const T *)d,
+                       out.p.ptr.data(), out.p.col.data(), pval);
+    out.p.lanes = choose_lanes(n, pnnz);
+    out.p.layout = a.layout;
+    if (hff[1] != INT_MAX) {
+        out.status = kSpaiNumericalFailure;
+        out.failing_column = hff[1];
+        out.message = "Preconditioner assembly outside binary64 range";
+    } else if (hff[0] != INT_MAX) {
+        int st = 0;
+        download(c, &st, out.c_status.data() + hff[0], 1);
+        sync(c);
+        out.status = st;
+        out.failing_column = hff[0];
+        out.message = "Column " + std::to_string(hff[0]) + ": " + spai_column_message(st);
+    } else {
+        out.status = kSpaiSuccess;
+    }
+    sync(c);
+}
+
+} // namespace
+
+std::unique_ptr<SpaiBuild> spai_build(Context &c, const DevCsr &a, double eps, long long alpha,
+                                      int beta, int precision) {
+    auto out = std::make_unique<SpaiBuild>();
+    out->n = a.n;
+    out->precision = precision;
+    out->status = kSpaiInvalidInput;
+    if (a.n == 0 || !std::isfinite(eps) || eps <= 0) {
+        out->message = "Require positive square dimensions and positive finite epsilon";
+        return out;
+    }
+    HIP_CHECK(hipEventRecord(c.ev0, c.stream));
+    if (precision == 0) {
+        out->d64.resize(a.n);
+        spai_build_typed<double>(c, a, a.val.data(), out->d64.data(), eps, alpha, beta, *out);
+    } else {
+        const_cast<DevCsr &>(a).ensure32(c);
+        out->d32.resize(a.n);
+        spai_build_typed<float>(c, a, a.val32.data(), out->d32.data(), (float)eps, alpha, beta,
+                                *out);
+    }
+    HIP_CHECK(hipEventRecord(c.ev1, c.stream));
+    HIP_CHECK(hipEventSynchronize(c.ev1));
+    float ms = 0;
+    HIP_CHECK(hipEventElapsedTime(&ms, c.ev0, c.ev1));
+    out->build_ms = ms;
+    return out;
+}
+
+// Gather per-column patterns into compact arrays (host ptr computed from q).
+void spai_patterns(Context &c, const SpaiBuild &b, std::int64_t *hptr, int *hidx,
+                   double *hcoef) {
+    const int n = (int)b.n;
+    std::vector<int> q(n);
+    download(c, q.data(), b.c_q.data(), n);
+    sync(c);
+    hptr[0] = 0;
+    for (int k = 0; k < n; ++k)
+        hptr[k + 1] = hptr[k] + q[k];
+    const std::int64_t tot = hptr[n];
+    DeviceArray<std::int64_t> optr(n + 1);
+    DeviceArray<int> oidx(tot > 0 ? tot : 1);
+    DeviceArray<double> ocoef(tot > 0 ? tot : 1);
+    upload(c, optr.data(), hptr, n + 1);
+    PoolTable pt{};
+    for (int p = 0; p < b.passes; ++p) {
+        pt.j[p] = b.pool_j[p].data();
+        pt.c[p] = b.pool_c[p].as<void>();
+    }
+    if (b.precision == 0)
+        hipLaunchKernelGGL(patterns_kernel<double>, dim3(ceil_div(n, 256)), dim3(256), 0,
+                           c.stream, n, b.c_q.data(), b.c_pass.data(), b.c_off.data(), pt,
+                           optr.data(), oidx.data(), ocoef.data());
+    else
+        hipLaunchKernelGGL(patterns_kernel<float>, dim3(ceil_div(n, 256)), dim3(256), 0,
+                           c.stream, n, b.c_q.data(), b.c_pass.data(), b.c_off.data(), pt,
+                           optr.data(), oidx.data(), ocoef.data());
+    HIP_LAUNCH_CHECK();
+    download(c, hidx, oidx.data(), tot);
+    download(c, hcoef, ocoef.data(), tot);
+    sync(c);
+}
+
+// ---------------------------------------------------------------- GMRES
+template <typename T>
+GmresEngine<T>::GmresEngine(Context &c, const Layout &L, int cap) : c_(c), L_(L), cap_(cap) {
+    const std::int64_t n = L.n;
+    w_.resize(n);
+    t_.resize(n);
+    prhs_.resize(n);
+    r0_.resize(n);
+    xz_.resize(n);
+    ax_.resize(n);
+    pax_.resize(n);
+    const std::int64_t hs = hessenberg_size(cap);
+    H_.resize((std::size_t)hs * L.nsys);
+    cs_.resize((std::size_t)cap * L.nsys);
+    sn_.resize((std::size_t)cap * L.nsys);
+    g_.resize((std::size_t)(cap + 1) * L.nsys);
+    y_.resize((std::size_t)cap * L.nsys);
+    part_.resize(2 * (std::size_t)std::max(1, L.ntiles));
+    cnt_.resize(L.nsys);
+    HIP_CHECK(hipMemsetAsync(cnt_.data(), 0, L.nsys * sizeof(int), c.stream));
+    st_.resize(L.nsys);
+    vdev_.resize(cap + 1);
+    vptrs_.reserve(cap + 1);
+    args_.st = st_.data();
+    args_.V = vdev_.data();
+    args_.w = w_.data();
+    args_.H = H_.data();
+    args_.cs = cs_.data();
+    args_.sn = sn_.data();
+    args_.g = g_.data();
+    args_.y = y_.data();
+    args_.hstride = hs;
+    args_.cap = cap;
+    args_.part = part_.data();
+    args_.cnt = cnt_.data();
+}
+
+template <typename T> void GmresEngine<T>::ensure_basis(int count) {
+    // Basis vectors are allocated in chunks as the Krylov space grows.
+    const int chunk = 16;
+    if (count <= vcount_)
+        return;
+    while ((int)vptrs_.size() < count) {
+        const int take = std::min(chunk, cap_ + 1 - (int)vptrs_.size());
+        chunks_.emplace_back((std::size_t)take * L_.n);
+        for (int i = 0; i < take; ++i)
+            vptrs_.push_back(chunks_.back().data() + (std::size_t)i * L_.n);
+    }
+    upload(c_, vdev_.data(), vptrs_.data(), vptrs_.size());
+    vcount_ = (int)vptrs_.size();
+}
+
+template <typename T>
+void GmresEngine<T>::solve(const DevCsr &m1, const DevCsr &m2, GmresMode mode, const T *rhs,
+                           const double *x0, double tol, double *x, const IrSysState *ir) {
+    hipStream_t s = c_.stream;
+    const Tiles tl = L_.tiles();
+    args_.x = mode == kLeft ? x : xz_.data();
+    args_.x0 = mode == kLeft ? x0 : nullptr;
+    ensure_basis(std::min(cap_ + 1, 2));
+    gmres_reset<T>(s, L_.nsys, args_, ir, L_.s_rows.data(), tol);
+    const int *active =
+        reinterpret_cast<const int *>(reinterpret_cast<const char *>(st_.data()) +
+                                      offsetof(SysState, active));
+    const int *trig = reinterpret_cast<const int *>(reinterpret_cast<const char *>(st_.data()) +
+                                                    offsetof(SysState, triggered));
+    const int stride = (int)(sizeof(SysState) / sizeof(int));
+    const T *m1v = m1.values<T>(), *m2v = m2.values<T>();
+    // ---- start (src/gmres.cpp:134-167)
+    if (mode == kLeft && x0) {
+        gmres_diff_norm<T, T, T>(s, tl, args_, rhs, (const T *)nullptr, nullptr, kNormRhs);
+        launch_spmv<T, double, double, double>(s, m1.lanes, tl, active, stride, m1.ptr.data(),
+                                               m1.col.data(), m1v, x0, ax_.data());
+        gmres_diff_norm<T, double, T>(s, tl, args_, rhs, ax_.data(), nullptr, kNormInitOrig);
+    } else {
+        // x0 = 0: ||rhs - A x0|| = ||rhs||, zero means zero_rhs.
+        gmres_diff_norm<T, T, T>(s, tl, args_, rhs, (const T *)nullptr, nullptr, kNormRhs);
+        gmres_diff_norm<T, T, T>(s, tl, args_, rhs, (const T *)nullptr, nullptr, kNormInitOrig);
+    }
+    const T *prhs = rhs;
+    if (mode == kLeft) {
+        launch_spmv<T, T, T, T>(s, m2.lanes, tl, active, stride, m2.ptr.data(), m2.col.data(),
+                                m2v, rhs, prhs_.data());
+        prhs = prhs_.data();
+    }
+    gmres_diff_norm<T, T, T>(s, tl, args_, prhs, (const T *)nullptr, nullptr, kNormDen);
+    const T *r0 = prhs;
+    if (mode == kLeft && x0) {
+        launch_spmv<T, double, double, double>(s, m2.lanes, tl, active, stride, m2.ptr.data(),
+                                               m2.col.data(), m2v, ax_.data(), pax_.data());
+        gmres_diff_norm<T, double, T>(s, tl, args_, prhs, pax_.data(), xz_.data(), kNormBeta);
+        launch_convert<double, T>(s, L_.n, xz_.data(), r0_.data());
+        r0 = r0_.data();
+    } else {
+        gmres_diff_norm<T, T, T>(s, tl, args_, prhs, (const T *)nullptr, nullptr, kNormBeta);
+    }
+    gmres_start_vector<T>(s, tl, args_, r0);
+    // ---- Arnoldi loop (src/gmres.cpp:172-254)
+    std::vector<SysState> host(L_.nsys);
+    for (int k = 0; k < cap_; ++k) {
+        ensure_basis(std::min(cap_ + 1, k + 2));
+        launch_spmv<T, T, T, T>(s, m1.lanes, tl, active, stride, m1.ptr.data(), m1.col.data(),
+                                m1v, vptrs_[k], t_.data());
+        launch_spmv<T, T, T, T>(s, m2.lanes, tl, active, stride, m2.ptr.data(), m2.col.data(),
+                                m2v, t_.data(), w_.data());
+        for (int j = 0; j <= k + 1; ++j)
+            gmres_mgs_round<T>(s, tl, args_, j, k);
+        if (k + 1 <= cap_)
+            gmres_normalize<T>(s, tl, args_, k);
+        gmres_candidate<T>(s, tl, args_);
+        if (mode == kLeft) {
+            launch_spmv<T, double, double, double>(s, m1.lanes, tl, trig, stride, m1.ptr.data(),
+                                                   m1.col.data(), m1v, x, ax_.data());
+            gmres_diff_norm<T, double, T>(s, tl, args_, rhs, ax_.data(), nullptr, kNormCheckOrig);
+            launch_spmv<T, double, double, double>(s, m2.lanes, tl, trig, stride, m2.ptr.data(),
+                                                   m2.col.data(), m2v, ax_.data(), pax_.data());
+            gmres_diff_norm<T, double, T>(s, tl, args_, prhs, pax_.data(), nullptr,
+                                          kNormCheckRel);
+        } else {
+            // x = P^T z, then ||g - A^T x|| (absolute and relative to ||g||).
+            launch_spmv<T, double, double, double>(s, m1.lanes, tl, trig, stride, m1.ptr.data(),
+                                                   m1.col.data(), m1v, xz_.data(), x);
+            launch_spmv<T, double, double, double>(s, m2.lanes, tl, trig, stride, m2.ptr.data(),
+                                                   m2.col.data(), m2v, x, ax_.data());
+            gmres_diff_norm<T, double, T>(s, tl, args_, rhs, ax_.data(), nullptr, kNormCheckOrig);
+            gmres_diff_norm<T, double, T>(s, tl, args_, rhs, ax_.data(), nullptr, kNormCheckRel);
+        }
+        if ((k + 1) % sync_every == 0 || k + 1 == cap_) {
+            download(c_, host.data(), st_.data(), L_.nsys);
+            sync(c_);
+            bool any = false;
+            for (const SysState &h : host)
+                any = any || h.active;
+            if (!any)
+                break;
+        }
+    }
+    if (mode == kRight) {
+        // Systems that ended at x0 = 0 (zero rhs, or init_rel <= tau).
+        args_.x = x;
+    }
+    gmres_finish<T>(s, tl, args_);
+    args_.x = mode == kLeft ? x : xz_.data();
+}
+
+template <typename T> std::vector<GmresOutcome> GmresEngine<T>::outcomes() {
+    std::vector<SysState> host(L_.nsys);
+    download(c_, host.data(), st_.data(), L_.nsys);
+    sync(c_);
+    std::vector<GmresOutcome> out(L_.nsys);
+    for (int i = 0; i < L_.nsys; ++i) {
+        const SysState &h = host[i];
+        out[i] = GmresOutcome{h.status,     h.converged,  h.iterations, h.init_orig,
+                              h.init_rel,   h.final_orig, h.final_rel};
+        if (h.active) { // cap reached without an accept (loop ended)
+            out[i].status = h.iterations < h.nrows ? kGmIterationLimit : kGmNonconvergence;
+        }
+    }
+    return out;
+}
+
+template class GmresEngine<double>;
+template class GmresEngine<float>;
+
+// ---------------------------------------------------------------- IR
+namespace {
+__global__ void ir_reset_kernel(int nsys, IrSysState *st) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nsys)
+        return;
+    IrSysState z = {};
+    z.active = 1;
+    z.dscale = 1.0;
+    z.pending_dscale = 1.0;
+    z.status = kIrIterationLimit;
+    st[s] = z;
+}
+__global__ void ir_cap_kernel(int nsys, IrSysState *st) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nsys)
+        return;
+    if (st[s].active) {
+        st[s].status = kIrIterationLimit;
+        st[s].active = 0;
+        st[s].done = 1;
+    }
+}
+} // namespace
+
+std::vector<IrResultHost> ir_run(Context &c, const DevCsr &a64, const DevCsr &m1, const DevCsr &m2,
+                                 GmresMode mode, const double *b, const IrOptionsC &o, double *x) {
+    hipStream_t s = c.stream;
+    const Layout &L = *a64.layout;
+    const Tiles tl = L.tiles();
+    const int nsys = L.nsys;
+    DeviceArray<IrSysState> st(nsys);
+    DeviceArray<double> part(4 * std::max(1, L.ntiles)), part2(2 * std::max(1, L.ntiles)),
+        part3(2 * std::max(1, L.ntiles));
+    DeviceArray<int> cnt(nsys);
+    DeviceArray<double> hist((std::size_t)nsys * kMaxOuterHistory * 4);
+    DeviceArray<int> inner((std::size_t)nsys * kMaxOuterHistory * 3);
+    DeviceArray<double> r(L.n), d(L.n);
+    HIP_CHECK(hipMemsetAsync(cnt.data(), 0, nsys * sizeof(int), s));
+    HIP_CHECK(hipMemsetAsync(hist.data(), 0, hist.size() * sizeof(double), s));
+    HIP_CHECK(hipMemsetAsync(inner.data(), 0, inner.size() * sizeof(int), s));
+    hipLaunchKernelGGL(ir_reset_kernel, dim3(ceil_div(nsys, 64)), dim3(64), 0, s, nsys, st.data());
+    IrArgs a{st.data(), part.data(), part2.data(), part3.data(), cnt.data(), hist.data(),
+             inner.data(), 10.0 * std::ldexp(1.0, -53), o.relres_target};
+    ir_ainf(s, tl, a64.ptr.data(), a64.val.data(), a);
+    ir_bnorm(s, tl, b, a);
+    // x0 = M b with M the preconditioner (src/ir.cpp:149)
+    const DevCsr &pre = mode == kLeft ? m2 : m1;
+    if (o.precision == 0)
+        launch_spmv<double, double, double, double>(s, pre.lanes, tl, nullptr, 0, pre.ptr.data(),
+                                                    pre.col.data(), pre.val.data(), b, x);
+    else
+        launch_spmv<float, double, double, double>(s, pre.lanes, tl, nullptr, 0, pre.ptr.data(),
+                                                   pre.col.data(), pre.val32.data(), b, x);
+    ir_diagnostics(s, tl, a64.ptr.data(), a64.col.data(), a64.val.data(), b, x, r.data(), a, 0);
+    const int cap = o.cap > 0 ? o.cap : (int)std::min<std::int64_t>(L.n, INT_MAX - 2);
+    // Largest system bounds the Krylov dimension of every system.
+    std::int64_t maxrows = 0;
+    for (int i = 0; i < nsys; ++i)
+        maxrows = std::max(maxrows, L.offsets[i + 1] - L.offsets[i]);
+    const int kcap = (int)std::min<std::int64_t>(cap, maxrows);
+    std::unique_ptr<GmresEngine<double>> g64;
+    std::unique_ptr<GmresEngine<float>> g32;
+    DeviceArray<float> rf;
+    if (o.precision == 0)
+        g64 = std::make_unique<GmresEngine<double>>(c, L, kcap);
+    else {
+        g32 = std::make_unique<GmresEngine<float>>(c, L, kcap);
+        rf.resize(L.n);
+    }
+    std::vector<IrSysState> host(nsys);
+    auto any_active = [&]() {
+        download(c, host.data(), st.data(), nsys);
+        sync(c);
+        for (const auto &h : host)
+            if (h.active)
+                return true;
+        return false;
+    };
+    bool running = any_active();
+    for (int step = 0; step < o.outer && running; ++step) {
+        if (o.precision == 0) {
+            g64->solve(m1, m2, mode, r.data(), nullptr, o.tol, d.data(), st.data());
+            ir_absorb_inner<double>(s, nsys, g64->args(), a);
+        } else {
+            ir_scale_rhs(s, tl, r.data(), rf.data(), a);
+            g32->solve(m1, m2, mode, rf.data(), nullptr, o.tol, d.data(), st.data());
+            ir_absorb_inner<float>(s, nsys, g32->args(), a);
+        }
+        ir_update(s, tl, x, d.data(), a);
+        ir_diagnostics(s, tl, a64.ptr.data(), a64.col.data(), a64.val.data(), b, x, r.data(), a,
+                       step + 1);
+        running = any_active();
+    }
+    hipLaunchKernelGGL(ir_cap_kernel, dim3(ceil_div(nsys, 64)), dim3(64), 0, s, nsys, st.data());
+    std::vector<IrResultHost> out(nsys);
+    std::vector<double> hh(hist.size());
+    std::vector<int> hi(inner.size());
+    download(c, host.data(), st.data(), nsys);
+    download(c, hh.data(), hist.data(), hh.size());
+    download(c, hi.data(), inner.data(), hi.size());
+    sync(c);
+    for (int i = 0; i < nsys; ++i) {
+        out[i].st = host[i];
+        std::copy(hh.begin() + (std::size_t)i * kMaxOuterHistory * 4,
+                  hh.begin() + (std::size_t)(i + 1) * kMaxOuterHistory * 4, &out[i].history[0][0]);
+        std::copy(hi.begin() + (std::size_t)i * kMaxOuterHistory * 3,
+                  hi.begin() + (std::size_t)(i + 1) * kMaxOuterHistory * 3, &out[i].inner[0][0]);
+    }
+    return out;
+}
+
+} // namespace bspai_hip

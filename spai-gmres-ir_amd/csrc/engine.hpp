@@ -1,0 +1,204 @@
+// Host-side engine: device buffers, matrices, SPAI / GMRES / IR drivers.
+#pragma once
+
+#include "common.hpp"
+#include "kernels.hpp"
+#include "krylov.hpp"
+
+#include <memory>
+#include <string>
+#include <vector>
+
+namespace bspai_hip {
+
+// ---------------------------------------------------------------- buffers
+class DeviceBuffer {
+  public:
+    DeviceBuffer() = default;
+    explicit DeviceBuffer(std::size_t bytes) { resize(bytes); }
+    ~DeviceBuffer() { release(); }
+    DeviceBuffer(const DeviceBuffer &) = delete;
+    DeviceBuffer &operator=(const DeviceBuffer &) = delete;
+    DeviceBuffer(DeviceBuffer &&o) noexcept : p_(o.p_), n_(o.n_) { o.p_ = nullptr, o.n_ = 0; }
+    DeviceBuffer &operator=(DeviceBuffer &&o) noexcept {
+        if (this != &o) {
+            release();
+            p_ = o.p_;
+            n_ = o.n_;
+            o.p_ = nullptr;
+            o.n_ = 0;
+        }
+        return *this;
+    }
+    void resize(std::size_t bytes) {
+        if (bytes <= n_)
+            return;
+        release();
+        if (bytes) {
+            const hipError_t e = hipMalloc(&p_, bytes);
+            if (e != hipSuccess) {
+                p_ = nullptr;
+                throw HipError("hipMalloc of " + std::to_string(bytes) +
+                               " bytes failed: " + hipGetErrorString(e));
+            }
+            n_ = bytes;
+        }
+    }
+    void release() {
+        if (p_)
+            (void)hipFree(p_);
+        p_ = nullptr;
+        n_ = 0;
+    }
+    template <typename T> T *as() const { return static_cast<T *>(p_); }
+    std::size_t bytes() const { return n_; }
+
+  private:
+    void *p_ = nullptr;
+    std::size_t n_ = 0;
+};
+
+template <typename T> class DeviceArray {
+  public:
+    DeviceArray() = default;
+    explicit DeviceArray(std::size_t n) { resize(n); }
+    void resize(std::size_t n) {
+        buf_.resize(n * sizeof(T));
+        n_ = n;
+    }
+    T *data() const { return buf_.as<T>(); }
+    std::size_t size() const { return n_; }
+
+  private:
+    DeviceBuffer buf_;
+    std::size_t n_ = 0;
+};
+
+struct Context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    std::string last_error;
+    // timing events
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
+// Block-diagonal system layout and its row tiles.
+struct Layout {
+    int nsys = 1;
+    std::int64_t n = 0;
+    std::vector<std::int64_t> offsets; // nsys+1
+    DeviceArray<int> t_sys, t_row0, t_row1, s_tile0, s_tile1, s_rows;
+    int ntiles = 0;
+    Tiles tiles() const {
+        return Tiles{ntiles, nsys, t_sys.data(), t_row0.data(), t_row1.data(), s_tile0.data(),
+                     s_tile1.data()};
+    }
+    void build(const Context &c, std::int64_t n_, const std::vector<std::int64_t> &offs);
+};
+
+// Device CSR with FP64 values (and an FP32 copy made on demand).
+struct DevCsr {
+    std::int64_t n = 0, nnz = 0;
+    DeviceArray<int> ptr, col;
+    DeviceArray<double> val;
+    DeviceArray<float> val32;
+    bool has32 = false, has64 = true;
+    int lanes = 1; // SpMV lanes per row
+    std::shared_ptr<Layout> layout;
+    void ensure32(const Context &c);
+    template <typename T> const T *values() const;
+};
+template <> inline const double *DevCsr::values<double>() const { return val.data(); }
+template <> inline const float *DevCsr::values<float>() const { return val32.data(); }
+
+int choose_lanes(std::int64_t n, std::int64_t nnz);
+
+struct SpaiBuild {
+    std::int64_t n = 0;
+    int precision = 0;
+    int status = 0;
+    std::int64_t failing_column = -1;
+    std::string message;
+    DevCsr p; // values in FP64 (plain) or FP32 (mixed, val32)
+    DeviceArray<double> d64;
+    DeviceArray<float> d32;
+    DeviceArray<int> c_status, c_fits, c_growths, c_clamps, c_q;
+    DeviceArray<double> c_res, c_piv;
+    DeviceArray<std::int64_t> c_off;   // per column: offset into its pass pool
+    DeviceArray<int> c_pass;           // per column: pass id
+    std::vector<DeviceArray<int>> pool_j;
+    std::vector<DeviceBuffer> pool_c;
+    std::int64_t p_nnz = 0, pattern_nnz = 0, fits = 0, growths = 0, clamps = 0;
+    int passes = 0;
+    double build_ms = 0;
+};
+
+std::unique_ptr<SpaiBuild> spai_build(Context &c, const DevCsr &a, double eps, long long alpha,
+                                      int beta, int precision);
+
+struct GmresOutcome {
+    int status, converged, iterations;
+    double init_orig, init_rel, final_orig, final_rel;
+};
+
+// GMRES modes: left-preconditioned P(A x) = P r (forward) or right-
+// preconditioned A^T (P^T u) = g, x = P^T u (adjoint).
+enum GmresMode { kLeft = 0, kRight = 1 };
+
+template <typename T> class GmresEngine {
+  public:
+    GmresEngine(Context &c, const Layout &L, int cap);
+    // m1 then m2 applied per Arnoldi step: left (A, P), right (P^T, A^T).
+    // rhs in T; x (FP64) receives the solution; ir (optional) masks systems.
+    void solve(const DevCsr &m1, const DevCsr &m2, GmresMode mode, const T *rhs, const double *x0,
+               double tol, double *x, const IrSysState *ir);
+    std::vector<GmresOutcome> outcomes();
+    KrylovArgs<T> args() const { return args_; }
+    int cap() const { return cap_; }
+    int sync_every = 4;
+
+  private:
+    void ensure_basis(int count);
+    Context &c_;
+    const Layout &L_;
+    int cap_;
+    std::vector<DeviceArray<T>> chunks_;
+    std::vector<T *> vptrs_;
+    DeviceArray<T *> vdev_;
+    int vcount_ = 0;
+    DeviceArray<T> w_, t_, prhs_, r0_;
+    DeviceArray<double> xz_, ax_, pax_, ax0_, tmpd_;
+    DeviceArray<double> H_, cs_, sn_, g_, y_, part_;
+    DeviceArray<int> cnt_;
+    DeviceArray<SysState> st_;
+    KrylovArgs<T> args_{};
+};
+
+struct IrResultHost {
+    IrSysState st;
+    double history[kMaxOuterHistory][4];
+    int inner[kMaxOuterHistory][3];
+};
+
+struct IrOptionsC {
+    double tol = 1e-8;
+    int cap = 0;
+    int outer = 20;
+    int precision = 0;
+    double relres_target = 0;
+};
+
+// Iterative refinement on a (batched) system; a64 is the FP64 matrix for the
+// residual, m1/m2 the inner operators (FP64 or FP32 values per precision).
+std::vector<IrResultHost> ir_run(Context &c, const DevCsr &a64, const DevCsr &m1, const DevCsr &m2,
+                                 GmresMode mode, const double *b, const IrOptionsC &o, double *x);
+
+void spmv_any(Context &c, const DevCsr &a, const double *x, double *y, int lanes);
+void spai_patterns(Context &c, const SpaiBuild &b, std::int64_t *hptr, int *hidx, double *hcoef);
+void launch_sddmm(hipStream_t s, const Tiles &tiles, const int *ptr, const int *col, double alpha,
+                  const double *u, const double *v, double *out);
+void spai_score_on_device(hipStream_t s, double rr, double dot, double den, int *status,
+                          int *clamped, double *score);
+
+} // namespace bspai_hip

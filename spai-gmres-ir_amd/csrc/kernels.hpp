@@ -1,0 +1,87 @@
+// Launch wrappers of the HIP kernels (internal interface of the library).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace bspai_hip {
+
+// SPAI column statuses; values mirror bspai::SpaiStatus (reference
+// include/bspai/spai.hpp:10-21) plus an internal capacity code.
+enum : int {
+    kSpaiSuccess = 0,
+    kSpaiNotAttempted = 1,
+    kSpaiInvalidInput = 2,
+    kSpaiRankDeficient = 3,
+    kSpaiNumericalFailure = 4,
+    kSpaiEmptyCandidates = 5,
+    kSpaiZeroDenominator = 6,
+    kSpaiInvalidScore = 7,
+    kSpaiNoGrowth = 8,
+    kSpaiIterationLimit = 9,
+    kSpaiOverflow = 100, // work arrays too small: re-run with larger caps
+};
+
+// Per-column results of a construction pass; column k's final pattern J and
+// coefficients sit at pool[pass[k]] + off[k], q[k] entries.
+struct SpaiColumnOut {
+    int *status, *fits, *growths, *clamps, *q;
+    double *residual, *pivot;
+    std::int64_t *off;
+    int *pass;
+    int pass_id;
+};
+
+std::int64_t spai_lane_doubles(int mcap, int qcap, int ccap);
+std::int64_t spai_lane_ints(int mcap, int qcap, int ccap);
+
+template <typename T>
+void launch_spai_columns(hipStream_t s, int blocks, int threads, int n, const int *aptr,
+                         const int *acol, const T *aval, const T *dscale, const int *bptr,
+                         const int *bcol, T eps, long long alpha, int beta, const int *worklist,
+                         int nwork, int *cursor, int mcap, int qcap, int ccap, T *dwork,
+                         int *iwork, SpaiColumnOut out, int *out_j, T *out_c,
+                         std::int64_t out_stride);
+template <typename T>
+void launch_scale_rows(hipStream_t s, int n, const int *ptr, const int *col, const T *val, T *d,
+                       int *first_bad);
+void launch_count_cols(hipStream_t s, std::int64_t nnz, const int *col, int *counts);
+template <typename T>
+void launch_transpose_fill(hipStream_t s, int n, const int *ptr, const int *col, const T *val,
+                           const int *tptr, int *cursor, int *tcol, T *tval);
+
+// exclusive/inclusive scan helpers (int32), in place on counts[0..n]:
+// counts[0] must be 0; after the call counts[i] = sum of counts[<i] inputs.
+void scan_inclusive_int(hipStream_t s, int *data, std::int64_t n, int *tmp);
+std::int64_t scan_tmp_ints(std::int64_t n);
+
+// ------------------------------------------------------------------ tiles
+// Row tiles: every vector kernel runs one block per tile; a tile lies inside
+// one system.  sys[t] = system of tile t, rows [row0[t], row1[t]); the tiles
+// of system s are [sys_tile0[s], sys_tile1[s]).
+struct Tiles {
+    int ntiles, nsys;
+    const int *sys;
+    const int *row0;
+    const int *row1;
+    const int *sys_tile0;
+    const int *sys_tile1;
+};
+constexpr int kTileRows = 2048;
+constexpr int kTileThreads = 256;
+constexpr int kTilePer = kTileRows / kTileThreads;
+
+// ------------------------------------------------------------------ SpMV
+// y[r] = sum_q val[q] * x[col[q]] over row r, accumulated in TA.  lanes = the
+// number of lanes cooperating on one row (1 = sequential CSR order, which is
+// bit-identical to the reference's CsrMatrix::multiply, src/cpu.cpp:80-94).
+// flag (optional): a tile runs only if flag[sys * flag_stride] != 0.
+template <typename TV, typename TX, typename TA, typename TY>
+void launch_spmv(hipStream_t s, int lanes, const Tiles &tiles, const int *flag, int flag_stride,
+                 const int *ptr, const int *col, const TV *val, const TX *x, TY *y);
+
+template <typename TS, typename TD>
+void launch_convert(hipStream_t s, std::int64_t n, const TS *src, TD *dst);
+
+} // namespace bspai_hip

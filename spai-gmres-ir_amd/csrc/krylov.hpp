@@ -1,0 +1,114 @@
+// Device state and launchers of the GMRES / IR kernels (internal interface).
+#pragma once
+
+#include "kernels.hpp"
+
+namespace bspai_hip {
+
+// GMRES statuses mirror bspai::GmresStatus (reference include/bspai/gmres.hpp:10-20).
+enum : int {
+    kGmSuccess = 0,
+    kGmZeroRhs = 1,
+    kGmExactInitial = 2,
+    kGmInvalidInput = 3,
+    kGmZeroPreconditionedRhs = 4,
+    kGmBreakdown = 5,
+    kGmIterationLimit = 6,
+    kGmNonconvergence = 7,
+    kGmNumericalFailure = 8,
+};
+// IR statuses mirror bspai::IrStatus (reference include/bspai/ir.hpp:9-17).
+enum : int {
+    kIrSuccess = 0,
+    kIrInvalidInput = 1,
+    kIrConstructionFailure = 2,
+    kIrInnerFailure = 3,
+    kIrStagnation = 4,
+    kIrIterationLimit = 5,
+    kIrNumericalFailure = 6,
+};
+enum : int {
+    kNormInitOrig = 0,
+    kNormRhs = 1,
+    kNormDen = 2,
+    kNormBeta = 3,
+    kNormCheckOrig = 4,
+    kNormCheckRel = 5,
+};
+
+constexpr int kMaxOuterHistory = 24; // >= outer cap 20 + initial
+
+// Column k of the rotated Hessenberg holds rows 0..k+1, packed.
+__host__ __device__ inline std::int64_t hcol_offset(int k) {
+    return (std::int64_t)k * (k + 3) / 2;
+}
+inline std::int64_t hessenberg_size(int cap) {
+    return hcol_offset(cap) + 2;
+}
+
+struct SysState {
+    double tol;
+    double den, beta, hnorm, est;
+    double init_orig, init_rel, final_orig, final_rel;
+    int nrows;
+    int participating, active, status, converged, iterations;
+    int triggered, ccount, zero_pivot, breakdown;
+    int rhs_zero, x0_is_solution, has_solution;
+};
+
+template <typename T> struct KrylovArgs {
+    SysState *st;
+    T *const *V; // device array of basis-vector pointers (cap+1)
+    T *w;
+    double *H, *cs, *sn, *g, *y;
+    std::int64_t hstride;
+    int cap;
+    double *part; // 2 per tile
+    int *cnt;     // per system
+    double *x;    // candidate / solution (FP64)
+    const double *x0;
+};
+
+struct IrSysState {
+    double ainf, bnorm, rinf, r2, nbe, cbe, xinf;
+    double dscale, pending_dscale;
+    int active, done, status, converged, outer_steps, inner_attempts, inner_status;
+    int history_len, unchanged;
+};
+
+struct IrArgs {
+    IrSysState *st;
+    double *part;  // 4 per tile
+    double *part2; // 2 per tile
+    double *part3; // 2 per tile (norm partials)
+    int *cnt;
+    double *history; // [nsys][kMaxOuterHistory][4]: rinf, r2, nbe, cbe
+    int *inner;      // [nsys][kMaxOuterHistory][3]: status, converged, iterations
+    double be_tol, relres_target;
+};
+
+template <typename T>
+void gmres_mgs_round(hipStream_t s, const Tiles &tl, const KrylovArgs<T> &a, int j, int k);
+template <typename T> void gmres_normalize(hipStream_t s, const Tiles &tl, const KrylovArgs<T> &a, int k);
+template <typename T> void gmres_candidate(hipStream_t s, const Tiles &tl, const KrylovArgs<T> &a);
+template <typename T>
+void gmres_start_vector(hipStream_t s, const Tiles &tl, const KrylovArgs<T> &a, const T *r0);
+template <typename T> void gmres_finish(hipStream_t s, const Tiles &tl, const KrylovArgs<T> &a);
+template <typename TU, typename TV, typename T>
+void gmres_diff_norm(hipStream_t s, const Tiles &tl, const KrylovArgs<T> &a, const TU *u,
+                     const TV *v, double *out, int what);
+template <typename T>
+void gmres_reset(hipStream_t s, int nsys, const KrylovArgs<T> &k, const IrSysState *ir,
+                 const int *nrows, double tol);
+
+void ir_ainf(hipStream_t s, const Tiles &tl, const int *ptr, const double *val, const IrArgs &a);
+void ir_bnorm(hipStream_t s, const Tiles &tl, const double *b, const IrArgs &a);
+void ir_diagnostics(hipStream_t s, const Tiles &tl, const int *ptr, const int *col,
+                    const double *val, const double *b, const double *x, double *r,
+                    const IrArgs &a, int step);
+void ir_update(hipStream_t s, const Tiles &tl, double *x, const double *d, const IrArgs &a);
+void ir_scale_rhs(hipStream_t s, const Tiles &tl, const double *r, float *rf, const IrArgs &a);
+template <typename T>
+void ir_absorb_inner(hipStream_t s, int nsys, const KrylovArgs<T> &k, const IrArgs &a);
+
+} // namespace bspai_hip

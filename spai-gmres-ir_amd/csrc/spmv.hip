@@ -1,0 +1,198 @@
+// CSR sparse matrix-vector products and small vector utilities for gfx950.
+//
+// spmv: a group of LANES consecutive lanes owns one row; it streams the row's
+// (col, val) pairs with unit stride (one coalesced run per group) and reduces
+// with a fixed butterfly, so results are deterministic.  LANES = 1 keeps the
+// reference's sequential row order (src/cpu.cpp:88-89) and is bit-identical to
+// CsrMatrix::multiply; wider groups suit long rows.  The grid walks row tiles,
+// and a tile whose system is inactive is skipped, which lets one launch serve
+// many independent systems.
+#include "common.hpp"
+#include "kernels.hpp"
+
+namespace bspai_hip {
+
+template <int LANES, typename TV, typename TX, typename TA, typename TY>
+__global__ void __launch_bounds__(256)
+spmv_kernel(Tiles tiles, const int *__restrict__ flag, int flag_stride, const int *__restrict__ ptr,
+            const int *__restrict__ col, const TV *__restrict__ val, const TX *__restrict__ x,
+            TY *__restrict__ y) {
+    const int t = blockIdx.x;
+    if (flag && !flag[(std::int64_t)tiles.sys[t] * flag_stride])
+        return;
+    const int groups = blockDim.x / LANES;
+    const int g = threadIdx.x / LANES, lane = threadIdx.x % LANES;
+    const int r1 = tiles.row1[t];
+    for (int row = tiles.row0[t] + g; row < r1; row += groups) {
+        TA acc = 0;
+        const int e1 = ptr[row + 1];
+        for (int e = ptr[row] + lane; e < e1; e += LANES)
+            acc += (TA)val[e] * (TA)x[col[e]];
+        if (LANES > 1)
+            acc = group_sum<LANES>(acc);
+        if (lane == 0)
+            y[row] = (TY)acc;
+    }
+}
+
+template <typename TV, typename TX, typename TA, typename TY>
+void launch_spmv(hipStream_t s, int lanes, const Tiles &tiles, const int *flag, int flag_stride,
+                 const int *ptr, const int *col, const TV *val, const TX *x, TY *y) {
+    if (tiles.ntiles == 0)
+        return;
+    const dim3 grid(tiles.ntiles), block(kTileThreads);
+#define BSPAI_SPMV_CASE(L)                                                                        \
+    case L:                                                                                      \
+        hipLaunchKernelGGL((spmv_kernel<L, TV, TX, TA, TY>), grid, block, 0, s, tiles, flag,     \
+                           flag_stride, ptr, col, val, x, y);                                    \
+        break;
+    switch (lanes) {
+        BSPAI_SPMV_CASE(1)
+        BSPAI_SPMV_CASE(2)
+        BSPAI_SPMV_CASE(4)
+        BSPAI_SPMV_CASE(8)
+        BSPAI_SPMV_CASE(16)
+        BSPAI_SPMV_CASE(32)
+        BSPAI_SPMV_CASE(64)
+    default:
+        throw std::invalid_argument("spmv lanes must be a power of two <= 64");
+    }
+#undef BSPAI_SPMV_CASE
+    HIP_LAUNCH_CHECK();
+}
+
+#define BSPAI_SPMV_INST(TV, TX, TA, TY)                                                           \
+    template void launch_spmv<TV, TX, TA, TY>(hipStream_t, int, const Tiles &, const int *, int,  \
+                                              const int *, const int *, const TV *, const TX *,   \
+                                              TY *);
+BSPAI_SPMV_INST(double, double, double, double)
+BSPAI_SPMV_INST(float, float, float, float)
+BSPAI_SPMV_INST(float, double, double, double)
+BSPAI_SPMV_INST(double, float, double, double)
+#undef BSPAI_SPMV_INST
+
+template <typename TS, typename TD>
+__global__ void convert_kernel(std::int64_t n, const TS *__restrict__ src, TD *__restrict__ dst) {
+    const std::int64_t i = (std::int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+        dst[i] = (TD)src[i];
+}
+template <typename TS, typename TD>
+void launch_convert(hipStream_t s, std::int64_t n, const TS *src, TD *dst) {
+    if (n == 0)
+        return;
+    hipLaunchKernelGGL((convert_kernel<TS, TD>), dim3(ceil_div(n, 256)), dim3(256), 0, s, n, src,
+                       dst);
+    HIP_LAUNCH_CHECK();
+}
+template void launch_convert<double, float>(hipStream_t, std::int64_t, const double *, float *);
+template void launch_convert<float, double>(hipStream_t, std::int64_t, const float *, double *);
+template void launch_convert<double, double>(hipStream_t, std::int64_t, const double *, double *);
+
+// ------------------------------------------------------------------ SDDMM
+// out[q] = alpha * u[row(q)] * v[col(q)] over the stored pattern: the gradient
+// of a bilinear loss u^T A v restricted to A's nonzeros.
+__global__ void __launch_bounds__(kTileThreads)
+sddmm_kernel(Tiles tiles, const int *__restrict__ ptr, const int *__restrict__ col, double alpha,
+             const double *__restrict__ u, const double *__restrict__ v, double *__restrict__ out) {
+    const int t = blockIdx.x;
+    const int r1 = tiles.row1[t];
+    for (int row = tiles.row0[t] + threadIdx.x / 8; row < r1; row += kTileThreads / 8) {
+        const double ur = alpha * u[row];
+        for (int e = ptr[row] + (threadIdx.x & 7); e < ptr[row + 1]; e += 8)
+            out[e] = ur * v[col[e]];
+    }
+}
+
+void launch_sddmm(hipStream_t s, const Tiles &tiles, const int *ptr, const int *col, double alpha,
+                  const double *u, const double *v, double *out) {
+    if (tiles.ntiles == 0)
+        return;
+    hipLaunchKernelGGL(sddmm_kernel, dim3(tiles.ntiles), dim3(kTileThreads), 0, s, tiles, ptr, col,
+                       alpha, u, v, out);
+    HIP_LAUNCH_CHECK();
+}
+
+// ------------------------------------------------------------------ scan
+// Inclusive scan of int32 data[0..n) in place: each block scans a chunk of
+// kScanChunk elements, block totals are scanned by one block, then added back.
+constexpr int kScanThreads = 256, kScanPer = 8, kScanChunk = kScanThreads * kScanPer;
+
+__global__ void scan_chunks_kernel(int *__restrict__ data, std::int64_t n, int *__restrict__ sums) {
+    __shared__ int lds[kScanThreads];
+    const std::int64_t base = (std::int64_t)blockIdx.x * kScanChunk + threadIdx.x * kScanPer;
+    int v[kScanPer];
+    int run = 0;
+#pragma unroll
+    for (int i = 0; i < kScanPer; ++i) {
+        run += (base + i < n) ? data[base + i] : 0;
+        v[i] = run;
+    }
+    lds[threadIdx.x] = run;
+    __syncthreads();
+    for (int off = 1; off < kScanThreads; off <<= 1) {
+        const int t = threadIdx.x >= off ? lds[threadIdx.x - off] : 0;
+        __syncthreads();
+        lds[threadIdx.x] += t;
+        __syncthreads();
+    }
+    const int prefix = threadIdx.x ? lds[threadIdx.x - 1] : 0;
+#pragma unroll
+    for (int i = 0; i < kScanPer; ++i)
+        if (base + i < n)
+            data[base + i] = v[i] + prefix;
+    if (threadIdx.x == kScanThreads - 1)
+        sums[blockIdx.x] = lds[kScanThreads - 1];
+}
+
+__global__ void scan_sums_kernel(int *__restrict__ sums, int nblocks) {
+    // single block; sequential carry across passes of kScanThreads
+    __shared__ int lds[kScanThreads];
+    int carry = 0;
+    for (int b0 = 0; b0 < nblocks; b0 += kScanThreads) {
+        const int i = b0 + threadIdx.x;
+        lds[threadIdx.x] = i < nblocks ? sums[i] : 0;
+        __syncthreads();
+        for (int off = 1; off < kScanThreads; off <<= 1) {
+            const int t = threadIdx.x >= off ? lds[threadIdx.x - off] : 0;
+            __syncthreads();
+            lds[threadIdx.x] += t;
+            __syncthreads();
+        }
+        if (i < nblocks)
+            sums[i] = lds[threadIdx.x] + carry;
+        const int total = lds[kScanThreads - 1];
+        __syncthreads();
+        carry += total;
+    }
+}
+
+__global__ void scan_add_kernel(int *__restrict__ data, std::int64_t n, const int *__restrict__ sums) {
+    if (blockIdx.x == 0)
+        return;
+    const int add = sums[blockIdx.x - 1];
+    const std::int64_t base = (std::int64_t)blockIdx.x * kScanChunk;
+    for (int i = threadIdx.x; i < kScanChunk; i += kScanThreads)
+        if (base + i < n)
+            data[base + i] += add;
+}
+
+std::int64_t scan_tmp_ints(std::int64_t n) {
+    return (n + kScanChunk - 1) / kScanChunk + 1;
+}
+
+void scan_inclusive_int(hipStream_t s, int *data, std::int64_t n, int *tmp) {
+    if (n == 0)
+        return;
+    const int nb = (int)((n + kScanChunk - 1) / kScanChunk);
+    hipLaunchKernelGGL(scan_chunks_kernel, dim3(nb), dim3(kScanThreads), 0, s, data, n, tmp);
+    HIP_LAUNCH_CHECK();
+    if (nb > 1) {
+        hipLaunchKernelGGL(scan_sums_kernel, dim3(1), dim3(kScanThreads), 0, s, tmp, nb);
+        HIP_LAUNCH_CHECK();
+        hipLaunchKernelGGL(scan_add_kernel, dim3(nb), dim3(kScanThreads), 0, s, data, n, tmp);
+        HIP_LAUNCH_CHECK();
+    }
+}
+
+} // namespace bspai_hip

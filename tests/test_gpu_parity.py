@@ -1,0 +1,276 @@
+"""GPU parity of the HIP path against the reference's golden outputs and the
+CPU oracle (bit-exact where the arithmetic is sequential, tolerance-bounded
+where the GPU reduces in parallel).
+
+Bars (stated per test):
+  * SpMV with lanes=1, SPAI FP64 (scaling, patterns, coefficients, P), IR
+    diagnostics of an identical iterate: bit-exact.
+  * GMRES / IR: status equal, inner iteration counts within +-1 of the
+    reference, final backward error <= 10*2^-53, solution within 1e-10
+    relative of the reference's.
+  * Mixed precision: P entries within 1e-5 relative of the FP64 P on the common
+    pattern (pattern differences counted), final rel-res <= 1e-12.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from conftest import bits
+
+pytestmark = pytest.mark.gpu
+
+
+def _problems():
+    from bspai_hip import problems as P
+
+    return P
+
+
+def test_context_and_abi(ctx):
+    import bspai_hip
+
+    assert bspai_hip.load().bspai_hip_abi_version() == 1
+    ctx.synchronize()
+
+
+@pytest.mark.parametrize("lanes", [1, 2, 8, 64])
+def test_spmv_c1(ctx, lanes):
+    """Row-sequential SpMV is bit-identical to CsrMatrix::multiply (src/cpu.cpp:80-94);
+    grouped lanes stay within the reference's row-length bound
+    (tests/test_cuda_spmv.cpp:26-39): |y - y_ref| <= (row nnz) * u * sum |a||x|."""
+    import bspai_hip
+
+    P = _problems()
+    a = P.convection_diffusion_2d()
+    rng = np.random.default_rng(7)
+    x = rng.standard_normal(a.n)
+    m = bspai_hip.Matrix(a, ctx)
+    y = m.multiply(x, lanes=lanes)
+    # sequential CPU product in the reference's order
+    ref = np.zeros(a.n)
+    for r in range(a.n):
+        s = 0.0
+        for q in range(a.ptr[r], a.ptr[r + 1]):
+            s += a.val[q] * x[a.col[q]]
+        ref[r] = s
+    if lanes == 1:
+        assert np.array_equal(bits(y), bits(ref))
+    else:
+        absprod = np.abs(a.to_scipy()) @ np.abs(x)
+        rowlen = np.diff(a.ptr)
+        assert np.all(np.abs(y - ref) <= rowlen * 2.0 ** -53 * absprod * 2)
+
+
+def test_spmv_exact_dyadic(ctx):
+    """Dyadic values give exact sums in any order (tests/test_cuda_spmv.cpp:73-89)."""
+    import bspai_hip
+
+    P = _problems()
+    a = P.band(1000, 8.0)
+    x = np.ldexp(1.0, -(np.arange(1000) % 7))
+    y = bspai_hip.Matrix(a, ctx).multiply(x, lanes=8)
+    assert np.array_equal(y, a.multiply(x))
+
+
+def test_spai_c1_bit_exact(ctx, c1_golden):
+    """GPU FP64 SPAI on the 64x64 stencil equals the reference's construct_spai bit for bit."""
+    import bspai_hip
+
+    P = _problems()
+    a = P.convection_diffusion_2d()
+    m = bspai_hip.Matrix(a, ctx)
+    s = bspai_hip.construct_spai(m, epsilon=0.3, alpha=-(-a.n // 8), beta=8)
+    assert s.status == "success"
+    cols = s.columns()
+    assert np.array_equal(cols["fits"], c1_golden["fits"])
+    assert np.array_equal(cols["growths"], c1_golden["growths"])
+    assert np.array_equal(cols["clamps"], c1_golden["clamps"])
+    assert np.array_equal(bits(cols["residual"]), bits(c1_golden["residual"]))
+    assert np.array_equal(bits(s.scaling()), bits(c1_golden["scaling"]))
+    ptr, col, val = s.preconditioner()
+    assert np.array_equal(ptr, c1_golden["p_ptr"])
+    assert np.array_equal(col, c1_golden["p_col"])
+    assert np.array_equal(bits(val), bits(c1_golden["p_val"]))
+
+
+@pytest.mark.parametrize("n", [16, 64, 256, 1024])
+@pytest.mark.parametrize("diag", [8, 5])
+def test_spai_band_bit_exact(ctx, band_golden, n, diag):
+    """Banded family of the reference's sparse GPU SPAI test (tests/test_cuda_spai_sparse.cpp:7)."""
+    import bspai_hip
+
+    P = _problems()
+    a = P.band(n, float(diag))
+    g = {k[len(f"n{n}_d{diag}_"):]: v for k, v in band_golden.items()
+         if k.startswith(f"n{n}_d{diag}_")}
+    s = bspai_hip.construct_spai(bspai_hip.Matrix(a, ctx), epsilon=0.3, alpha=5, beta=8)
+    assert s.status == str(g["spai_status"])
+    cols = s.columns()
+    assert np.array_equal(cols["fits"], g["fits"])
+    assert np.array_equal(cols["growths"], g["growths"])
+    ptr, col, val = s.preconditioner()
+    assert np.array_equal(ptr, g["p_ptr"])
+    assert np.array_equal(col, g["p_col"])
+    assert np.array_equal(bits(val), bits(g["p_val"]))
+
+
+def test_spai_tiny_closed_form(ctx):
+    """[[4,1,0],[2,5,1],[0,1,3]] at eps 1e-12, alpha 2: P is the exact inverse
+    [[14,-3,1],[-6,12,-4],[2,-4,18]]/50 (reference tests/test_spai.cpp:139-168)."""
+    import bspai_hip
+
+    P = _problems()
+    a = P._from_coo(3, np.array([0, 0, 1, 1, 1, 2, 2]), np.array([0, 1, 0, 1, 2, 1, 2]),
+                    np.array([4.0, 1, 2, 5, 1, 1, 3]))
+    s = bspai_hip.construct_spai(bspai_hip.Matrix(a, ctx), epsilon=1e-12, alpha=2, beta=8)
+    assert s.status == "success"
+    ptr, col, val = s.preconditioner()
+    dense = np.zeros((3, 3))
+    for r in range(3):
+        dense[r, col[ptr[r]:ptr[r + 1]]] = val[ptr[r]:ptr[r + 1]]
+    exact = np.array([[14, -3, 1], [-6, 12, -4], [2, -4, 18]]) / 50.0
+    assert np.allclose(dense, exact, rtol=0, atol=768 * 2.0 ** -52)
+
+
+@pytest.mark.parametrize("case,expected", [
+    ("zero_diagonal", "invalid_input"),
+    ("rank_zero_pivot", "rank_deficient"),
+    ("scaling_overflow", "invalid_input"),
+    ("beta_zero", "no_growth"),
+    ("alpha_zero", "iteration_limit"),
+])
+def test_spai_failure_statuses(ctx, case, expected):
+    """Failure fixtures of the reference (tests/release_cpu_snapshot.cpp:362-376,
+    tests/test_spai.cpp:200-239)."""
+    import bspai_hip
+
+    P = _problems()
+
+    def dense(m):
+        m = np.asarray(m, dtype=float)
+        r, c = np.nonzero(m)
+        return P._from_coo(m.shape[0], r, c, m[r, c])
+
+    tiny = [[4, 1, 0], [2, 5, 1], [0, 1, 3]]
+    mats = {"zero_diagonal": ([[0, 1], [1, 1]], 1e-12, 3, 8),
+            "rank_zero_pivot": ([[1, 2], [1, 2]], 1e-12, 3, 8),
+            "scaling_overflow": ([[1e-320, 0], [0, 1]], 1e-12, 2, 8),
+            "beta_zero": (tiny, 1e-12, 2, 0),
+            "alpha_zero": (tiny, 1e-12, 0, 8)}
+    m, eps, alpha, beta = mats[case]
+    s = bspai_hip.construct_spai(bspai_hip.Matrix(dense(m), ctx), epsilon=eps, alpha=alpha,
+                                 beta=beta)
+    assert s.status == expected
+
+
+def test_score_device_boundaries(ctx):
+    """Restricted score known answers (reference tests/test_spai.cpp:240-250)."""
+    import bspai_hip
+
+    st, sc, cl = bspai_hip.evaluate_spai_score(4, 1, 2, ctx)
+    assert st == "success" and sc == math.sqrt(3.5) and not cl
+    st, sc, cl = bspai_hip.evaluate_spai_score(1, math.nextafter(1.0, 2.0), 1, ctx)
+    assert st == "success" and cl and sc == 0
+    bd = 1 + 16 * 2.0 ** -52
+    assert bspai_hip.evaluate_spai_score(1, bd, 1, ctx)[0] == "success"
+    assert bspai_hip.evaluate_spai_score(1, math.nextafter(bd, 2.0), 1, ctx)[0] == "invalid_score"
+    assert bspai_hip.evaluate_spai_score(1, 1, 0, ctx)[0] == "zero_denominator"
+
+
+def test_gmres_c1_vs_oracle(ctx):
+    """One FP64 GMRES solve with the oracle's P: same status, iterations within 1."""
+    import bspai_hip
+    from oracle import pyoracle as O
+
+    P = _problems()
+    a = P.convection_diffusion_2d()
+    s = O.construct_spai(a)
+    p = P.Csr(a.n, s.p_ptr, s.p_col, s.p_val)
+    b = P.equal_rhs(a.n)
+    ref = O.gmres(a, p, b, 1e-8, 0)
+    g = bspai_hip.left_preconditioned_gmres(bspai_hip.Matrix(a, ctx), bspai_hip.Matrix(p, ctx),
+                                            b, 1e-8, 0)
+    assert g.status == ref.status == "success"
+    assert abs(g.iterations - ref.iterations) <= 1
+    assert g.final_preconditioned_relative <= 1e-8
+    x = np.asarray(g.solution)
+    assert np.linalg.norm(x - ref.solution) <= 1e-6 * np.linalg.norm(ref.solution)
+
+
+def test_ir_c1_fp64_vs_reference(ctx, c1_golden):
+    """Full FP64 SPAI-GMRES-IR on the 64x64 system against cpu_fp64_ir's result."""
+    import bspai_hip
+
+    P = _problems()
+    a = P.convection_diffusion_2d()
+    b = P.equal_rhs(a.n)
+    r = bspai_hip.spai_gmres_ir(bspai_hip.Matrix(a, ctx), b, epsilon=0.3, beta=8,
+                                tolerance=1e-8, max_iterations=0, max_outer_steps=20)
+    assert r.status == str(c1_golden["ir_status"]) == "success"
+    assert r.outer_steps == int(c1_golden["ir_outer_steps"])
+    ref_it = c1_golden["ir_inner_iterations"]
+    assert len(r.inner_iterations) == len(ref_it)
+    assert all(abs(x - y) <= 1 for x, y in zip(r.inner_iterations, ref_it))
+    assert r.normwise_backward_error <= 10 * 2.0 ** -53
+    # x0 = P b and its diagnostics are bit-identical (same P, sequential SpMV)
+    assert np.array_equal(bits(r.history[0]), bits(c1_golden["ir_history"][0]))
+    x = np.asarray(r.solution)
+    xr = c1_golden["ir_solution"]
+    assert np.max(np.abs(x - xr)) <= 1e-10 * np.max(np.abs(xr))
+    assert np.linalg.norm(b - a.multiply(x)) / np.linalg.norm(b) <= 1e-12
+
+
+def test_ir_c1_mixed(ctx, c1_golden):
+    """Mixed FP32/FP64: FP32 SPAI within 1e-5 of the FP64 P, IR to rel-res 1e-12."""
+    import bspai_hip
+
+    P = _problems()
+    a = P.convection_diffusion_2d()
+    b = P.equal_rhs(a.n)
+    m = bspai_hip.Matrix(a, ctx)
+    s32 = bspai_hip.construct_spai(m, epsilon=0.3, alpha=-(-a.n // 8), beta=8, precision="mixed")
+    assert s32.status == "success"
+    ptr, col, val = s32.preconditioner()
+    gp, gc, gv = c1_golden["p_ptr"], c1_golden["p_col"], c1_golden["p_val"]
+    diff_rows = 0
+    rel = []
+    for r in range(a.n):
+        c32, v32 = col[ptr[r]:ptr[r + 1]], val[ptr[r]:ptr[r + 1]]
+        c64, v64 = gc[gp[r]:gp[r + 1]], gv[gp[r]:gp[r + 1]]
+        if not np.array_equal(c32, c64):
+            diff_rows += 1
+            continue
+        rel.append(np.abs(v32 - v64) / np.abs(v64))
+    rel = np.concatenate(rel)
+    print(f"mixed SPAI: pattern-differing columns {diff_rows}/{a.n}, max rel {rel.max():.3e}")
+    assert np.percentile(rel, 99) <= 1e-5
+    r = bspai_hip.spai_gmres_ir(m, b, precision="mixed", tolerance=1e-4, relres_target=1e-12)
+    assert r.status == "success", r.message
+    x = np.asarray(r.solution)
+    assert np.linalg.norm(b - a.multiply(x)) / np.linalg.norm(b) <= 1e-12
+    assert r.normwise_backward_error <= 10 * 2.0 ** -53
+
+
+def test_batched_systems_match_single(ctx):
+    """Block-diagonal batch: each system's result equals its own solve."""
+    import bspai_hip
+
+    P = _problems()
+    rng = np.random.default_rng(3)
+    blocks = []
+    for side in (32, 40, 48):
+        kappa = np.exp(rng.standard_normal((side, side)))
+        blocks.append(P.variable_coefficient_2d(side, kappa, rng.uniform(0, 10)))
+    big = P.block_diagonal(blocks)
+    offs = np.cumsum([0] + [b.n for b in blocks])
+    rhs = P.equal_rhs_blocks(offs)
+    res = bspai_hip.spai_gmres_ir(bspai_hip.Matrix(big, ctx, systems=offs), rhs)
+    assert len(res) == 3
+    x = np.asarray(res[0].solution)
+    for s, blk in enumerate(blocks):
+        single = bspai_hip.spai_gmres_ir(bspai_hip.Matrix(blk, ctx), rhs[offs[s]:offs[s + 1]])
+        assert res[s].status == single.status == "success"
+        assert res[s].inner_iterations == single.inner_iterations
+        xs = np.asarray(single.solution)
+        assert np.array_equal(bits(x[offs[s]:offs[s + 1]]), bits(xs))
