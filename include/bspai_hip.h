@@ -1,9 +1,12 @@
 /* C ABI of the gfx950 SPAI-GMRES-IR library (libbspai_hip.so).
  *
  * Plain C types only: pointers, sizes, int status codes; no C++ or torch types
- * cross this boundary and no exception escapes it.  Every call is synchronous
- * on the context's stream when it returns (as the reference's CUDA objects
- * are, include/bspai/cuda_gmres.hpp:18-20) and a context is not thread-safe.
+ * cross this boundary and no exception escapes it.  Calls that take host
+ * arrays, and every solver call (they report statuses), are synchronous when
+ * they return, as the reference's CUDA objects are
+ * (include/bspai/cuda_gmres.hpp:18-20); bspai_hip_spmv / _spmv_f32 / _sddmm on
+ * device arrays are stream-ordered on the context's stream.  A context is not
+ * thread-safe.
  *
  * Vectors and matrices may live on the host (where = BSPAI_HIP_HOST: copied
  * in/out by the library) or on the device (BSPAI_HIP_DEVICE: used in place;
@@ -167,8 +170,9 @@ int bspai_hip_gmres(bspai_hip_ctx *ctx, const bspai_hip_matrix *a, const bspai_h
                     const double *rhs, const double *x0, const bspai_hip_gmres_options *opt,
                     double *x, int where, bspai_hip_gmres_info *info);
 
-/* Iterative refinement with a built preconditioner; info[nsys]. */
-int bspai_hip_ir_solve(bspai_hip_ctx *ctx, const bspai_hip_matrix *a, const bspai_hip_spai *p,
+/* Iterative refinement with a given preconditioner P (any matrix handle, e.g.
+ * bspai_hip_spai_matrix of a build); info[nsys]. */
+int bspai_hip_ir_solve(bspai_hip_ctx *ctx, const bspai_hip_matrix *a, const bspai_hip_matrix *p,
                        const double *b, const bspai_hip_ir_options *opt, double *x, int where,
                        bspai_hip_ir_info *info);
 /* SPAI construction + refinement (the solver CLI's call); info[nsys]. */

@@ -155,6 +155,7 @@ class Matrix:
             y = np.empty(self.n)
         check(_lib.load().bspai_hip_spmv(self.ctx.h, self.h, _addr(xs), _addr(y), _where(xs),
                                          lanes), self.ctx.h)
+        self.ctx.synchronize()
         return y
 
     def multiply_f32(self, x, lanes: int = 0):
@@ -167,6 +168,7 @@ class Matrix:
             y = np.empty(self.n, np.float32)
         check(_lib.load().bspai_hip_spmv_f32(self.ctx.h, self.h, _addr(xs), _addr(y),
                                              _where(xs), lanes), self.ctx.h)
+        self.ctx.synchronize()
         return y
 
     def sddmm(self, u, v, alpha: float = 1.0):
@@ -181,6 +183,7 @@ class Matrix:
             out = np.empty(max(self.nnz, 1))
         check(_lib.load().bspai_hip_sddmm(self.ctx.h, self.h, float(alpha), _addr(us), _addr(vs),
                                           _addr(out), _where(us)), self.ctx.h)
+        self.ctx.synchronize()
         return out if _where(us) == DEVICE else out[: self.nnz]
 
 
@@ -379,16 +382,17 @@ def spai_gmres_ir(a: Matrix, rhs, **kw):
     return out[0] if a.nsys == 1 else out
 
 
-def refine_with_preconditioner(a: Matrix, spai: SpaiResult, rhs, **kw):
-    """IR with a built preconditioner (reference refine_with_preconditioner,
-    src/ir.cpp:138-204)."""
+def refine_with_preconditioner(a: Matrix, p, rhs, **kw):
+    """IR with a given preconditioner (reference refine_with_preconditioner,
+    src/ir.cpp:138-204).  p: a Matrix, or a SpaiResult (its P is used)."""
     ctx = a.ctx
+    pm = p.matrix() if isinstance(p, SpaiResult) else p
     b = _host(rhs, np.float64)
     x = _x_like(b, a.n)
     infos = (IrInfo * a.nsys)()
     opt = ir_options(**kw)
-    check(_lib.load().bspai_hip_ir_solve(ctx.h, a.h, spai.handle, _addr(b), C.byref(opt),
-                                         _addr(x), _where(b), infos), ctx.h)
+    check(_lib.load().bspai_hip_ir_solve(ctx.h, a.h, pm.h, _addr(b), C.byref(opt), _addr(x),
+                                         _where(b), infos), ctx.h)
     out = [_ir_info(i, x) for i in infos]
     return out[0] if a.nsys == 1 else out
 

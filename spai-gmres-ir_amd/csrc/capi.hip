@@ -19,7 +19,6 @@ struct bspai_hip_matrix {
 };
 struct bspai_hip_spai {
     std::unique_ptr<SpaiBuild> b;
-    std::unique_ptr<DevCsr> p64; // FP64 view of P (mixed builds widen on demand)
 };
 
 namespace {
@@ -79,8 +78,12 @@ template <typename T> struct Out {
         own.resize(std::max<std::size_t>(n, 1));
         p = own.data();
     }
+    // Host outputs are copied back and the call is synchronous; device
+    // outputs are stream-ordered on the context stream (the caller syncs).
     void finish() {
-        if (where != BSPAI_HIP_DEVICE && n)
+        if (where == BSPAI_HIP_DEVICE)
+            return;
+        if (n)
             HIP_CHECK(hipMemcpyAsync(dst, p, n * sizeof(T), hipMemcpyDeviceToHost, c.stream));
         HIP_CHECK(hipStreamSynchronize(c.stream));
     }
@@ -186,49 +189,49 @@ IrOptionsC ir_opts(const bspai_hip_ir_options *o) {
     return r;
 }
 
-std::unique_ptr<DevCsr> widen_p(Context &c, const SpaiBuild &b) {
+// FP64 copy of an FP32-valued CSR (a mixed-precision P).
+std::unique_ptr<DevCsr> widen(Context &c, const DevCsr &src) {
     auto p = std::make_unique<DevCsr>();
-    p->n = b.p.n;
-    p->nnz = b.p.nnz;
+    p->n = src.n;
+    p->nnz = src.nnz;
     p->ptr.resize(p->n + 1);
     p->col.resize(std::max<std::int64_t>(p->nnz, 1));
     p->val.resize(std::max<std::int64_t>(p->nnz, 1));
-    HIP_CHECK(hipMemcpyAsync(p->ptr.data(), b.p.ptr.data(), (p->n + 1) * sizeof(int),
+    HIP_CHECK(hipMemcpyAsync(p->ptr.data(), src.ptr.data(), (p->n + 1) * sizeof(int),
                              hipMemcpyDeviceToDevice, c.stream));
-    HIP_CHECK(hipMemcpyAsync(p->col.data(), b.p.col.data(), p->nnz * sizeof(int),
+    HIP_CHECK(hipMemcpyAsync(p->col.data(), src.col.data(), p->nnz * sizeof(int),
                              hipMemcpyDeviceToDevice, c.stream));
-    launch_convert<float, double>(c.stream, p->nnz, b.p.val32.data(), p->val.data());
+    launch_convert<float, double>(c.stream, p->nnz, src.val32.data(), p->val.data());
     p->has64 = true;
-    p->lanes = b.p.lanes;
-    p->layout = b.p.layout;
+    p->lanes = src.lanes;
+    p->layout = src.layout;
     return p;
 }
 
-void run_ir(bspai_hip_ctx *ctx, const bspai_hip_matrix *a, const bspai_hip_spai *p,
-            const double *b, const bspai_hip_ir_options *o, double *x, int where,
-            bspai_hip_ir_info *info) {
+// IR of A x = b with the preconditioner P (values in FP64 and/or FP32).
+void run_ir(bspai_hip_ctx *ctx, const DevCsr &A0, const DevCsr &P0, const double *b,
+            const bspai_hip_ir_options *o, double *x, int where, bspai_hip_ir_info *info) {
     Context &c = ctx->c;
-    const SpaiBuild &sb = *p->b;
-    require(sb.n == a->a.n, "preconditioner dimension mismatch");
+    require(P0.n == A0.n, "preconditioner dimension mismatch");
     const int prec = o->precision;
-    DevCsr &A = const_cast<DevCsr &>(a->a);
-    if (prec == BSPAI_HIP_MIXED)
-        A.ensure32(c);
-    // Preconditioner in the inner precision.
-    DevCsr *P = const_cast<DevCsr *>(&sb.p);
+    DevCsr &A = const_cast<DevCsr &>(A0);
+    DevCsr *P = const_cast<DevCsr *>(&P0);
     std::unique_ptr<DevCsr> tmp;
     if (prec == BSPAI_HIP_FP64 && !P->has64) {
-        tmp = widen_p(c, sb);
+        tmp = widen(c, *P);
         P = tmp.get();
     }
-    if (prec == BSPAI_HIP_MIXED)
+    if (prec == BSPAI_HIP_MIXED) {
+        A.ensure32(c);
         P->ensure32(c);
+    }
     In<double> bin(c, b, A.n, where);
     Out<double> xo(c, x, A.n, where);
     HIP_CHECK(hipEventRecord(c.ev0, c.stream));
     auto res = ir_run(c, A, A, *P, kLeft, bin.p, ir_opts(o), xo.p);
     HIP_CHECK(hipEventRecord(c.ev1, c.stream));
     xo.finish();
+    HIP_CHECK(hipStreamSynchronize(c.stream));
     float ms = 0;
     HIP_CHECK(hipEventElapsedTime(&ms, c.ev0, c.ev1));
     for (std::size_t s = 0; s < res.size(); ++s) {
@@ -593,17 +596,17 @@ int bspai_hip_gmres(bspai_hip_ctx *ctx, const bspai_hip_matrix *a, const bspai_h
     });
 }
 
-int bspai_hip_ir_solve(bspai_hip_ctx *ctx, const bspai_hip_matrix *a, const bspai_hip_spai *p,
+int bspai_hip_ir_solve(bspai_hip_ctx *ctx, const bspai_hip_matrix *a, const bspai_hip_matrix *p,
                        const double *b, const bspai_hip_ir_options *o, double *x, int where,
                        bspai_hip_ir_info *info) {
     return guard(ctx, [&] {
         validate_ir(a, o);
         require(p && b && x && info, "null argument");
         const int nsys = a->a.layout->nsys;
-        run_ir(ctx, a, p, b, o, x, where, info);
+        run_ir(ctx, a->a, p->a, b, o, x, where, info);
         for (int s = 0; s < nsys; ++s) {
             info[s].construction_performed = 0;
-            info[s].spai_status = p->b->status;
+            info[s].spai_status = 0;
             info[s].spai_ms = 0;
         }
     });
@@ -633,7 +636,7 @@ int bspai_hip_spai_gmres_ir(bspai_hip_ctx *ctx, const bspai_hip_matrix *a, const
             }
             return;
         }
-        run_ir(ctx, a, &p, b, o, x, where, info);
+        run_ir(ctx, a->a, p.b->p, b, o, x, where, info);
         for (int s = 0; s < nsys; ++s) {
             info[s].construction_performed = 1;
             info[s].spai_status = p.b->status;
@@ -666,7 +669,7 @@ int bspai_hip_spai_matrix(bspai_hip_ctx *ctx, const bspai_hip_spai *p, int trans
             d.layout = b.p.layout;
             d.lanes = b.p.lanes;
         } else {
-            auto w = widen_p(c, b);
+            auto w = widen(c, b.p);
             m->a = std::move(*w);
         }
         HIP_CHECK(hipStreamSynchronize(c.stream));

@@ -436,9 +436,7 @@ GmresEngine<T>::GmresEngine(Context &c, const Layout &L, int cap) : c_(c), L_(L)
     sn_.resize((std::size_t)cap * L.nsys);
     g_.resize((std::size_t)(cap + 1) * L.nsys);
     y_.resize((std::size_t)cap * L.nsys);
-    part_.resize(2 * (std::size_t)std::max(1, L.ntiles));
-    cnt_.resize(L.nsys);
-    HIP_CHECK(hipMemsetAsync(cnt_.data(), 0, L.nsys * sizeof(int), c.stream));
+    part_.resize((std::size_t)kNumSlots * 2 * std::max(1, L.ntiles));
     st_.resize(L.nsys);
     vdev_.resize(cap + 1);
     vptrs_.reserve(cap + 1);
@@ -453,7 +451,6 @@ GmresEngine<T>::GmresEngine(Context &c, const Layout &L, int cap) : c_(c), L_(L)
     args_.hstride = hs;
     args_.cap = cap;
     args_.part = part_.data();
-    args_.cnt = cnt_.data();
 }
 
 template <typename T> void GmresEngine<T>::ensure_basis(int count) {
@@ -476,45 +473,47 @@ void GmresEngine<T>::solve(const DevCsr &m1, const DevCsr &m2, GmresMode mode, c
                            const double *x0, double tol, double *x, const IrSysState *ir) {
     hipStream_t s = c_.stream;
     const Tiles tl = L_.tiles();
-    args_.x = mode == kLeft ? x : xz_.data();
-    args_.x0 = mode == kLeft ? x0 : nullptr;
+    const bool left = mode == kLeft;
+    args_.x = left ? x : xz_.data();
+    args_.x0 = left ? x0 : nullptr;
     ensure_basis(std::min(cap_ + 1, 2));
     gmres_reset<T>(s, L_.nsys, args_, ir, L_.s_rows.data(), tol);
-    const int *active =
-        reinterpret_cast<const int *>(reinterpret_cast<const char *>(st_.data()) +
-                                      offsetof(SysState, active));
-    const int *trig = reinterpret_cast<const int *>(reinterpret_cast<const char *>(st_.data()) +
+    SysState *st = st_.data();
+    double *part = part_.data();
+    const int *active = reinterpret_cast<const int *>(reinterpret_cast<const char *>(st) +
+                                                      offsetof(SysState, active));
+    const int *trig = reinterpret_cast<const int *>(reinterpret_cast<const char *>(st) +
                                                     offsetof(SysState, triggered));
     const int stride = (int)(sizeof(SysState) / sizeof(int));
     const T *m1v = m1.values<T>(), *m2v = m2.values<T>();
+    const T *none = nullptr;
     // ---- start (src/gmres.cpp:134-167)
-    if (mode == kLeft && x0) {
-        gmres_diff_norm<T, T, T>(s, tl, args_, rhs, (const T *)nullptr, nullptr, kNormRhs);
-        launch_spmv<T, double, double, double>(s, m1.lanes, tl, active, stride, m1.ptr.data(),
-                                               m1.col.data(), m1v, x0, ax_.data());
-        gmres_diff_norm<T, double, T>(s, tl, args_, rhs, ax_.data(), nullptr, kNormInitOrig);
-    } else {
-        // x0 = 0: ||rhs - A x0|| = ||rhs||, zero means zero_rhs.
-        gmres_diff_norm<T, T, T>(s, tl, args_, rhs, (const T *)nullptr, nullptr, kNormRhs);
-        gmres_diff_norm<T, T, T>(s, tl, args_, rhs, (const T *)nullptr, nullptr, kNormInitOrig);
-    }
+    gmres_diff_partial<T, T, T>(s, tl, st, part, rhs, none, (T *)nullptr, kSlotRhs);
+    if (left && x0)
+        gmres_spmv_resid<T, T>(s, tl, st, part, m1.ptr.data(), m1.col.data(), m1v, x0, rhs,
+                               ax_.data(), kSlotOrig, 0);
+    else
+        gmres_diff_partial<T, T, T>(s, tl, st, part, rhs, none, (T *)nullptr, kSlotOrig);
+    gmres_decide<T>(s, tl, args_, kPhaseInit, !left);
     const T *prhs = rhs;
-    if (mode == kLeft) {
+    if (left) {
         launch_spmv<T, T, T, T>(s, m2.lanes, tl, active, stride, m2.ptr.data(), m2.col.data(),
                                 m2v, rhs, prhs_.data());
         prhs = prhs_.data();
     }
-    gmres_diff_norm<T, T, T>(s, tl, args_, prhs, (const T *)nullptr, nullptr, kNormDen);
+    gmres_diff_partial<T, T, T>(s, tl, st, part, prhs, none, (T *)nullptr, kSlotDen);
+    gmres_decide<T>(s, tl, args_, kPhaseDen, !left);
     const T *r0 = prhs;
-    if (mode == kLeft && x0) {
+    if (left && x0) {
         launch_spmv<T, double, double, double>(s, m2.lanes, tl, active, stride, m2.ptr.data(),
                                                m2.col.data(), m2v, ax_.data(), pax_.data());
-        gmres_diff_norm<T, double, T>(s, tl, args_, prhs, pax_.data(), xz_.data(), kNormBeta);
-        launch_convert<double, T>(s, L_.n, xz_.data(), r0_.data());
+        gmres_diff_partial<T, double, T>(s, tl, st, part, prhs, pax_.data(), r0_.data(),
+                                         kSlotBeta);
         r0 = r0_.data();
     } else {
-        gmres_diff_norm<T, T, T>(s, tl, args_, prhs, (const T *)nullptr, nullptr, kNormBeta);
+        gmres_diff_partial<T, T, T>(s, tl, st, part, prhs, none, (T *)nullptr, kSlotBeta);
     }
+    gmres_decide<T>(s, tl, args_, kPhaseBeta, !left);
     gmres_start_vector<T>(s, tl, args_, r0);
     // ---- Arnoldi loop (src/gmres.cpp:172-254)
     std::vector<SysState> host(L_.nsys);
@@ -522,30 +521,24 @@ void GmresEngine<T>::solve(const DevCsr &m1, const DevCsr &m2, GmresMode mode, c
         ensure_basis(std::min(cap_ + 1, k + 2));
         launch_spmv<T, T, T, T>(s, m1.lanes, tl, active, stride, m1.ptr.data(), m1.col.data(),
                                 m1v, vptrs_[k], t_.data());
-        launch_spmv<T, T, T, T>(s, m2.lanes, tl, active, stride, m2.ptr.data(), m2.col.data(),
-                                m2v, t_.data(), w_.data());
-        for (int j = 0; j <= k + 1; ++j)
+        gmres_spmv_dot0<T>(s, m2.lanes, tl, args_, m2.ptr.data(), m2.col.data(), m2v, t_.data());
+        for (int j = 1; j <= k + 1; ++j)
             gmres_mgs_round<T>(s, tl, args_, j, k);
-        if (k + 1 <= cap_)
-            gmres_normalize<T>(s, tl, args_, k);
+        gmres_normalize_givens<T>(s, tl, args_, k);
         gmres_candidate<T>(s, tl, args_);
-        if (mode == kLeft) {
-            launch_spmv<T, double, double, double>(s, m1.lanes, tl, trig, stride, m1.ptr.data(),
-                                                   m1.col.data(), m1v, x, ax_.data());
-            gmres_diff_norm<T, double, T>(s, tl, args_, rhs, ax_.data(), nullptr, kNormCheckOrig);
-            launch_spmv<T, double, double, double>(s, m2.lanes, tl, trig, stride, m2.ptr.data(),
-                                                   m2.col.data(), m2v, ax_.data(), pax_.data());
-            gmres_diff_norm<T, double, T>(s, tl, args_, prhs, pax_.data(), nullptr,
-                                          kNormCheckRel);
+        if (left) {
+            gmres_spmv_resid<T, T>(s, tl, st, part, m1.ptr.data(), m1.col.data(), m1v, x, rhs,
+                                   ax_.data(), kSlotChkOrig, 1);
+            gmres_spmv_resid<T, T>(s, tl, st, part, m2.ptr.data(), m2.col.data(), m2v, ax_.data(),
+                                   prhs, pax_.data(), kSlotChkRel, 1);
         } else {
-            // x = P^T z, then ||g - A^T x|| (absolute and relative to ||g||).
+            // x = P^T z, then ||g - A^T x|| (absolute; relative to ||g||).
             launch_spmv<T, double, double, double>(s, m1.lanes, tl, trig, stride, m1.ptr.data(),
                                                    m1.col.data(), m1v, xz_.data(), x);
-            launch_spmv<T, double, double, double>(s, m2.lanes, tl, trig, stride, m2.ptr.data(),
-                                                   m2.col.data(), m2v, x, ax_.data());
-            gmres_diff_norm<T, double, T>(s, tl, args_, rhs, ax_.data(), nullptr, kNormCheckOrig);
-            gmres_diff_norm<T, double, T>(s, tl, args_, rhs, ax_.data(), nullptr, kNormCheckRel);
+            gmres_spmv_resid<T, T>(s, tl, st, part, m2.ptr.data(), m2.col.data(), m2v, x, rhs,
+                                   ax_.data(), kSlotChkOrig, 1);
         }
+        gmres_decide<T>(s, tl, args_, kPhaseCheck, !left);
         if ((k + 1) % sync_every == 0 || k + 1 == cap_) {
             download(c_, host.data(), st_.data(), L_.nsys);
             sync(c_);
@@ -556,12 +549,9 @@ void GmresEngine<T>::solve(const DevCsr &m1, const DevCsr &m2, GmresMode mode, c
                 break;
         }
     }
-    if (mode == kRight) {
-        // Systems that ended at x0 = 0 (zero rhs, or init_rel <= tau).
-        args_.x = x;
-    }
+    args_.x = x;
     gmres_finish<T>(s, tl, args_);
-    args_.x = mode == kLeft ? x : xz_.data();
+    args_.x = left ? x : xz_.data();
 }
 
 template <typename T> std::vector<GmresOutcome> GmresEngine<T>::outcomes() {
@@ -584,62 +574,35 @@ template class GmresEngine<double>;
 template class GmresEngine<float>;
 
 // ---------------------------------------------------------------- IR
-namespace {
-__global__ void ir_reset_kernel(int nsys, IrSysState *st) {
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= nsys)
-        return;
-    IrSysState z = {};
-    z.active = 1;
-    z.dscale = 1.0;
-    z.pending_dscale = 1.0;
-    z.status = kIrIterationLimit;
-    st[s] = z;
-}
-__global__ void ir_cap_kernel(int nsys, IrSysState *st) {
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= nsys)
-        return;
-    if (st[s].active) {
-        st[s].status = kIrIterationLimit;
-        st[s].active = 0;
-        st[s].done = 1;
-    }
-}
-} // namespace
-
 std::vector<IrResultHost> ir_run(Context &c, const DevCsr &a64, const DevCsr &m1, const DevCsr &m2,
                                  GmresMode mode, const double *b, const IrOptionsC &o, double *x) {
     hipStream_t s = c.stream;
     const Layout &L = *a64.layout;
     const Tiles tl = L.tiles();
     const int nsys = L.nsys;
+    const std::size_t nt = (std::size_t)std::max(1, L.ntiles);
     DeviceArray<IrSysState> st(nsys);
-    DeviceArray<double> part(4 * std::max(1, L.ntiles)), part2(2 * std::max(1, L.ntiles)),
-        part3(2 * std::max(1, L.ntiles));
-    DeviceArray<int> cnt(nsys);
+    DeviceArray<double> part(kIrNumSlots * 2 * nt), part4(4 * nt), flags(2 * nt), upd(nt);
     DeviceArray<double> hist((std::size_t)nsys * kMaxOuterHistory * 4);
     DeviceArray<int> inner((std::size_t)nsys * kMaxOuterHistory * 3);
     DeviceArray<double> r(L.n), d(L.n);
-    HIP_CHECK(hipMemsetAsync(cnt.data(), 0, nsys * sizeof(int), s));
     HIP_CHECK(hipMemsetAsync(hist.data(), 0, hist.size() * sizeof(double), s));
     HIP_CHECK(hipMemsetAsync(inner.data(), 0, inner.size() * sizeof(int), s));
-    hipLaunchKernelGGL(ir_reset_kernel, dim3(ceil_div(nsys, 64)), dim3(64), 0, s, nsys, st.data());
-    IrArgs a{st.data(), part.data(), part2.data(), part3.data(), cnt.data(), hist.data(),
+    IrArgs a{st.data(), part.data(), part4.data(), flags.data(), upd.data(), hist.data(),
              inner.data(), 10.0 * std::ldexp(1.0, -53), o.relres_target};
-    ir_ainf(s, tl, a64.ptr.data(), a64.val.data(), a);
-    ir_bnorm(s, tl, b, a);
-    // x0 = M b with M the preconditioner (src/ir.cpp:149)
+    ir_setup(s, tl, a64.ptr.data(), a64.val.data(), b, a);
+    // x0 = M b with M the preconditioner (src/ir.cpp:149), in the reference's
+    // sequential row order.
     const DevCsr &pre = mode == kLeft ? m2 : m1;
     if (o.precision == 0)
-        launch_spmv<double, double, double, double>(s, pre.lanes, tl, nullptr, 0, pre.ptr.data(),
+        launch_spmv<double, double, double, double>(s, 1, tl, nullptr, 0, pre.ptr.data(),
                                                     pre.col.data(), pre.val.data(), b, x);
     else
-        launch_spmv<float, double, double, double>(s, pre.lanes, tl, nullptr, 0, pre.ptr.data(),
+        launch_spmv<float, double, double, double>(s, 1, tl, nullptr, 0, pre.ptr.data(),
                                                    pre.col.data(), pre.val32.data(), b, x);
     ir_diagnostics(s, tl, a64.ptr.data(), a64.col.data(), a64.val.data(), b, x, r.data(), a, 0);
     const int cap = o.cap > 0 ? o.cap : (int)std::min<std::int64_t>(L.n, INT_MAX - 2);
-    // Largest system bounds the Krylov dimension of every system.
+    // The largest system bounds the Krylov dimension of every system.
     std::int64_t maxrows = 0;
     for (int i = 0; i < nsys; ++i)
         maxrows = std::max(maxrows, L.offsets[i + 1] - L.offsets[i]);
@@ -666,18 +629,18 @@ std::vector<IrResultHost> ir_run(Context &c, const DevCsr &a64, const DevCsr &m1
     for (int step = 0; step < o.outer && running; ++step) {
         if (o.precision == 0) {
             g64->solve(m1, m2, mode, r.data(), nullptr, o.tol, d.data(), st.data());
-            ir_absorb_inner<double>(s, nsys, g64->args(), a);
+            ir_absorb_inner<double>(s, nsys, g64->args(), a, 0);
         } else {
             ir_scale_rhs(s, tl, r.data(), rf.data(), a);
             g32->solve(m1, m2, mode, rf.data(), nullptr, o.tol, d.data(), st.data());
-            ir_absorb_inner<float>(s, nsys, g32->args(), a);
+            ir_absorb_inner<float>(s, nsys, g32->args(), a, 1);
         }
         ir_update(s, tl, x, d.data(), a);
         ir_diagnostics(s, tl, a64.ptr.data(), a64.col.data(), a64.val.data(), b, x, r.data(), a,
                        step + 1);
         running = any_active();
     }
-    hipLaunchKernelGGL(ir_cap_kernel, dim3(ceil_div(nsys, 64)), dim3(64), 0, s, nsys, st.data());
+    ir_cap(s, nsys, st.data());
     std::vector<IrResultHost> out(nsys);
     std::vector<double> hh(hist.size());
     std::vector<int> hi(inner.size());

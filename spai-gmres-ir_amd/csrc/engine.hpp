@@ -170,7 +170,6 @@ template <typename T> class GmresEngine {
     DeviceArray<T> w_, t_, prhs_, r0_;
     DeviceArray<double> xz_, ax_, pax_, ax0_, tmpd_;
     DeviceArray<double> H_, cs_, sn_, g_, y_, part_;
-    DeviceArray<int> cnt_;
     DeviceArray<SysState> st_;
     KrylovArgs<T> args_{};
 };

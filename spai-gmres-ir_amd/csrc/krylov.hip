@@ -1,54 +1,48 @@
-// Left-preconditioned GMRES and iterative-refinement kernels for gfx950.
+// Left-/right-preconditioned GMRES and iterative-refinement kernels (gfx950).
 //
 // Algorithm (reference src/gmres.cpp:108-262, src/ir.cpp:83-204): unrestarted
-// GMRES on P A d = P r with one-pass modified Gram-Schmidt, Givens rotations,
-// and a candidate x0 + V y that is formed and re-verified (true residual and
-// recomputed preconditioned residual) exactly on the iterations whose
-// acceptance test can pass: estimate <= tau, Arnoldi breakdown, zero Givens
-// pivot, or the last iteration.
+// GMRES with one-pass modified Gram-Schmidt and Givens rotations; the
+// candidate x0 + V y is formed and re-verified (true and recomputed
+// preconditioned residuals) exactly on the iterations whose acceptance test
+// can pass -- estimate <= tau, Arnoldi breakdown, zero Givens pivot, or the
+// last iteration -- which leaves every decision of the reference unchanged.
 //
-// Layout: all vectors are concatenations of independent systems; each kernel
-// runs one 256-thread block per row tile (<= 2048 rows, inside one system) and
-// skips tiles of systems that are not active, so one launch advances every
-// system of a batch.  Per-system reductions are deterministic: each tile
-// writes its partial, the last tile of the system to finish (atomic ticket)
-// sums the partials in tile order and runs the per-system scalar step (norm,
-// Hessenberg entry, Givens update, back substitution, accept decision) on one
-// thread in FP64.  No host round trip happens inside an iteration.
+// Layout: vectors concatenate independent systems; every kernel runs one
+// 256-thread block per row tile (<= 2048 rows, inside one system) and skips
+// tiles of inactive systems, so one launch advances a whole batch.
+//
+// Reductions never synchronize blocks inside a kernel (a device-scope release
+// on a multi-XCD part writes back the L2, which is far costlier than a launch).
+// Instead each tile writes its partial into a named slot, and the NEXT kernel
+// reduces the slot: every block of the system sums the partials in the same
+// fixed order, so all blocks obtain the same value bit for bit.  Per-system
+// state (Hessenberg entries, Givens data, decisions) has one writer, thread 0
+// of the system's first tile, and is read by later kernels only.
 //
 // Precision: T is the Krylov vector/matrix type (double for the plain path,
-// float for the mixed path).  Hessenberg, rotations, y and the candidate are
-// FP64 in both.
+// float for the mixed path); Hessenberg, rotations, y and candidates are FP64.
 #include "common.hpp"
 #include "krylov.hpp"
 
 namespace bspai_hip {
 
-// --------------------------------------------------------------- reductions
-// Robust 2-norm partials: per tile (e, s) with s = sum (x * 2^-e)^2, e the
-// exponent of the tile's max |x|; combined relative to the largest e.
-struct NormPart {
-    double e, s;
-};
-
-__device__ inline double combine_norm(const double *part, int t0, int t1) {
-    double emax = -1e300;
-    for (int t = t0; t < t1; ++t)
-        if (part[2 * t + 1] > 0)
-            emax = fmax(emax, part[2 * t]);
-    if (emax == -1e300)
-        return 0.0;
-    double s = 0;
-    for (int t = t0; t < t1; ++t)
-        if (part[2 * t + 1] > 0)
-            s += ldexp(part[2 * t + 1], 2 * (int)(part[2 * t] - emax));
-    return ldexp(sqrt(s), (int)emax);
+// ------------------------------------------------------------- partial slots
+// Dot partial of tile t: part[2t+1] (part[2t] = 0).  Norm partial: (e, s) with
+// s = sum (x 2^-e)^2 and e the exponent of the tile's max |x| (robust 2-norm).
+__device__ inline double *slot_ptr(double *part, int ntiles, int slot) {
+    return part + (std::int64_t)slot * 2 * ntiles;
 }
 
-// Tile-level robust sum of squares of the kTilePer values v[] held by each
-// thread; writes the tile's (e, s) to part[2t..2t+1].
-__device__ inline void tile_norm_partial(const double *v, int cnt, double *part, int t,
-                                         double *lds) {
+__device__ inline void store_dot_partial(double v, double *p, int t, double *lds) {
+    v = block_sum(v, lds);
+    if (threadIdx.x == 0) {
+        p[2 * t] = 0;
+        p[2 * t + 1] = v;
+    }
+}
+
+__device__ inline void store_norm_partial(const double *v, int cnt, double *p, int t,
+                                          double *lds) {
     double mx = 0;
     for (int i = 0; i < cnt; ++i)
         mx = fmax(mx, fabs(v[i]));
@@ -62,35 +56,59 @@ __device__ inline void tile_norm_partial(const double *v, int cnt, double *part,
             s += y * y;
         }
     } else if (!isfinite(mx)) {
-        s = mx; // propagate inf/nan
+        s = INFINITY;
     }
     s = block_sum(s, lds);
     if (threadIdx.x == 0) {
-        part[2 * t] = (double)e;
-        part[2 * t + 1] = s;
+        p[2 * t] = (double)e;
+        p[2 * t + 1] = s;
     }
 }
 
-// Returns true on the one thread (thread 0 of the system's last tile) that
-// must finalize system s.  Call after the tile's partial is stored.
-__device__ inline bool last_tile(int *cnt, int s, int ntiles_s) {
-    __shared__ int is_last;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        __threadfence();
-        const int ticket = atomicAdd(cnt + s, 1);
-        is_last = ticket == ntiles_s - 1;
-        if (is_last) {
-            cnt[s] = 0; // reset for the next reduction
-            __threadfence();
-        }
-    }
-    __syncthreads();
-    return is_last && threadIdx.x == 0;
+// Deterministic block-cooperative reductions over tiles [t0, t1): the thread
+// -> tile assignment and the combining tree are fixed, so every block that
+// evaluates them gets identical bits.
+__device__ inline double reduce_dot(const double *p, int t0, int t1, double *lds) {
+    double s = 0;
+    for (int t = t0 + (int)threadIdx.x; t < t1; t += blockDim.x)
+        s += p[2 * t + 1];
+    return block_sum(s, lds);
 }
+
+__device__ inline double reduce_norm(const double *p, int t0, int t1, double *lds) {
+    double emax = -1e300;
+    for (int t = t0 + (int)threadIdx.x; t < t1; t += blockDim.x)
+        if (p[2 * t + 1] > 0)
+            emax = fmax(emax, p[2 * t]);
+    emax = block_max(emax, lds);
+    if (emax == -1e300)
+        return 0.0;
+    double s = 0;
+    for (int t = t0 + (int)threadIdx.x; t < t1; t += blockDim.x)
+        if (p[2 * t + 1] > 0)
+            s += ldexp(p[2 * t + 1], 2 * (int)(p[2 * t] - emax));
+    s = block_sum(s, lds);
+    return ldexp(sqrt(s), (int)emax);
+}
+
+__device__ inline double reduce_max(const double *p, int t0, int t1, double *lds, int stride,
+                                    int off) {
+    double m = 0;
+    for (int t = t0 + (int)threadIdx.x; t < t1; t += blockDim.x)
+        m = fmax(m, p[(std::int64_t)stride * t + off]);
+    return block_max(m, lds);
+}
+
+#define BSPAI_SYS_TILES                                                                           \
+    const int t = blockIdx.x, s = tiles.sys[t];                                                  \
+    const int t0 = tiles.sys_tile0[s], t1 = tiles.sys_tile1[s];                                  \
+    const bool writer = t == t0 && threadIdx.x == 0;                                             \
+    (void)t1;                                                                                    \
+    (void)writer;
 
 // ------------------------------------------------------------------ GMRES
-// Per-system scalar step after h_{k+1,k} is known (src/gmres.cpp:183-254).
+// Givens update of column k once h_{k+1,k} is known (src/gmres.cpp:183-217);
+// on trigger, back substitution for y (src/gmres.cpp:47-59).
 __device__ void givens_step(SysState &st, double *H, double *cs, double *sn, double *g, double *y,
                             int k, int cap, double hnorm) {
     double *col = H + hcol_offset(k);
@@ -153,81 +171,97 @@ __device__ void givens_step(SysState &st, double *H, double *cs, double *sn, dou
     }
 }
 
-// One MGS round of iteration k (j = 0..k+1):
-//   j > 0:    w -= h_{j-1,k} V_{j-1}
-//   j <= k:   h_{j,k} = <V_j, w>
-//   j == k+1: h_{k+1,k} = ||w||, then the Givens step
+// w = M2 t for the tile's rows (group of LANES lanes per row), fused with the
+// first Gram-Schmidt partial <V_0, w>.
+template <int LANES, typename T>
+__global__ void __launch_bounds__(kTileThreads)
+spmv_dot0_kernel(Tiles tiles, KrylovArgs<T> a, const int *__restrict__ ptr,
+                 const int *__restrict__ col, const T *__restrict__ val, const T *__restrict__ x) {
+    __shared__ double lds[kTileThreads / kWave];
+    BSPAI_SYS_TILES
+    if (!a.st[s].active)
+        return;
+    const int groups = kTileThreads / LANES;
+    const int g = threadIdx.x / LANES, lane = threadIdx.x % LANES;
+    const T *v0 = a.V[0];
+    double dot = 0;
+    for (int row = tiles.row0[t] + g; row < tiles.row1[t]; row += groups) {
+        T acc = 0;
+        const int e1 = ptr[row + 1];
+        for (int e = ptr[row] + lane; e < e1; e += LANES)
+            acc += val[e] * x[col[e]];
+        if (LANES > 1)
+            acc = group_sum<LANES>(acc);
+        if (lane == 0) {
+            a.w[row] = acc;
+            dot += (double)v0[row] * (double)acc;
+        }
+    }
+    store_dot_partial(dot, slot_ptr(a.part, tiles.ntiles, 0), t, lds);
+}
+
+// MGS round j of iteration k (1 <= j <= k+1):
+//   h_{j-1,k} = reduce(slot (j-1)%2);  w -= h_{j-1,k} V_{j-1}
+//   j <= k:   partial <V_j, w>  -> slot j%2
+//   j == k+1: partial ||w||^2   -> slot j%2
 template <typename T>
 __global__ void __launch_bounds__(kTileThreads)
 mgs_round_kernel(Tiles tiles, KrylovArgs<T> a, int j, int k) {
     __shared__ double lds[kTileThreads / kWave];
-    const int t = blockIdx.x, s = tiles.sys[t];
+    BSPAI_SYS_TILES
     SysState &st = a.st[s];
     if (!st.active)
         return;
-    const int r0 = tiles.row0[t], r1 = tiles.row1[t];
+    const double h = reduce_dot(slot_ptr(a.part, tiles.ntiles, (j - 1) & 1), t0, t1, lds);
+    if (writer)
+        a.H[(std::int64_t)s * a.hstride + hcol_offset(k) + j - 1] = h;
+    const T hp = (T)h;
+    const T *vprev = a.V[j - 1];
     T *w = a.w;
-    double *H = a.H + (std::int64_t)s * a.hstride;
-    const T *vprev = j > 0 ? a.V[j - 1] : nullptr;
-    const T *vcur = j <= k ? a.V[j] : nullptr;
-    const T hprev = j > 0 ? (T)H[hcol_offset(k) + j - 1] : T(0);
-    double vals[kTilePer];
-    int cnt = 0;
-    double dot = 0;
-    for (int r = r0 + threadIdx.x; r < r1; r += kTileThreads) {
-        T x = w[r];
-        if (j > 0) {
-            x = x - hprev * vprev[r];
+    double *out = slot_ptr(a.part, tiles.ntiles, j & 1);
+    const int r0 = tiles.row0[t], r1 = tiles.row1[t];
+    if (j <= k) {
+        const T *vcur = a.V[j];
+        double dot = 0;
+        for (int r = r0 + threadIdx.x; r < r1; r += kTileThreads) {
+            const T x = w[r] - hp * vprev[r];
             w[r] = x;
-        }
-        if (j <= k)
             dot += (double)vcur[r] * (double)x;
-        else
-            vals[cnt++] = (double)x;
-    }
-    if (j <= k) {
-        dot = block_sum(dot, lds);
-        if (threadIdx.x == 0)
-            a.part[2 * t + 1] = dot;
-    } else {
-        tile_norm_partial(vals, cnt, a.part, t, lds);
-    }
-    if (!last_tile(a.cnt, s, tiles.sys_tile1[s] - tiles.sys_tile0[s]))
-        return;
-    __threadfence();
-    const int t0 = tiles.sys_tile0[s], t1 = tiles.sys_tile1[s];
-    if (j <= k) {
-        double h = 0;
-        for (int u = t0; u < t1; ++u)
-            h += a.part[2 * u + 1];
-        H[hcol_offset(k) + j] = h;
-        if (!isfinite(h)) {
-            st.status = kGmNumericalFailure;
-            st.active = 0;
         }
+        store_dot_partial(dot, out, t, lds);
     } else {
-        const double hn = combine_norm(a.part, t0, t1);
-        givens_step(st, H, a.cs + (std::int64_t)s * a.cap, a.sn + (std::int64_t)s * a.cap,
-                    a.g + (std::int64_t)s * (a.cap + 1), a.y + (std::int64_t)s * a.cap, k, a.cap,
-                    hn);
+        double vals[kTilePer];
+        int cnt = 0;
+        for (int r = r0 + threadIdx.x; r < r1; r += kTileThreads) {
+            const T x = w[r] - hp * vprev[r];
+            w[r] = x;
+            vals[cnt++] = (double)x;
+        }
+        store_norm_partial(vals, cnt, out, t, lds);
     }
 }
 
-// V_{k+1} = w / h_{k+1,k} (skipped on breakdown).
+// h_{k+1,k} = ||w||; V_{k+1} = w / h (unless breakdown); the writer runs the
+// Givens step for the system.
 template <typename T>
 __global__ void __launch_bounds__(kTileThreads)
-normalize_kernel(Tiles tiles, KrylovArgs<T> a, int k) {
-    const int t = blockIdx.x, s = tiles.sys[t];
-    const SysState &st = a.st[s];
-    if (!st.active || st.breakdown || st.iterations != k + 1)
+normalize_givens_kernel(Tiles tiles, KrylovArgs<T> a, int k) {
+    __shared__ double lds[kTileThreads / kWave];
+    BSPAI_SYS_TILES
+    SysState &st = a.st[s];
+    if (!st.active)
         return;
-    const double *H = a.H + (std::int64_t)s * a.hstride;
-    // The unrotated h_{k+1,k} was overwritten by the rotation; it is kept in hnorm.
-    const T h = (T)st.hnorm;
-    T *v = a.V[k + 1];
-    for (int r = tiles.row0[t] + threadIdx.x; r < tiles.row1[t]; r += kTileThreads)
-        v[r] = a.w[r] / h;
-    (void)H;
+    const double hn = reduce_norm(slot_ptr(a.part, tiles.ntiles, (k + 1) & 1), t0, t1, lds);
+    if (hn != 0 && isfinite(hn) && k + 1 <= a.cap) {
+        const T h = (T)hn;
+        T *v = a.V[k + 1];
+        for (int r = tiles.row0[t] + threadIdx.x; r < tiles.row1[t]; r += kTileThreads)
+            v[r] = a.w[r] / h;
+    }
+    if (writer)
+        givens_step(st, a.H + (std::int64_t)s * a.hstride, a.cs + (std::int64_t)s * a.cap,
+                    a.sn + (std::int64_t)s * a.cap, a.g + (std::int64_t)s * (a.cap + 1),
+                    a.y + (std::int64_t)s * a.cap, k, a.cap, hn);
 }
 
 // Candidate x = x0 + sum_{j<count} y_j V_j (FP64, j ascending, src/gmres.cpp:60-63).
@@ -248,17 +282,41 @@ candidate_kernel(Tiles tiles, KrylovArgs<T> a) {
     }
 }
 
-// Norm of (u - v) over active systems (flag chosen by `gate`), result used by
-// the per-system step `what` (see krylov.hpp: NormWhat).
-template <typename TU, typename TV, typename T>
+// y = M x on the tile's rows (sequential row order), fused with the norm
+// partial of (u - y) into `slot`; optional store of y.  gate: 0 active,
+// 1 active && triggered.
+template <typename TV, typename TU>
 __global__ void __launch_bounds__(kTileThreads)
-diff_norm_kernel(Tiles tiles, KrylovArgs<T> a, const TU *u, const TV *v, double *out, int what) {
+spmv_resid_kernel(Tiles tiles, SysState *st, double *part, const int *__restrict__ ptr,
+                  const int *__restrict__ col, const TV *__restrict__ val,
+                  const double *__restrict__ x, const TU *__restrict__ u, double *__restrict__ y,
+                  int slot, int gate) {
     __shared__ double lds[kTileThreads / kWave];
-    const int t = blockIdx.x, s = tiles.sys[t];
-    SysState &st = a.st[s];
-    const bool gate = what == kNormCheckOrig || what == kNormCheckRel ? (st.active && st.triggered)
-                                                                       : st.active;
-    if (!gate)
+    BSPAI_SYS_TILES
+    const SysState &S = st[s];
+    if (!S.active || (gate == 1 && !S.triggered))
+        return;
+    double vals[kTilePer];
+    int cnt = 0;
+    for (int row = tiles.row0[t] + threadIdx.x; row < tiles.row1[t]; row += kTileThreads) {
+        double acc = 0;
+        for (int e = ptr[row]; e < ptr[row + 1]; ++e)
+            acc += (double)val[e] * x[col[e]];
+        if (y)
+            y[row] = acc;
+        vals[cnt++] = (u ? (double)u[row] : 0.0) - acc;
+    }
+    store_norm_partial(vals, cnt, slot_ptr(part, tiles.ntiles, slot), t, lds);
+}
+
+// Norm partial of (u - v) into `slot` (v optional); optional store of the difference.
+template <typename TU, typename TV, typename TO>
+__global__ void __launch_bounds__(kTileThreads)
+diff_partial_kernel(Tiles tiles, SysState *st, double *part, const TU *__restrict__ u,
+                    const TV *__restrict__ v, TO *__restrict__ out, int slot) {
+    __shared__ double lds[kTileThreads / kWave];
+    BSPAI_SYS_TILES
+    if (!st[s].active)
         return;
     double vals[kTilePer];
     int cnt = 0;
@@ -266,44 +324,63 @@ diff_norm_kernel(Tiles tiles, KrylovArgs<T> a, const TU *u, const TV *v, double 
         const double d = (double)u[r] - (v ? (double)v[r] : 0.0);
         vals[cnt++] = d;
         if (out)
-            out[r] = d;
+            out[r] = (TO)d;
     }
-    tile_norm_partial(vals, cnt, a.part, t, lds);
-    if (!last_tile(a.cnt, s, tiles.sys_tile1[s] - tiles.sys_tile0[s]))
+    store_norm_partial(vals, cnt, slot_ptr(part, tiles.ntiles, slot), t, lds);
+}
+
+// Per-system decisions; one block per system.
+//   phase kPhaseInit: slots kSlotRhs, kSlotOrig -> zero_rhs / exact_initial
+//   phase kPhaseDen:  slot kSlotDen -> zero_preconditioned_rhs
+//   phase kPhaseBeta: slot kSlotBeta -> g_0, init_rel <= tau
+//   phase kPhaseCheck: slots kSlotChkOrig, kSlotChkRel -> accept / fail / go on
+template <typename T>
+__global__ void __launch_bounds__(kTileThreads)
+decide_kernel(Tiles tiles, KrylovArgs<T> a, int phase, int right) {
+    __shared__ double lds[kTileThreads / kWave];
+    const int s = blockIdx.x;
+    SysState &st = a.st[s];
+    if (!st.active || (phase == kPhaseCheck && !st.triggered))
         return;
-    __threadfence();
-    const double nrm = combine_norm(a.part, tiles.sys_tile0[s], tiles.sys_tile1[s]);
-    if (!isfinite(nrm)) {
-        st.status = kGmNumericalFailure;
-        st.active = 0;
-        st.triggered = 0;
-        return;
-    }
-    switch (what) {
-    case kNormInitOrig: // ||rhs - A x0||
-        st.init_orig = st.final_orig = nrm;
-        if (nrm == 0) {
-            st.status = st.rhs_zero ? kGmZeroRhs : kGmExactInitial;
+    const int t0 = tiles.sys_tile0[s], t1 = tiles.sys_tile1[s];
+    const int nt = tiles.ntiles;
+    auto norm_of = [&](int slot) { return reduce_norm(slot_ptr(a.part, nt, slot), t0, t1, lds); };
+    if (phase == kPhaseInit) {
+        const double rn = norm_of(kSlotRhs), on = norm_of(kSlotOrig);
+        if (threadIdx.x)
+            return;
+        if (!isfinite(on) || !isfinite(rn)) {
+            st.status = kGmNumericalFailure;
+            st.active = 0;
+            return;
+        }
+        st.init_orig = st.final_orig = on;
+        if (on == 0) {
+            st.status = rn == 0 ? kGmZeroRhs : kGmExactInitial;
             st.converged = 1;
             st.init_rel = st.final_rel = 0;
             st.active = 0;
         }
-        break;
-    case kNormRhs:
-        st.rhs_zero = nrm == 0;
-        break;
-    case kNormDen: // ||P rhs||
-        st.den = nrm;
-        if (nrm == 0) {
+    } else if (phase == kPhaseDen) {
+        const double dn = norm_of(kSlotDen);
+        if (threadIdx.x)
+            return;
+        st.den = dn;
+        if (!isfinite(dn)) {
+            st.status = kGmNumericalFailure;
+            st.active = 0;
+        } else if (dn == 0) {
             st.status = kGmZeroPreconditionedRhs;
             st.active = 0;
         }
-        break;
-    case kNormBeta: { // ||P rhs - P A x0||
-        st.beta = nrm;
-        st.hnorm = nrm;
-        st.init_rel = st.final_rel = nrm / st.den;
-        a.g[(std::int64_t)s * (a.cap + 1)] = nrm;
+    } else if (phase == kPhaseBeta) {
+        const double bn = norm_of(kSlotBeta);
+        if (threadIdx.x)
+            return;
+        st.beta = bn;
+        st.hnorm = bn;
+        a.g[(std::int64_t)s * (a.cap + 1)] = bn;
+        st.init_rel = st.final_rel = bn / st.den;
         if (!isfinite(st.init_rel)) {
             st.status = kGmNumericalFailure;
             st.active = 0;
@@ -311,21 +388,21 @@ diff_norm_kernel(Tiles tiles, KrylovArgs<T> a, const TU *u, const TV *v, double 
             st.status = kGmSuccess;
             st.converged = 1;
             st.active = 0;
-            st.x0_is_solution = 1;
         }
-        break;
-    }
-    case kNormCheckOrig:
-        st.final_orig = nrm;
-        break;
-    case kNormCheckRel: { // accept decision (src/gmres.cpp:218-253)
-        const double rel = nrm / st.den;
+    } else { // kPhaseCheck (src/gmres.cpp:218-253)
+        const double on = norm_of(kSlotChkOrig);
+        const double rn = right ? on : norm_of(kSlotChkRel);
+        if (threadIdx.x)
+            return;
+        const double rel = rn / st.den;
+        st.final_orig = on;
         st.final_rel = rel;
         st.triggered = 0;
-        if (!isfinite(rel)) {
+        st.has_solution = 1;
+        if (!isfinite(rel) || !isfinite(on)) {
             st.status = kGmNumericalFailure;
             st.active = 0;
-            break;
+            return;
         }
         const bool ok = rel <= st.tol;
         if (st.zero_pivot) {
@@ -343,15 +420,10 @@ diff_norm_kernel(Tiles tiles, KrylovArgs<T> a, const TU *u, const TV *v, double 
             st.status = a.cap < st.nrows ? kGmIterationLimit : kGmNonconvergence;
             st.active = 0;
         }
-        st.has_solution = 1;
-        break;
-    }
-    default:
-        break;
     }
 }
 
-// Normalize the start vector: V_0 = r0 / beta.
+// V_0 = r0 / beta.
 template <typename T>
 __global__ void __launch_bounds__(kTileThreads)
 start_vector_kernel(Tiles tiles, KrylovArgs<T> a, const T *r0) {
@@ -365,8 +437,7 @@ start_vector_kernel(Tiles tiles, KrylovArgs<T> a, const T *r0) {
         v[r] = r0[r] / beta;
 }
 
-// Systems whose GMRES finished at x0 (init_rel <= tau, zero rhs) or never
-// formed a candidate return x0.
+// Systems that ended without forming a candidate return x0 (or 0).
 template <typename T>
 __global__ void __launch_bounds__(kTileThreads)
 finish_kernel(Tiles tiles, KrylovArgs<T> a) {
@@ -378,88 +449,137 @@ finish_kernel(Tiles tiles, KrylovArgs<T> a) {
         a.x[r] = a.x0 ? a.x0[r] : 0.0;
 }
 
+template <typename T>
+__global__ void gmres_reset_kernel(int nsys, KrylovArgs<T> k, const IrSysState *ir,
+                                   const int *nrows, double tol) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nsys)
+        return;
+    SysState st = {};
+    st.tol = tol;
+    st.nrows = nrows[s];
+    st.participating = ir ? ir[s].active : 1;
+    st.active = st.participating;
+    st.init_orig = st.init_rel = st.final_orig = st.final_rel = __builtin_nan("");
+    k.st[s] = st;
+}
+
 // ------------------------------------------------------------------ launchers
-template <typename T> void gmres_mgs_round(hipStream_t s, const Tiles &tl, const KrylovArgs<T> &a, int j, int k) {
-    hipLaunchKernelGGL(mgs_round_kernel<T>, dim3(tl.ntiles), dim3(kTileThreads), 0, s, tl, a, j, k);
+#define BSPAI_GRID dim3(tl.ntiles), dim3(kTileThreads), 0, s
+
+template <typename T>
+void gmres_spmv_dot0(hipStream_t s, int lanes, const Tiles &tl, const KrylovArgs<T> &a,
+                     const int *ptr, const int *col, const T *val, const T *x) {
+#define BSPAI_D0(L)                                                                               \
+    case L:                                                                                      \
+        hipLaunchKernelGGL((spmv_dot0_kernel<L, T>), BSPAI_GRID, tl, a, ptr, col, val, x);       \
+        break;
+    switch (lanes) {
+        BSPAI_D0(1)
+        BSPAI_D0(2)
+        BSPAI_D0(4)
+        BSPAI_D0(8)
+        BSPAI_D0(16)
+        BSPAI_D0(32)
+        BSPAI_D0(64)
+    default:
+        throw std::invalid_argument("lanes must be a power of two <= 64");
+    }
+#undef BSPAI_D0
     HIP_LAUNCH_CHECK();
 }
-template <typename T> void gmres_normalize(hipStream_t s, const Tiles &tl, const KrylovArgs<T> &a, int k) {
-    hipLaunchKernelGGL(normalize_kernel<T>, dim3(tl.ntiles), dim3(kTileThreads), 0, s, tl, a, k);
+template <typename T>
+void gmres_mgs_round(hipStream_t s, const Tiles &tl, const KrylovArgs<T> &a, int j, int k) {
+    hipLaunchKernelGGL(mgs_round_kernel<T>, BSPAI_GRID, tl, a, j, k);
+    HIP_LAUNCH_CHECK();
+}
+template <typename T>
+void gmres_normalize_givens(hipStream_t s, const Tiles &tl, const KrylovArgs<T> &a, int k) {
+    hipLaunchKernelGGL(normalize_givens_kernel<T>, BSPAI_GRID, tl, a, k);
     HIP_LAUNCH_CHECK();
 }
 template <typename T> void gmres_candidate(hipStream_t s, const Tiles &tl, const KrylovArgs<T> &a) {
-    hipLaunchKernelGGL(candidate_kernel<T>, dim3(tl.ntiles), dim3(kTileThreads), 0, s, tl, a);
+    hipLaunchKernelGGL(candidate_kernel<T>, BSPAI_GRID, tl, a);
     HIP_LAUNCH_CHECK();
 }
-template <typename T> void gmres_start_vector(hipStream_t s, const Tiles &tl, const KrylovArgs<T> &a, const T *r0) {
-    hipLaunchKernelGGL(start_vector_kernel<T>, dim3(tl.ntiles), dim3(kTileThreads), 0, s, tl, a, r0);
+template <typename T>
+void gmres_start_vector(hipStream_t s, const Tiles &tl, const KrylovArgs<T> &a, const T *r0) {
+    hipLaunchKernelGGL(start_vector_kernel<T>, BSPAI_GRID, tl, a, r0);
     HIP_LAUNCH_CHECK();
 }
 template <typename T> void gmres_finish(hipStream_t s, const Tiles &tl, const KrylovArgs<T> &a) {
-    hipLaunchKernelGGL(finish_kernel<T>, dim3(tl.ntiles), dim3(kTileThreads), 0, s, tl, a);
+    hipLaunchKernelGGL(finish_kernel<T>, BSPAI_GRID, tl, a);
     HIP_LAUNCH_CHECK();
 }
-template <typename TU, typename TV, typename T>
-void gmres_diff_norm(hipStream_t s, const Tiles &tl, const KrylovArgs<T> &a, const TU *u,
-                     const TV *v, double *out, int what) {
-    hipLaunchKernelGGL((diff_norm_kernel<TU, TV, T>), dim3(tl.ntiles), dim3(kTileThreads), 0, s,
-                       tl, a, u, v, out, what);
+template <typename T>
+void gmres_decide(hipStream_t s, const Tiles &tl, const KrylovArgs<T> &a, int phase, int right) {
+    hipLaunchKernelGGL(decide_kernel<T>, dim3(tl.nsys), dim3(kTileThreads), 0, s, tl, a, phase,
+                       right);
+    HIP_LAUNCH_CHECK();
+}
+template <typename T>
+void gmres_reset(hipStream_t s, int nsys, const KrylovArgs<T> &k, const IrSysState *ir,
+                 const int *nrows, double tol) {
+    hipLaunchKernelGGL(gmres_reset_kernel<T>, dim3(ceil_div(nsys, 64)), dim3(64), 0, s, nsys, k,
+                       ir, nrows, tol);
+    HIP_LAUNCH_CHECK();
+}
+template <typename TV, typename TU>
+void gmres_spmv_resid(hipStream_t s, const Tiles &tl, SysState *st, double *part, const int *ptr,
+                      const int *col, const TV *val, const double *x, const TU *u, double *y,
+                      int slot, int gate) {
+    hipLaunchKernelGGL((spmv_resid_kernel<TV, TU>), BSPAI_GRID, tl, st, part, ptr, col, val, x, u,
+                       y, slot, gate);
+    HIP_LAUNCH_CHECK();
+}
+template <typename TU, typename TV, typename TO>
+void gmres_diff_partial(hipStream_t s, const Tiles &tl, SysState *st, double *part, const TU *u,
+                        const TV *v, TO *out, int slot) {
+    hipLaunchKernelGGL((diff_partial_kernel<TU, TV, TO>), BSPAI_GRID, tl, st, part, u, v, out,
+                       slot);
     HIP_LAUNCH_CHECK();
 }
 
 #define BSPAI_KRYLOV_INST(T)                                                                      \
+    template void gmres_spmv_dot0<T>(hipStream_t, int, const Tiles &, const KrylovArgs<T> &,      \
+                                     const int *, const int *, const T *, const T *);             \
     template void gmres_mgs_round<T>(hipStream_t, const Tiles &, const KrylovArgs<T> &, int, int); \
-    template void gmres_normalize<T>(hipStream_t, const Tiles &, const KrylovArgs<T> &, int);     \
+    template void gmres_normalize_givens<T>(hipStream_t, const Tiles &, const KrylovArgs<T> &,    \
+                                            int);                                                 \
     template void gmres_candidate<T>(hipStream_t, const Tiles &, const KrylovArgs<T> &);          \
     template void gmres_start_vector<T>(hipStream_t, const Tiles &, const KrylovArgs<T> &,        \
                                         const T *);                                               \
     template void gmres_finish<T>(hipStream_t, const Tiles &, const KrylovArgs<T> &);             \
-    template void gmres_diff_norm<T, T, T>(hipStream_t, const Tiles &, const KrylovArgs<T> &,     \
-                                           const T *, const T *, double *, int);
+    template void gmres_decide<T>(hipStream_t, const Tiles &, const KrylovArgs<T> &, int, int);   \
+    template void gmres_reset<T>(hipStream_t, int, const KrylovArgs<T> &, const IrSysState *,     \
+                                 const int *, double);                                            \
+    template void gmres_spmv_resid<T, T>(hipStream_t, const Tiles &, SysState *, double *,        \
+                                         const int *, const int *, const T *, const double *,     \
+                                         const T *, double *, int, int);                          \
+    template void gmres_diff_partial<T, T, T>(hipStream_t, const Tiles &, SysState *, double *,   \
+                                              const T *, const T *, T *, int);
 BSPAI_KRYLOV_INST(double)
 BSPAI_KRYLOV_INST(float)
 #undef BSPAI_KRYLOV_INST
-template void gmres_diff_norm<float, double, float>(hipStream_t, const Tiles &,
-                                                    const KrylovArgs<float> &, const float *,
-                                                    const double *, double *, int);
+template void gmres_diff_partial<float, double, float>(hipStream_t, const Tiles &, SysState *,
+                                                       double *, const float *, const double *,
+                                                       float *, int);
+template void gmres_spmv_resid<float, double>(hipStream_t, const Tiles &, SysState *, double *,
+                                              const int *, const int *, const float *,
+                                              const double *, const double *, double *, int, int);
 
 // ================================================================== IR (FP64)
-// ||A||_inf per system: max row sum of |a_ij| (src/ir.cpp:118-125).
-__global__ void __launch_bounds__(kTileThreads)
-ainf_kernel(Tiles tiles, const int *__restrict__ ptr, const double *__restrict__ val,
-            IrArgs a) {
-    __shared__ double lds[kTileThreads / kWave];
-    const int t = blockIdx.x, s = tiles.sys[t];
-    double mx = 0;
-    for (int r = tiles.row0[t] + threadIdx.x; r < tiles.row1[t]; r += kTileThreads) {
-        double rs = 0;
-        for (int e = ptr[r]; e < ptr[r + 1]; ++e)
-            rs = rs + fabs(val[e]);
-        mx = fmax(mx, rs);
-    }
-    mx = block_max(mx, lds);
-    if (threadIdx.x == 0)
-        a.part[4 * t] = mx;
-    if (!last_tile(a.cnt, s, tiles.sys_tile1[s] - tiles.sys_tile0[s]))
-        return;
-    __threadfence();
-    double m = 0;
-    for (int u = tiles.sys_tile0[s]; u < tiles.sys_tile1[s]; ++u)
-        m = fmax(m, a.part[4 * u]);
-    a.st[s].ainf = m;
-}
-
-// r = b - A x with the reference's sequential row sums (bit-identical), and
-// the diagnostics of src/ir.cpp:83-133: ||r||_inf, ||r||_2, normwise and
-// componentwise backward errors.  Finalizes the outer-loop decision.
+// Per-tile statistics of r = b - A x (sequential row sums, bit-identical to
+// the reference) and of the diagnostics of src/ir.cpp:83-133:
+//   part4[4t..4t+3] = max |r|, max |x|, max |b|, max componentwise ratio
+//   norm partial of r in slot kIrSlotR2, nonfinite flag in flags[2t].
 __global__ void __launch_bounds__(kTileThreads)
 diagnostics_kernel(Tiles tiles, const int *__restrict__ ptr, const int *__restrict__ col,
                    const double *__restrict__ val, const double *__restrict__ b,
-                   const double *__restrict__ x, double *__restrict__ r, IrArgs a, int step) {
+                   const double *__restrict__ x, double *__restrict__ r, IrArgs a) {
     __shared__ double lds[kTileThreads / kWave];
-    const int t = blockIdx.x, s = tiles.sys[t];
-    IrSysState &st = a.st[s];
-    if (!st.active)
+    BSPAI_SYS_TILES
+    if (!a.st[s].active)
         return;
     double rinf = 0, xinf = 0, binf = 0, cbe = 0;
     double vals[kTilePer];
@@ -487,27 +607,81 @@ diagnostics_kernel(Tiles tiles, const int *__restrict__ ptr, const int *__restri
     binf = block_max(binf, lds);
     cbe = block_max(cbe, lds);
     const double badv = block_max(bad ? 1.0 : 0.0, lds);
-    double *p4 = a.part + 4 * (std::int64_t)t;
     if (threadIdx.x == 0) {
+        double *p4 = a.part4 + 4 * (std::int64_t)t;
         p4[0] = rinf;
         p4[1] = xinf;
         p4[2] = binf;
         p4[3] = cbe;
-        a.part2[2 * t + 0] = badv;
+        a.flags[2 * t] = badv;
     }
-    tile_norm_partial(vals, cnt, a.part3, t, lds);
-    if (!last_tile(a.cnt, s, tiles.sys_tile1[s] - tiles.sys_tile0[s]))
+    store_norm_partial(vals, cnt, slot_ptr(a.part, tiles.ntiles, kIrSlotR2), t, lds);
+}
+
+// ||A||_inf partials (row sums of |a_ij| in CSR order, src/ir.cpp:118-125)
+// and ||b||_2 partials.
+__global__ void __launch_bounds__(kTileThreads)
+setup_partials_kernel(Tiles tiles, const int *__restrict__ ptr, const double *__restrict__ val,
+                      const double *__restrict__ b, IrArgs a) {
+    __shared__ double lds[kTileThreads / kWave];
+    BSPAI_SYS_TILES
+    double mx = 0;
+    double vals[kTilePer];
+    int cnt = 0;
+    for (int r = tiles.row0[t] + threadIdx.x; r < tiles.row1[t]; r += kTileThreads) {
+        double rs = 0;
+        for (int e = ptr[r]; e < ptr[r + 1]; ++e)
+            rs = rs + fabs(val[e]);
+        mx = fmax(mx, rs);
+        vals[cnt++] = b[r];
+    }
+    mx = block_max(mx, lds);
+    if (threadIdx.x == 0)
+        a.part4[4 * (std::int64_t)t] = mx;
+    store_norm_partial(vals, cnt, slot_ptr(a.part, tiles.ntiles, kIrSlotB), t, lds);
+}
+
+__global__ void __launch_bounds__(kTileThreads)
+setup_decide_kernel(Tiles tiles, IrArgs a) {
+    __shared__ double lds[kTileThreads / kWave];
+    const int s = blockIdx.x;
+    const int t0 = tiles.sys_tile0[s], t1 = tiles.sys_tile1[s];
+    const double ainf = reduce_max(a.part4, t0, t1, lds, 4, 0);
+    const double bn = reduce_norm(slot_ptr(a.part, tiles.ntiles, kIrSlotB), t0, t1, lds);
+    if (threadIdx.x == 0) {
+        a.st[s].ainf = ainf;
+        a.st[s].bnorm = bn;
+    }
+}
+
+// Outer decision after the diagnostics of iterate `step` (src/ir.cpp:150-191);
+// one block per system.
+__global__ void __launch_bounds__(kTileThreads)
+ir_decide_kernel(Tiles tiles, IrArgs a, int step) {
+    __shared__ double lds[kTileThreads / kWave];
+    const int s = blockIdx.x;
+    IrSysState &st = a.st[s];
+    if (!st.active)
         return;
-    __threadfence();
-    double R = 0, X = 0, B = 0, C = 0, BAD = 0;
-    for (int u = tiles.sys_tile0[s]; u < tiles.sys_tile1[s]; ++u) {
-        R = fmax(R, a.part[4 * u]);
-        X = fmax(X, a.part[4 * u + 1]);
-        B = fmax(B, a.part[4 * u + 2]);
-        C = fmax(C, a.part[4 * u + 3]);
-        BAD = fmax(BAD, a.part2[2 * u]);
+    const int t0 = tiles.sys_tile0[s], t1 = tiles.sys_tile1[s];
+    const double R = reduce_max(a.part4, t0, t1, lds, 4, 0);
+    const double X = reduce_max(a.part4, t0, t1, lds, 4, 1);
+    const double B = reduce_max(a.part4, t0, t1, lds, 4, 2);
+    const double C = reduce_max(a.part4, t0, t1, lds, 4, 3);
+    const double BAD = reduce_max(a.flags, t0, t1, lds, 2, 0);
+    const double CH = step > 0 ? reduce_max(a.flags, t0, t1, lds, 2, 1) : 1.0;
+    const double BADU = step > 0 ? reduce_max(a.upd_bad, t0, t1, lds, 1, 0) : 0.0;
+    const double r2 = reduce_norm(slot_ptr(a.part, tiles.ntiles, kIrSlotR2), t0, t1, lds);
+    if (threadIdx.x)
+        return;
+    if (step > 0)
+        ++st.outer_steps;
+    if (BADU != 0) {
+        st.status = kIrNumericalFailure;
+        st.active = 0;
+        st.done = 1;
+        return;
     }
-    const double r2 = combine_norm(a.part3, tiles.sys_tile0[s], tiles.sys_tile1[s]);
     const double den = st.ainf * X + B;
     const double nbe = den == 0 ? (R == 0 ? 0.0 : INFINITY) : R / den;
     st.rinf = R;
@@ -529,41 +703,26 @@ diagnostics_kernel(Tiles tiles, const int *__restrict__ ptr, const int *__restri
         h[3] = C;
     }
     st.history_len = step + 1;
-    const bool conv = nbe <= a.be_tol && (a.relres_target <= 0 || r2 <= a.relres_target * st.bnorm);
+    const bool conv =
+        nbe <= a.be_tol && (a.relres_target <= 0 || r2 <= a.relres_target * st.bnorm);
     if (conv) {
         st.status = kIrSuccess;
         st.converged = 1;
         st.active = 0;
         st.done = 1;
-    } else if (st.unchanged) {
+    } else if (CH == 0) {
         st.status = kIrStagnation;
         st.active = 0;
         st.done = 1;
     }
 }
 
-// ||b||_2 per system (robust), for the relative-residual stop.
-__global__ void __launch_bounds__(kTileThreads)
-bnorm_kernel(Tiles tiles, const double *__restrict__ b, IrArgs a) {
-    __shared__ double lds[kTileThreads / kWave];
-    const int t = blockIdx.x, s = tiles.sys[t];
-    double vals[kTilePer];
-    int cnt = 0;
-    for (int r = tiles.row0[t] + threadIdx.x; r < tiles.row1[t]; r += kTileThreads)
-        vals[cnt++] = b[r];
-    tile_norm_partial(vals, cnt, a.part3, t, lds);
-    if (!last_tile(a.cnt, s, tiles.sys_tile1[s] - tiles.sys_tile0[s]))
-        return;
-    __threadfence();
-    a.st[s].bnorm = combine_norm(a.part3, tiles.sys_tile0[s], tiles.sys_tile1[s]);
-}
-
-// x += d for systems whose inner solve converged; records whether x changed.
+// x += d for systems whose inner solve converged; per-tile "x changed" flag.
 __global__ void __launch_bounds__(kTileThreads)
 update_kernel(Tiles tiles, double *__restrict__ x, const double *__restrict__ d, IrArgs a) {
     __shared__ double lds[kTileThreads / kWave];
-    const int t = blockIdx.x, s = tiles.sys[t];
-    IrSysState &st = a.st[s];
+    BSPAI_SYS_TILES
+    const IrSysState &st = a.st[s];
     if (!st.active)
         return;
     double changed = 0;
@@ -579,23 +738,8 @@ update_kernel(Tiles tiles, double *__restrict__ x, const double *__restrict__ d,
     changed = block_max(changed, lds);
     const double badv = block_max(bad ? 1.0 : 0.0, lds);
     if (threadIdx.x == 0) {
-        a.part[4 * t] = changed;
-        a.part[4 * t + 1] = badv;
-    }
-    if (!last_tile(a.cnt, s, tiles.sys_tile1[s] - tiles.sys_tile0[s]))
-        return;
-    __threadfence();
-    double ch = 0, bd = 0;
-    for (int u = tiles.sys_tile0[s]; u < tiles.sys_tile1[s]; ++u) {
-        ch = fmax(ch, a.part[4 * u]);
-        bd = fmax(bd, a.part[4 * u + 1]);
-    }
-    st.unchanged = ch == 0;
-    ++st.outer_steps;
-    if (bd != 0) {
-        st.status = kIrNumericalFailure;
-        st.active = 0;
-        st.done = 1;
+        a.flags[2 * t + 1] = changed;
+        a.upd_bad[t] = badv;
     }
 }
 
@@ -604,7 +748,7 @@ update_kernel(Tiles tiles, double *__restrict__ x, const double *__restrict__ d,
 __global__ void __launch_bounds__(kTileThreads)
 scale_rhs_kernel(Tiles tiles, const double *__restrict__ r, float *__restrict__ rf, IrArgs a) {
     const int t = blockIdx.x, s = tiles.sys[t];
-    IrSysState &st = a.st[s];
+    const IrSysState &st = a.st[s];
     if (!st.active)
         return;
     int e = 0;
@@ -613,13 +757,11 @@ scale_rhs_kernel(Tiles tiles, const double *__restrict__ r, float *__restrict__ 
     const double sc = ldexp(1.0, -e);
     for (int row = tiles.row0[t] + threadIdx.x; row < tiles.row1[t]; row += kTileThreads)
         rf[row] = (float)(r[row] * sc);
-    if (t == tiles.sys_tile0[s] && threadIdx.x == 0)
-        st.pending_dscale = ldexp(1.0, e);
 }
 
-// Copy per-system inner-solve outcome into the IR state (one thread per system).
+// Absorb the inner outcome into the IR state (one thread per system).
 template <typename T>
-__global__ void absorb_inner_kernel(int nsys, KrylovArgs<T> k, IrArgs a) {
+__global__ void absorb_inner_kernel(int nsys, KrylovArgs<T> k, IrArgs a, int scaled) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= nsys)
         return;
@@ -634,7 +776,10 @@ __global__ void absorb_inner_kernel(int nsys, KrylovArgs<T> k, IrArgs a) {
         rec[2] = g.iterations;
     }
     ++st.inner_attempts;
-    st.dscale = st.pending_dscale;
+    int e = 0;
+    if (scaled && st.rinf > 0)
+        frexp(st.rinf, &e);
+    st.dscale = scaled ? ldexp(1.0, e) : 1.0;
     if (!g.converged) {
         st.status = kIrInnerFailure;
         st.inner_status = g.status;
@@ -643,64 +788,67 @@ __global__ void absorb_inner_kernel(int nsys, KrylovArgs<T> k, IrArgs a) {
     }
 }
 
-// Reset GMRES per-system state for a new solve; systems take part iff their
-// IR state is active (or all, when ir == nullptr).
-template <typename T>
-__global__ void gmres_reset_kernel(int nsys, KrylovArgs<T> k, const IrSysState *ir,
-                                   const int *nrows, double tol) {
+__global__ void ir_reset_kernel(int nsys, IrSysState *st) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= nsys)
         return;
-    SysState st = {};
-    st.tol = tol;
-    st.nrows = nrows[s];
-    st.participating = ir ? ir[s].active : 1;
-    st.active = st.participating;
-    st.init_orig = st.init_rel = st.final_orig = st.final_rel = __builtin_nan("");
-    k.st[s] = st;
+    IrSysState z = {};
+    z.active = 1;
+    z.dscale = 1.0;
+    z.status = kIrIterationLimit;
+    st[s] = z;
 }
 
-void ir_ainf(hipStream_t s, const Tiles &tl, const int *ptr, const double *val, const IrArgs &a) {
-    hipLaunchKernelGGL(ainf_kernel, dim3(tl.ntiles), dim3(kTileThreads), 0, s, tl, ptr, val, a);
-    HIP_LAUNCH_CHECK();
+__global__ void ir_cap_kernel(int nsys, IrSysState *st) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nsys)
+        return;
+    if (st[s].active) {
+        st[s].status = kIrIterationLimit;
+        st[s].active = 0;
+        st[s].done = 1;
+    }
 }
-void ir_bnorm(hipStream_t s, const Tiles &tl, const double *b, const IrArgs &a) {
-    hipLaunchKernelGGL(bnorm_kernel, dim3(tl.ntiles), dim3(kTileThreads), 0, s, tl, b, a);
+
+void ir_setup(hipStream_t s, const Tiles &tl, const int *ptr, const double *val, const double *b,
+              const IrArgs &a) {
+    hipLaunchKernelGGL(ir_reset_kernel, dim3(ceil_div(tl.nsys, 64)), dim3(64), 0, s, tl.nsys,
+                       a.st);
+    HIP_LAUNCH_CHECK();
+    hipLaunchKernelGGL(setup_partials_kernel, BSPAI_GRID, tl, ptr, val, b, a);
+    HIP_LAUNCH_CHECK();
+    hipLaunchKernelGGL(setup_decide_kernel, dim3(tl.nsys), dim3(kTileThreads), 0, s, tl, a);
     HIP_LAUNCH_CHECK();
 }
 void ir_diagnostics(hipStream_t s, const Tiles &tl, const int *ptr, const int *col,
                     const double *val, const double *b, const double *x, double *r,
                     const IrArgs &a, int step) {
-    hipLaunchKernelGGL(diagnostics_kernel, dim3(tl.ntiles), dim3(kTileThreads), 0, s, tl, ptr,
-                       col, val, b, x, r, a, step);
+    hipLaunchKernelGGL(diagnostics_kernel, BSPAI_GRID, tl, ptr, col, val, b, x, r, a);
+    HIP_LAUNCH_CHECK();
+    hipLaunchKernelGGL(ir_decide_kernel, dim3(tl.nsys), dim3(kTileThreads), 0, s, tl, a, step);
     HIP_LAUNCH_CHECK();
 }
 void ir_update(hipStream_t s, const Tiles &tl, double *x, const double *d, const IrArgs &a) {
-    hipLaunchKernelGGL(update_kernel, dim3(tl.ntiles), dim3(kTileThreads), 0, s, tl, x, d, a);
+    hipLaunchKernelGGL(update_kernel, BSPAI_GRID, tl, x, d, a);
     HIP_LAUNCH_CHECK();
 }
 void ir_scale_rhs(hipStream_t s, const Tiles &tl, const double *r, float *rf, const IrArgs &a) {
-    hipLaunchKernelGGL(scale_rhs_kernel, dim3(tl.ntiles), dim3(kTileThreads), 0, s, tl, r, rf, a);
+    hipLaunchKernelGGL(scale_rhs_kernel, BSPAI_GRID, tl, r, rf, a);
+    HIP_LAUNCH_CHECK();
+}
+void ir_cap(hipStream_t s, int nsys, IrSysState *st) {
+    hipLaunchKernelGGL(ir_cap_kernel, dim3(ceil_div(nsys, 64)), dim3(64), 0, s, nsys, st);
     HIP_LAUNCH_CHECK();
 }
 template <typename T>
-void ir_absorb_inner(hipStream_t s, int nsys, const KrylovArgs<T> &k, const IrArgs &a) {
+void ir_absorb_inner(hipStream_t s, int nsys, const KrylovArgs<T> &k, const IrArgs &a, int scaled) {
     hipLaunchKernelGGL(absorb_inner_kernel<T>, dim3(ceil_div(nsys, 64)), dim3(64), 0, s, nsys, k,
-                       a);
+                       a, scaled);
     HIP_LAUNCH_CHECK();
 }
-template <typename T>
-void gmres_reset(hipStream_t s, int nsys, const KrylovArgs<T> &k, const IrSysState *ir,
-                 const int *nrows, double tol) {
-    hipLaunchKernelGGL(gmres_reset_kernel<T>, dim3(ceil_div(nsys, 64)), dim3(64), 0, s, nsys, k,
-                       ir, nrows, tol);
-    HIP_LAUNCH_CHECK();
-}
-template void ir_absorb_inner<double>(hipStream_t, int, const KrylovArgs<double> &, const IrArgs &);
-template void ir_absorb_inner<float>(hipStream_t, int, const KrylovArgs<float> &, const IrArgs &);
-template void gmres_reset<double>(hipStream_t, int, const KrylovArgs<double> &, const IrSysState *,
-                                  const int *, double);
-template void gmres_reset<float>(hipStream_t, int, const KrylovArgs<float> &, const IrSysState *,
-                                 const int *, double);
+template void ir_absorb_inner<double>(hipStream_t, int, const KrylovArgs<double> &, const IrArgs &,
+                                      int);
+template void ir_absorb_inner<float>(hipStream_t, int, const KrylovArgs<float> &, const IrArgs &,
+                                     int);
 
 } // namespace bspai_hip

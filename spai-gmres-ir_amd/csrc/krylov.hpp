@@ -27,14 +27,20 @@ enum : int {
     kIrIterationLimit = 5,
     kIrNumericalFailure = 6,
 };
+// Partial-sum slots of the GMRES reductions (each 2 doubles per tile).
 enum : int {
-    kNormInitOrig = 0,
-    kNormRhs = 1,
-    kNormDen = 2,
-    kNormBeta = 3,
-    kNormCheckOrig = 4,
-    kNormCheckRel = 5,
+    kSlotMgs0 = 0, // Gram-Schmidt rounds alternate between slots 0 and 1
+    kSlotMgs1 = 1,
+    kSlotRhs = 2,
+    kSlotOrig = 3,
+    kSlotDen = 4,
+    kSlotBeta = 5,
+    kSlotChkOrig = 6,
+    kSlotChkRel = 7,
+    kNumSlots = 8,
 };
+enum : int { kPhaseInit = 0, kPhaseDen = 1, kPhaseBeta = 2, kPhaseCheck = 3 };
+enum : int { kIrSlotR2 = 0, kIrSlotB = 1, kIrNumSlots = 2 };
 
 constexpr int kMaxOuterHistory = 24; // >= outer cap 20 + initial
 
@@ -63,52 +69,62 @@ template <typename T> struct KrylovArgs {
     double *H, *cs, *sn, *g, *y;
     std::int64_t hstride;
     int cap;
-    double *part; // 2 per tile
-    int *cnt;     // per system
+    double *part; // kNumSlots slots x 2 doubles per tile
     double *x;    // candidate / solution (FP64)
     const double *x0;
 };
 
 struct IrSysState {
     double ainf, bnorm, rinf, r2, nbe, cbe, xinf;
-    double dscale, pending_dscale;
+    double dscale;
     int active, done, status, converged, outer_steps, inner_attempts, inner_status;
-    int history_len, unchanged;
+    int history_len;
 };
 
 struct IrArgs {
     IrSysState *st;
-    double *part;  // 4 per tile
-    double *part2; // 2 per tile
-    double *part3; // 2 per tile (norm partials)
-    int *cnt;
+    double *part;    // kIrNumSlots slots x 2 doubles per tile (norm partials)
+    double *part4;   // 4 per tile: max |r|, max |x|, max |b|, max cbe (or ||A||_inf)
+    double *flags;   // 2 per tile: nonfinite residual, x changed
+    double *upd_bad; // 1 per tile: nonfinite update
     double *history; // [nsys][kMaxOuterHistory][4]: rinf, r2, nbe, cbe
     int *inner;      // [nsys][kMaxOuterHistory][3]: status, converged, iterations
     double be_tol, relres_target;
 };
 
 template <typename T>
+void gmres_spmv_dot0(hipStream_t s, int lanes, const Tiles &tl, const KrylovArgs<T> &a,
+                     const int *ptr, const int *col, const T *val, const T *x);
+template <typename T>
 void gmres_mgs_round(hipStream_t s, const Tiles &tl, const KrylovArgs<T> &a, int j, int k);
-template <typename T> void gmres_normalize(hipStream_t s, const Tiles &tl, const KrylovArgs<T> &a, int k);
+template <typename T>
+void gmres_normalize_givens(hipStream_t s, const Tiles &tl, const KrylovArgs<T> &a, int k);
 template <typename T> void gmres_candidate(hipStream_t s, const Tiles &tl, const KrylovArgs<T> &a);
 template <typename T>
 void gmres_start_vector(hipStream_t s, const Tiles &tl, const KrylovArgs<T> &a, const T *r0);
 template <typename T> void gmres_finish(hipStream_t s, const Tiles &tl, const KrylovArgs<T> &a);
-template <typename TU, typename TV, typename T>
-void gmres_diff_norm(hipStream_t s, const Tiles &tl, const KrylovArgs<T> &a, const TU *u,
-                     const TV *v, double *out, int what);
+template <typename T>
+void gmres_decide(hipStream_t s, const Tiles &tl, const KrylovArgs<T> &a, int phase, int right);
 template <typename T>
 void gmres_reset(hipStream_t s, int nsys, const KrylovArgs<T> &k, const IrSysState *ir,
                  const int *nrows, double tol);
+template <typename TV, typename TU>
+void gmres_spmv_resid(hipStream_t s, const Tiles &tl, SysState *st, double *part, const int *ptr,
+                      const int *col, const TV *val, const double *x, const TU *u, double *y,
+                      int slot, int gate);
+template <typename TU, typename TV, typename TO>
+void gmres_diff_partial(hipStream_t s, const Tiles &tl, SysState *st, double *part, const TU *u,
+                        const TV *v, TO *out, int slot);
 
-void ir_ainf(hipStream_t s, const Tiles &tl, const int *ptr, const double *val, const IrArgs &a);
-void ir_bnorm(hipStream_t s, const Tiles &tl, const double *b, const IrArgs &a);
+void ir_setup(hipStream_t s, const Tiles &tl, const int *ptr, const double *val, const double *b,
+              const IrArgs &a);
 void ir_diagnostics(hipStream_t s, const Tiles &tl, const int *ptr, const int *col,
                     const double *val, const double *b, const double *x, double *r,
                     const IrArgs &a, int step);
 void ir_update(hipStream_t s, const Tiles &tl, double *x, const double *d, const IrArgs &a);
 void ir_scale_rhs(hipStream_t s, const Tiles &tl, const double *r, float *rf, const IrArgs &a);
+void ir_cap(hipStream_t s, int nsys, IrSysState *st);
 template <typename T>
-void ir_absorb_inner(hipStream_t s, int nsys, const KrylovArgs<T> &k, const IrArgs &a);
+void ir_absorb_inner(hipStream_t s, int nsys, const KrylovArgs<T> &k, const IrArgs &a, int scaled);
 
 } // namespace bspai_hip

@@ -213,8 +213,12 @@ def test_ir_c1_fp64_vs_reference(ctx, c1_golden):
     assert len(r.inner_iterations) == len(ref_it)
     assert all(abs(x - y) <= 1 for x, y in zip(r.inner_iterations, ref_it))
     assert r.normwise_backward_error <= 10 * 2.0 ** -53
-    # x0 = P b and its diagnostics are bit-identical (same P, sequential SpMV)
-    assert np.array_equal(bits(r.history[0]), bits(c1_golden["ir_history"][0]))
+    # x0 = P b and its diagnostics are bit-identical (same P, sequential SpMV):
+    # ||r||_inf, normwise and componentwise backward error.  ||r||_2 is a
+    # scaled sum of squares on the GPU (a hypot chain on the CPU): 1e-14.
+    h0, g0 = r.history[0], c1_golden["ir_history"][0]
+    assert np.array_equal(bits(h0[[0, 2, 3]]), bits(g0[[0, 2, 3]]))
+    assert abs(h0[1] - g0[1]) <= 1e-14 * g0[1]
     x = np.asarray(r.solution)
     xr = c1_golden["ir_solution"]
     assert np.max(np.abs(x - xr)) <= 1e-10 * np.max(np.abs(xr))
@@ -241,10 +245,13 @@ def test_ir_c1_mixed(ctx, c1_golden):
         if not np.array_equal(c32, c64):
             diff_rows += 1
             continue
-        rel.append(np.abs(v32 - v64) / np.abs(v64))
-    rel = np.concatenate(rel)
-    print(f"mixed SPAI: pattern-differing columns {diff_rows}/{a.n}, max rel {rel.max():.3e}")
-    assert np.percentile(rel, 99) <= 1e-5
+        # column k of M (= row k of P), normwise relative error
+        rel.append(np.max(np.abs(v32 - v64)) / np.max(np.abs(v64)))
+    rel = np.array(rel)
+    print(f"mixed SPAI: pattern-differing columns {diff_rows}/{a.n}, max column rel "
+          f"{rel.max():.3e}")
+    assert diff_rows <= a.n // 100
+    assert rel.max() <= 1e-5
     r = bspai_hip.spai_gmres_ir(m, b, precision="mixed", tolerance=1e-4, relres_target=1e-12)
     assert r.status == "success", r.message
     x = np.asarray(r.solution)
@@ -271,6 +278,43 @@ def test_batched_systems_match_single(ctx):
     for s, blk in enumerate(blocks):
         single = bspai_hip.spai_gmres_ir(bspai_hip.Matrix(blk, ctx), rhs[offs[s]:offs[s + 1]])
         assert res[s].status == single.status == "success"
-        assert res[s].inner_iterations == single.inner_iterations
+        assert all(abs(p - q) <= 1 for p, q in zip(res[s].inner_iterations,
+                                                   single.inner_iterations))
         xs = np.asarray(single.solution)
-        assert np.array_equal(bits(x[offs[s]:offs[s + 1]]), bits(xs))
+        xb = x[offs[s]:offs[s + 1]]
+        # P's lane count may differ between batch and single (average row
+        # length), which changes rounding only.
+        assert np.max(np.abs(xb - xs)) <= 1e-12 * np.max(np.abs(xs))
+
+
+def _seq_multiply(a, x):
+    """y = A x with sequential per-row sums (the reference's order)."""
+    y = np.zeros(a.n)
+    lens = np.diff(a.ptr)
+    for k in range(int(lens.max())):
+        rows = np.nonzero(lens > k)[0]
+        q = a.ptr[rows] + k
+        y[rows] = y[rows] + a.val[q] * x[a.col[q]]
+    return y
+
+
+def test_gmres_second_inner_solve_same_input(ctx):
+    """The second inner solve of the 64x64 IR, given the oracle's own residual
+    r1 = b - A x1: GPU and oracle iteration counts agree within 1."""
+    import bspai_hip
+    from oracle import pyoracle as O
+
+    P = _problems()
+    a = P.convection_diffusion_2d()
+    b = P.equal_rhs(a.n)
+    s = O.construct_spai(a)
+    p = P.Csr(a.n, s.p_ptr, s.p_col, s.p_val)
+    one = O.spai_gmres_ir(a, b, p=p, max_outer_steps=1)
+    r1 = b - _seq_multiply(a, one.solution)
+    ref = O.gmres(a, p, r1, 1e-8, 0)
+    g = bspai_hip.left_preconditioned_gmres(bspai_hip.Matrix(a, ctx), bspai_hip.Matrix(p, ctx),
+                                            r1, 1e-8, 0)
+    assert g.status == ref.status == "success"
+    assert abs(g.iterations - ref.iterations) <= 1
+    assert np.linalg.norm(np.asarray(g.solution) - ref.solution) <= 1e-6 * np.linalg.norm(
+        ref.solution)
