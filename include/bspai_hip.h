@@ -23,6 +23,9 @@
  *   bspai_hip_spai_gmres_ir                     cuda_gpu_spai_gmres_ir include/bspai/cuda_ir.hpp:26-28
  *                                               spai_gmres_ir     include/bspai/ir.hpp:65-66
  *   bspai_hip_adjoint_solve, bspai_hip_sddmm    no reference counterpart (differentiable solve)
+ *   bspai_hip_ctx_set_timing / _kernel_times    telemetry: the reference's GPU results carry
+ *                                               CudaSpaiTelemetry include/bspai/cuda_spai.hpp:9-37
+ *                                               and CudaIrResult counters include/bspai/cuda_ir.hpp:7-16
  */
 #ifndef BSPAI_HIP_H
 #define BSPAI_HIP_H
@@ -74,6 +77,7 @@ typedef struct {
     int64_t fits, growths, clamps;
     int passes;              /* capacity buckets used */
     double build_ms;         /* device time of the construction */
+    double qr_flops;         /* sum over accepted fits of 2mq^2 - 2q^3/3 + 4mq + q^2 */
     char message[256];
 } bspai_hip_spai_info;
 
@@ -119,6 +123,24 @@ int bspai_hip_ctx_create(int device, void *stream, bspai_hip_ctx **out);
 void bspai_hip_ctx_destroy(bspai_hip_ctx *ctx);
 const char *bspai_hip_last_error(const bspai_hip_ctx *ctx);
 int bspai_hip_synchronize(bspai_hip_ctx *ctx);
+
+/* Kernel timing telemetry.  While enabled, each launch of the hot kernels
+ * (SPAI columns, the two SpMVs and the Gram-Schmidt passes of every Arnoldi
+ * step) is bracketed by two HIP events on the context's stream.
+ * bspai_hip_ctx_kernel_times synchronizes and returns the totals per kernel
+ * label since timing was last enabled: it writes at most `capacity` entries
+ * and sets *count to the number of labels.  Enabling or disabling clears
+ * the totals. */
+typedef struct {
+    char name[96];       /* kernel name as rocprofv3 lists it, plus the operand role */
+    int64_t launches;
+    double total_ms;     /* sum of the event-measured durations */
+    double bytes;        /* algorithmic HBM bytes, summed over the launches */
+    double flops;        /* algorithmic flops, summed over the launches */
+} bspai_hip_kernel_time;
+int bspai_hip_ctx_set_timing(bspai_hip_ctx *ctx, int enable);
+int bspai_hip_ctx_kernel_times(bspai_hip_ctx *ctx, bspai_hip_kernel_time *out, int capacity,
+                               int *count);
 
 /* n x n CSR (canonical: sorted columns, no duplicates).  nsys > 1 declares a
  * block-diagonal batch of independent systems: system s owns rows

@@ -34,7 +34,12 @@ class SpaiInfo(C.Structure):
     _fields_ = [("status", C.c_int), ("failing_column", C.c_int64), ("n", C.c_int64),
                 ("p_nnz", C.c_int64), ("pattern_nnz", C.c_int64), ("fits", C.c_int64),
                 ("growths", C.c_int64), ("clamps", C.c_int64), ("passes", C.c_int),
-                ("build_ms", C.c_double), ("message", C.c_char * 256)]
+                ("build_ms", C.c_double), ("qr_flops", C.c_double), ("message", C.c_char * 256)]
+
+
+class KernelTime(C.Structure):
+    _fields_ = [("name", C.c_char * 96), ("launches", C.c_int64), ("total_ms", C.c_double),
+                ("bytes", C.c_double), ("flops", C.c_double)]
 
 
 class GmresOptions(C.Structure):
@@ -78,6 +83,8 @@ SIGNATURES = {
     "bspai_hip_ctx_destroy": (None, [_VP]),
     "bspai_hip_last_error": (C.c_char_p, [_VP]),
     "bspai_hip_synchronize": (C.c_int, [_VP]),
+    "bspai_hip_ctx_set_timing": (C.c_int, [_VP, C.c_int]),
+    "bspai_hip_ctx_kernel_times": (C.c_int, [_VP, _VP, C.c_int, C.POINTER(C.c_int)]),
     "bspai_hip_matrix_create": (C.c_int, [_VP, C.c_int64, C.c_int64, _VP, _VP, _VP, C.c_int,
                                           C.c_int, _VP, _PP]),
     "bspai_hip_matrix_destroy": (None, [_VP]),

@@ -32,9 +32,15 @@ def _addr(a):
 
 
 def _where(a):
+    """HOST or DEVICE.  A device tensor may still be being written by torch on
+    its current stream, which is not the library's stream in general: wait
+    for that stream before the library reads the tensor."""
     if a is None or isinstance(a, np.ndarray):
         return HOST
     if getattr(a, "is_cuda", False):
+        import torch
+
+        torch.cuda.current_stream(a.device).synchronize()
         return DEVICE
     return HOST
 
@@ -72,6 +78,27 @@ class Context:
 
     def synchronize(self):
         check(_lib.load().bspai_hip_synchronize(self.h), self.h)
+
+    def set_timing(self, enable: bool = True):
+        """Start (clearing earlier records) or stop per-kernel HIP-event timing."""
+        check(_lib.load().bspai_hip_ctx_set_timing(self.h, int(bool(enable))), self.h)
+
+    def kernel_times(self):
+        """Per-kernel totals since timing was enabled:
+        {label: dict(launches, total_ms, avg_us, bytes, flops)}."""
+        L = _lib.load()
+        count = C.c_int(0)
+        check(L.bspai_hip_ctx_kernel_times(self.h, None, 0, C.byref(count)), self.h)
+        buf = (_lib.KernelTime * max(1, count.value))()
+        got = C.c_int(0)
+        check(L.bspai_hip_ctx_kernel_times(self.h, buf, len(buf), C.byref(got)), self.h)
+        out = {}
+        for i in range(min(got.value, len(buf))):
+            k = buf[i]
+            out[k.name.decode()] = dict(launches=int(k.launches), total_ms=k.total_ms,
+                                        avg_us=1e3 * k.total_ms / max(1, k.launches),
+                                        bytes=k.bytes, flops=k.flops)
+        return out
 
 
 _default_ctx = None

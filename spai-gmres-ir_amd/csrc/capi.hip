@@ -297,6 +297,35 @@ int bspai_hip_synchronize(bspai_hip_ctx *ctx) {
     return guard(ctx, [&] { HIP_CHECK(hipStreamSynchronize(ctx->c.stream)); });
 }
 
+int bspai_hip_ctx_set_timing(bspai_hip_ctx *ctx, int enable) {
+    if (!ctx)
+        return BSPAI_HIP_EINVAL;
+    return guard(ctx, [&] {
+        HIP_CHECK(hipStreamSynchronize(ctx->c.stream));
+        ctx->c.timer.clear();
+        ctx->c.timer.on = enable != 0;
+    });
+}
+
+int bspai_hip_ctx_kernel_times(bspai_hip_ctx *ctx, bspai_hip_kernel_time *out, int capacity,
+                               int *count) {
+    if (!ctx || !count || capacity < 0 || (capacity > 0 && !out))
+        return BSPAI_HIP_EINVAL;
+    return guard(ctx, [&] {
+        const auto totals = ctx->c.timer.collect();
+        *count = (int)totals.size();
+        for (int i = 0; i < capacity && i < (int)totals.size(); ++i) {
+            const auto &t = totals[i];
+            std::memset(out[i].name, 0, sizeof(out[i].name));
+            std::strncpy(out[i].name, t.name.c_str(), sizeof(out[i].name) - 1);
+            out[i].launches = t.launches;
+            out[i].total_ms = t.ms;
+            out[i].bytes = t.bytes;
+            out[i].flops = t.flops;
+        }
+    });
+}
+
 int bspai_hip_matrix_create(bspai_hip_ctx *ctx, int64_t n, int64_t nnz, const int32_t *ptr,
                             const int32_t *col, const double *val, int where, int nsys,
                             const int64_t *sys_offsets, bspai_hip_matrix **out) {
@@ -480,6 +509,7 @@ int bspai_hip_spai_get_info(const bspai_hip_spai *p, bspai_hip_spai_info *info) 
     info->clamps = b.clamps;
     info->passes = b.passes;
     info->build_ms = b.build_ms;
+    info->qr_flops = b.qr_flops;
     set_msg(info->message, b.message);
     return BSPAI_HIP_OK;
 }
@@ -707,6 +737,7 @@ int bspai_hip_adjoint_solve(bspai_hip_ctx *ctx, const bspai_hip_matrix *at,
         auto res = ir_run(c, AT, PT, AT, kRight, gi.p, ir_opts(o), lo.p);
         HIP_CHECK(hipEventRecord(c.ev1, c.stream));
         lo.finish();
+        HIP_CHECK(hipEventSynchronize(c.ev1));
         float ms = 0;
         HIP_CHECK(hipEventElapsedTime(&ms, c.ev0, c.ev1));
         for (std::size_t s = 0; s < res.size(); ++s) {

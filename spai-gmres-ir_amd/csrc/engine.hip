@@ -5,6 +5,7 @@
 #include <climits>
 #include <cmath>
 #include <cstddef>
+#include <cstdlib>
 
 namespace bspai_hip {
 
@@ -21,6 +22,68 @@ void sync(const Context &c) {
     HIP_CHECK(hipStreamSynchronize(c.stream));
 }
 } // namespace
+
+// ---------------------------------------------------------------- timing
+// Timer labels: the kernel names rocprofv3 reports, plus the operand role.
+template <typename T> constexpr const char *kSpaiKernelName = "spai_columns_kernel<double>";
+template <> constexpr const char *kSpaiKernelName<float> = "spai_columns_kernel<float>";
+
+KernelTimer::~KernelTimer() {
+    for (hipEvent_t e : pool_)
+        (void)hipEventDestroy(e);
+}
+hipEvent_t KernelTimer::event(int i) {
+    while ((int)pool_.size() <= i) {
+        hipEvent_t e;
+        HIP_CHECK(hipEventCreate(&e));
+        pool_.push_back(e);
+    }
+    return pool_[i];
+}
+int KernelTimer::begin(hipStream_t s) {
+    const int first = used_;
+    HIP_CHECK(hipEventRecord(event(first), s));
+    used_ += 2;
+    return first;
+}
+void KernelTimer::end(hipStream_t s, int first, const char *name, double bytes, double flops) {
+    HIP_CHECK(hipEventRecord(event(first + 1), s));
+    recs_.push_back(Rec{name, first, bytes, flops});
+}
+void KernelTimer::add_flops(const char *name, double flops) {
+    for (auto it = recs_.rbegin(); it != recs_.rend(); ++it)
+        if (it->name == name) {
+            it->flops += flops;
+            return;
+        }
+}
+const std::vector<KernelTimer::Total> &KernelTimer::collect() {
+    std::vector<Total> &out = totals_;
+    for (const Rec &r : recs_) {
+        HIP_CHECK(hipEventSynchronize(pool_[r.first + 1]));
+        float ms = 0;
+        HIP_CHECK(hipEventElapsedTime(&ms, pool_[r.first], pool_[r.first + 1]));
+        auto it = std::find_if(out.begin(), out.end(),
+                               [&](const Total &t) { return t.name == r.name; });
+        if (it == out.end()) {
+            out.push_back(Total{r.name});
+            it = out.end() - 1;
+        }
+        it->launches += 1;
+        it->ms += ms;
+        it->bytes += r.bytes;
+        it->flops += r.flops;
+    }
+    recs_.clear();
+    used_ = 0;
+    return out;
+}
+
+std::string spmv_label(int lanes, const char *tname, const char *role) {
+    std::string k = lanes == 1 ? std::string("spmv_stream_kernel")
+                               : "spmv_group_kernel<" + std::to_string(lanes) + ">";
+    return k + "<" + tname + "> " + role;
+}
 
 // ---------------------------------------------------------------- layout
 void Layout::build(const Context &c, std::int64_t n_, const std::vector<std::int64_t> &offs) {
@@ -54,14 +117,14 @@ void Layout::build(const Context &c, std::int64_t n_, const std::vector<std::int
     sync(c);
 }
 
+// Streaming (lanes = 1, sequential row order) for short rows; lane groups
+// for long rows, where one thread per row would serialize.
 int choose_lanes(std::int64_t n, std::int64_t nnz) {
     const double avg = n ? (double)nnz / (double)n : 0;
-    if (avg <= 10)
+    if (avg <= 48)
         return 1;
-    if (avg <= 24)
-        return 4;
-    if (avg <= 64)
-        return 8;
+    if (avg <= 96)
+        return 16;
     return 32;
 }
 
@@ -104,17 +167,24 @@ __global__ void collect_overflow_kernel(int nwork, const int *work, const int *s
 }
 // Reduce per-column outcomes: first failing column, totals.
 __global__ void column_summary_kernel(int n, const int *status, const int *fits, const int *growths,
-                                      const int *clamps, const int *q, int *first_fail,
-                                      unsigned long long *tot) {
+                                      const int *clamps, const int *q, const double *flops,
+                                      int *first_fail, unsigned long long *tot, double *ftot) {
+    // Block sums first (exact: the counts stay far below 2^53), one atomic
+    // per block and counter.
+    __shared__ double lds[256 / kWave];
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= n)
-        return;
-    if (status[k] != kSpaiSuccess)
+    const bool in = k < n;
+    if (in && status[k] != kSpaiSuccess)
         atomicMin(first_fail, k);
-    atomicAdd(tot + 0, (unsigned long long)fits[k]);
-    atomicAdd(tot + 1, (unsigned long long)growths[k]);
-    atomicAdd(tot + 2, (unsigned long long)clamps[k]);
-    atomicAdd(tot + 3, (unsigned long long)q[k]);
+    const int *src[4] = {fits, growths, clamps, q};
+    for (int c = 0; c < 4; ++c) {
+        const double v = block_sum(in ? (double)src[c][k] : 0.0, lds);
+        if (threadIdx.x == 0)
+            atomicAdd(tot + c, (unsigned long long)v);
+    }
+    const double f = block_sum(in ? flops[k] : 0.0, lds);
+    if (threadIdx.x == 0)
+        atomicAdd(ftot, f); // telemetry only: summation order does not matter
 }
 // Column pool pointers: per column (pass, offset) -> the pools' base pointers.
 struct PoolTable {
@@ -238,6 +308,7 @@ void spai_build_typed(Context &c, const DevCsr &a, const T *aval, T *d, double e
     out.c_q.resize(n);
     out.c_res.resize(n);
     out.c_piv.resize(n);
+    out.c_flops.resize(n);
     out.c_off.resize(n);
     out.c_pass.resize(n);
     DeviceArray<int> work(n), next(n), counter(2);
@@ -246,7 +317,16 @@ void spai_build_typed(Context &c, const DevCsr &a, const T *aval, T *d, double e
     int mcap = 64, qcap = 16, ccap = 128;
     hipDeviceProp_t prop;
     HIP_CHECK(hipGetDeviceProperties(&prop, c.device));
-    const int max_waves = prop.multiProcessorCount * 8; // lanes in flight
+    // Columns in flight: the column kernel is latency-bound (dependent loads
+    // of the lane's work area), so occupancy pays.  Overridable for tuning.
+    int waves_per_cu = 32;
+    if (const char *env = std::getenv("BSPAI_SPAI_WAVES_PER_CU"))
+        waves_per_cu = std::max(1, std::atoi(env));
+    const int max_waves = prop.multiProcessorCount * waves_per_cu;
+    std::size_t free_bytes = 0, total_bytes = 0;
+    HIP_CHECK(hipMemGetInfo(&free_bytes, &total_bytes));
+    const std::int64_t area_cap =
+        std::min<std::int64_t>(8LL << 30, std::max<std::int64_t>(1LL << 28, free_bytes / 4));
     DeviceBuffer dwork, iwork;
     while (nwork > 0) {
         if (out.passes >= 8)
@@ -254,11 +334,12 @@ void spai_build_typed(Context &c, const DevCsr &a, const T *aval, T *d, double e
         const int pass = out.passes++;
         const std::int64_t ld = spai_lane_doubles(mcap, qcap, ccap);
         const std::int64_t li = spai_lane_ints(mcap, qcap, ccap);
-        // Lanes in flight: bounded by the device and by a 2 GiB work area.
+        // Lanes in flight: bounded by the device and by the work-area cap
+        // (8 GiB, or a quarter of free memory).
         const int threads = 256, waves_per_block = threads / kWave;
         std::int64_t waves = std::min<std::int64_t>(max_waves, (nwork + kWave - 1) / kWave);
         const std::int64_t per_wave = (ld * (std::int64_t)sizeof(T) + li * 4) * kWave;
-        waves = std::max<std::int64_t>(1, std::min<std::int64_t>(waves, (2LL << 30) / per_wave));
+        waves = std::max<std::int64_t>(1, std::min<std::int64_t>(waves, area_cap / per_wave));
         // Every launched lane may take a column, so the work area covers whole blocks.
         const int blocks = (int)((waves + waves_per_block - 1) / waves_per_block);
         waves = (std::int64_t)blocks * waves_per_block;
@@ -269,11 +350,15 @@ void spai_build_typed(Context &c, const DevCsr &a, const T *aval, T *d, double e
         HIP_CHECK(hipMemsetAsync(counter.data(), 0, 2 * sizeof(int), s));
         SpaiColumnOut o{out.c_status.data(), out.c_fits.data(), out.c_growths.data(),
                         out.c_clamps.data(), out.c_q.data(), out.c_res.data(),
-                        out.c_piv.data(), out.c_off.data(), out.c_pass.data(), pass};
-        launch_spai_columns<T>(s, blocks, threads, n, a.ptr.data(), a.col.data(), aval, d,
-                               bptr.data(), bcol.data(), (T)eps, alpha, beta, work.data(), nwork,
-                               counter.data(), mcap, qcap, ccap, dwork.as<T>(), iwork.as<int>(),
-                               o, out.pool_j.back().data(), out.pool_c.back().as<T>(), qcap);
+                        out.c_piv.data(), out.c_flops.data(), out.c_off.data(),
+                        out.c_pass.data(), pass};
+        timed(c, kSpaiKernelName<T>, 0.0, 0.0, [&] {
+            launch_spai_columns<T>(s, blocks, threads, n, a.ptr.data(), a.col.data(), aval, d,
+                                   bptr.data(), bcol.data(), (T)eps, alpha, beta, work.data(),
+                                   nwork, counter.data(), mcap, qcap, ccap, dwork.as<T>(),
+                                   iwork.as<int>(), o, out.pool_j.back().data(),
+                                   out.pool_c.back().as<T>(), qcap);
+        });
         hipLaunchKernelGGL(collect_overflow_kernel, dim3(ceil_div(nwork, 256)), dim3(256), 0, s,
                            nwork, work.data(), out.c_status.data(), next.data(),
                            counter.data() + 1);
@@ -289,14 +374,17 @@ void spai_build_typed(Context &c, const DevCsr &a, const T *aval, T *d, double e
     // ---- summary and P assembly (src/spai.cpp:352-372)
     DeviceArray<int> ff(2);
     DeviceArray<unsigned long long> tot(4);
+    DeviceArray<double> ftot(1);
     {
         const int init[2] = {INT_MAX, INT_MAX};
         upload(c, ff.data(), init, 2);
         HIP_CHECK(hipMemsetAsync(tot.data(), 0, 4 * sizeof(unsigned long long), s));
+        HIP_CHECK(hipMemsetAsync(ftot.data(), 0, sizeof(double), s));
     }
     hipLaunchKernelGGL(column_summary_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, s, n,
                        out.c_status.data(), out.c_fits.data(), out.c_growths.data(),
-                       out.c_clamps.data(), out.c_q.data(), ff.data(), tot.data());
+                       out.c_clamps.data(), out.c_q.data(), out.c_flops.data(), ff.data(),
+                       tot.data(), ftot.data());
     PoolTable pt{};
     for (int p = 0; p < out.passes; ++p) {
         pt.j[p] = out.pool_j[p].data();
@@ -315,7 +403,9 @@ void spai_build_typed(Context &c, const DevCsr &a, const T *aval, T *d, double e
     download(c, hff, ff.data(), 2);
     download(c, htot, tot.data(), 4);
     download(c, &pnnz, out.p.ptr.data() + n, 1);
+    download(c, &out.qr_flops, ftot.data(), 1);
     sync(c);
+    c.timer.add_flops(kSpaiKernelName<T>, out.qr_flops);
     out.fits = (std::int64_t)htot[0];
     out.growths = (std::int64_t)htot[1];
     out.clamps = (std::int64_t)htot[2];
@@ -437,6 +527,10 @@ GmresEngine<T>::GmresEngine(Context &c, const Layout &L, int cap) : c_(c), L_(L)
     g_.resize((std::size_t)(cap + 1) * L.nsys);
     y_.resize((std::size_t)cap * L.nsys);
     part_.resize((std::size_t)kNumSlots * 2 * std::max(1, L.ntiles));
+    gp_.resize((std::size_t)2 * (cap + 1) * std::max(1, L.ntiles));
+    q_.resize((std::size_t)(cap + 1) * L.nsys);
+    h_.resize((std::size_t)(cap + 1) * L.nsys);
+    gram_.resize((std::size_t)gram_size(cap) * L.nsys);
     st_.resize(L.nsys);
     vdev_.resize(cap + 1);
     vptrs_.reserve(cap + 1);
@@ -451,6 +545,11 @@ GmresEngine<T>::GmresEngine(Context &c, const Layout &L, int cap) : c_(c), L_(L)
     args_.hstride = hs;
     args_.cap = cap;
     args_.part = part_.data();
+    args_.gp = gp_.data();
+    args_.q = q_.data();
+    args_.h = h_.data();
+    args_.gram = gram_.data();
+    args_.gstride = gram_size(cap);
 }
 
 template <typename T> void GmresEngine<T>::ensure_basis(int count) {
@@ -517,14 +616,41 @@ void GmresEngine<T>::solve(const DevCsr &m1, const DevCsr &m2, GmresMode mode, c
     gmres_start_vector<T>(s, tl, args_, r0);
     // ---- Arnoldi loop (src/gmres.cpp:172-254)
     std::vector<SysState> host(L_.nsys);
+    // Timer labels and algorithmic bytes: SpMV reads the matrix (value +
+    // int32 column per entry, int32 row offsets) and x once, writes y; the
+    // Gram-Schmidt passes stream V_0..V_k plus w and V_k (project) or w in
+    // and out (update).
+    const char *tname = sizeof(T) == 8 ? "double" : "float";
+    const char *role1 = left ? "A" : "P^T", *role2 = left ? "P" : "A^T";
+    const std::string l1 = spmv_label(m1.lanes, tname, role1), l2 = spmv_label(m2.lanes, tname, role2);
+    const std::string lproj = std::string("project_kernel<") + tname + ">";
+    const std::string lred = std::string("gram_reduce_kernel<") + tname + ">";
+    const std::string lsol = std::string("gram_solve_kernel<") + tname + ">";
+    const std::string lupd = std::string("gram_update_kernel<") + tname + ">";
+    const std::string lnrm = std::string("normalize_givens_kernel<") + tname + ">";
+    const double sz = sizeof(T), nn = (double)L_.n;
+    auto spmv_bytes = [&](const DevCsr &m) {
+        return (double)m.nnz * (sz + 4) + 4.0 * (double)(m.n + 1) + 2.0 * (double)m.n * sz;
+    };
     for (int k = 0; k < cap_; ++k) {
         ensure_basis(std::min(cap_ + 1, k + 2));
-        launch_spmv<T, T, T, T>(s, m1.lanes, tl, active, stride, m1.ptr.data(), m1.col.data(),
-                                m1v, vptrs_[k], t_.data());
-        gmres_spmv_dot0<T>(s, m2.lanes, tl, args_, m2.ptr.data(), m2.col.data(), m2v, t_.data());
-        for (int j = 1; j <= k + 1; ++j)
-            gmres_mgs_round<T>(s, tl, args_, j, k);
-        gmres_normalize_givens<T>(s, tl, args_, k);
+        timed(c_, l1.c_str(), spmv_bytes(m1), 2.0 * (double)m1.nnz, [&] {
+            launch_spmv<T, T, T, T>(s, m1.lanes, tl, active, stride, m1.ptr.data(),
+                                    m1.col.data(), m1v, vptrs_[k], t_.data());
+        });
+        timed(c_, l2.c_str(), spmv_bytes(m2), 2.0 * (double)m2.nnz, [&] {
+            launch_spmv<T, T, T, T>(s, m2.lanes, tl, active, stride, m2.ptr.data(),
+                                    m2.col.data(), m2v, t_.data(), w_.data());
+        });
+        timed(c_, lproj.c_str(), (k + 3) * nn * sz, 2.0 * (2 * k + 1) * nn,
+              [&] { gmres_project<T>(s, tl, args_, k); });
+        timed(c_, lred.c_str(), 2.0 * (2 * k + 1) * tl.ntiles * 8.0, 0.0,
+              [&] { gmres_gram_reduce<T>(s, tl, args_, k); });
+        timed(c_, lsol.c_str(), 0.0, 0.0, [&] { gmres_gram_solve<T>(s, tl, args_, k); });
+        timed(c_, lupd.c_str(), (k + 3) * nn * sz, 2.0 * (k + 1) * nn,
+              [&] { gmres_gram_update<T>(s, tl, args_, k); });
+        timed(c_, lnrm.c_str(), 2.0 * nn * sz, 0.0,
+              [&] { gmres_normalize_givens<T>(s, tl, args_, k); });
         gmres_candidate<T>(s, tl, args_);
         if (left) {
             gmres_spmv_resid<T, T>(s, tl, st, part, m1.ptr.data(), m1.col.data(), m1v, x, rhs,

@@ -74,6 +74,44 @@ template <typename T> class DeviceArray {
     std::size_t n_ = 0;
 };
 
+// Per-kernel timing with HIP events on the context's stream (telemetry, off
+// unless enabled): every timed launch is bracketed by two events, and
+// collect() synchronizes and aggregates the durations by kernel name.
+class KernelTimer {
+  public:
+    struct Total {
+        std::string name;
+        long long launches = 0;
+        double ms = 0, bytes = 0, flops = 0;
+    };
+    KernelTimer() = default;
+    KernelTimer(const KernelTimer &) = delete;
+    KernelTimer &operator=(const KernelTimer &) = delete;
+    ~KernelTimer();
+    bool on = false;
+    int begin(hipStream_t s);
+    void end(hipStream_t s, int first, const char *name, double bytes, double flops);
+    // Adds flops known only after the launch (device-side counts) to the
+    // most recent record of `name`.
+    void add_flops(const char *name, double flops);
+    // Folds the pending records into the running totals (synchronizing on
+    // their events) and returns the totals since the last clear().
+    const std::vector<Total> &collect();
+    void clear() { recs_.clear(), totals_.clear(), used_ = 0; }
+
+  private:
+    struct Rec {
+        std::string name;
+        int first;
+        double bytes, flops;
+    };
+    hipEvent_t event(int i);
+    std::vector<hipEvent_t> pool_;
+    std::vector<Rec> recs_;
+    std::vector<Total> totals_;
+    int used_ = 0;
+};
+
 struct Context {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -81,7 +119,21 @@ struct Context {
     std::string last_error;
     // timing events
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    KernelTimer timer;
 };
+
+// Runs `launch` (one kernel launch on c.stream), timed when the context's
+// timer is on.
+template <class F>
+inline void timed(Context &c, const char *name, double bytes, double flops, F &&launch) {
+    if (!c.timer.on) {
+        launch();
+        return;
+    }
+    const int first = c.timer.begin(c.stream);
+    launch();
+    c.timer.end(c.stream, first, name, bytes, flops);
+}
 
 // Block-diagonal system layout and its row tiles.
 struct Layout {
@@ -124,7 +176,7 @@ struct SpaiBuild {
     DeviceArray<double> d64;
     DeviceArray<float> d32;
     DeviceArray<int> c_status, c_fits, c_growths, c_clamps, c_q;
-    DeviceArray<double> c_res, c_piv;
+    DeviceArray<double> c_res, c_piv, c_flops;
     DeviceArray<std::int64_t> c_off;   // per column: offset into its pass pool
     DeviceArray<int> c_pass;           // per column: pass id
     std::vector<DeviceArray<int>> pool_j;
@@ -132,6 +184,7 @@ struct SpaiBuild {
     std::int64_t p_nnz = 0, pattern_nnz = 0, fits = 0, growths = 0, clamps = 0;
     int passes = 0;
     double build_ms = 0;
+    double qr_flops = 0; // algorithmic QR flops of the accepted fits
 };
 
 std::unique_ptr<SpaiBuild> spai_build(Context &c, const DevCsr &a, double eps, long long alpha,
@@ -169,7 +222,7 @@ template <typename T> class GmresEngine {
     int vcount_ = 0;
     DeviceArray<T> w_, t_, prhs_, r0_;
     DeviceArray<double> xz_, ax_, pax_, ax0_, tmpd_;
-    DeviceArray<double> H_, cs_, sn_, g_, y_, part_;
+    DeviceArray<double> H_, cs_, sn_, g_, y_, part_, gp_, q_, h_, gram_;
     DeviceArray<SysState> st_;
     KrylovArgs<T> args_{};
 };

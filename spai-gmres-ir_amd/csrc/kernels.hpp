@@ -28,6 +28,7 @@ enum : int {
 struct SpaiColumnOut {
     int *status, *fits, *growths, *clamps, *q;
     double *residual, *pivot;
+    double *flops; // QR flops of the column's fits, 2mq^2 - 2q^3/3 + 4mq + q^2 each
     std::int64_t *off;
     int *pass;
     int pass_id;
@@ -68,9 +69,14 @@ struct Tiles {
     const int *sys_tile0;
     const int *sys_tile1;
 };
+// A tile is one block of the vector kernels: 256 threads x 8 rows, one
+// partial per tile and reduction.
 constexpr int kTileRows = 2048;
 constexpr int kTileThreads = 256;
 constexpr int kTilePer = kTileRows / kTileThreads;
+// SpMV blocks cover 256 consecutive rows of a tile (one row per thread).
+constexpr int kSpmvRows = 256;
+constexpr int kSpmvSub = kTileRows / kSpmvRows;
 
 // ------------------------------------------------------------------ SpMV
 // y[r] = sum_q val[q] * x[col[q]] over row r, accumulated in TA.  lanes = the

@@ -1,15 +1,26 @@
 // Left-/right-preconditioned GMRES and iterative-refinement kernels (gfx950).
 //
 // Algorithm (reference src/gmres.cpp:108-262, src/ir.cpp:83-204): unrestarted
-// GMRES with one-pass modified Gram-Schmidt and Givens rotations; the
-// candidate x0 + V y is formed and re-verified (true and recomputed
-// preconditioned residuals) exactly on the iterations whose acceptance test
-// can pass -- estimate <= tau, Arnoldi breakdown, zero Givens pivot, or the
-// last iteration -- which leaves every decision of the reference unchanged.
+// GMRES with modified Gram-Schmidt and Givens rotations; the candidate
+// x0 + V y is formed and re-verified (true and recomputed preconditioned
+// residuals) exactly on the iterations whose acceptance test can pass --
+// estimate <= tau, Arnoldi breakdown, zero Givens pivot, or the last
+// iteration -- which leaves every decision of the reference unchanged.
+//
+// Gram-Schmidt runs in compact form.  The MGS coefficients h of w against
+// V_0..V_k (h_j = <w - sum_{i<j} h_i V_i, V_j>, src/gmres.cpp:176-181) satisfy
+//     (I + L) h = V^T w,   L = strictly lower triangle of V^T V,
+// an identity of exact arithmetic that holds for any V (the inverse compact
+// WY form of MGS: Swirydowicz, Langou, Ananthan, Yang and Thomas, "Low
+// synchronization Gram-Schmidt and GMRES algorithms", Numer. Linear Algebra
+// Appl. 28, 2021).  One streaming pass forms V^T w and the new Gram row
+// V_k^T V_{0..k-1}, a 64-lane forward substitution gives h, and a second pass
+// forms w - V h: two passes over the basis and four launches per iteration
+// instead of k+2 dependent rounds, with the same coefficients up to rounding.
 //
 // Layout: vectors concatenate independent systems; every kernel runs one
-// 256-thread block per row tile (<= 2048 rows, inside one system) and skips
-// tiles of inactive systems, so one launch advances a whole batch.
+// block per row tile (<= 2048 rows, inside one system) and skips tiles of
+// inactive systems, so one launch advances a whole batch.
 //
 // Reductions never synchronize blocks inside a kernel (a device-scope release
 // on a multi-XCD part writes back the L2, which is far costlier than a launch).
@@ -107,21 +118,34 @@ __device__ inline double reduce_max(const double *p, int t0, int t1, double *lds
     (void)writer;
 
 // ------------------------------------------------------------------ GMRES
-// Givens update of column k once h_{k+1,k} is known (src/gmres.cpp:183-217);
-// on trigger, back substitution for y (src/gmres.cpp:47-59).
+// The previous rotations 0..k-1 applied to rows 0..k of column k
+// (src/gmres.cpp:183-189).  They never touch row k+1 = ||w||, so they run as
+// soon as the MGS coefficients are known; the carried row stays in a register.
+__device__ bool apply_old_rotations(double *__restrict__ col, const double *__restrict__ cs,
+                                    const double *__restrict__ sn, int k) {
+    bool bad = false;
+    double up = col[0];
+    for (int r = 0; r < k; ++r) {
+        const double lo = col[r + 1], c = cs[r], z = sn[r];
+        const double top = c * up + z * lo;
+        up = -z * up + c * lo;
+        col[r] = top;
+        bad = bad || !isfinite(top) || !isfinite(up);
+    }
+    col[k] = up;
+    return bad;
+}
+
+// Givens update of column k once h_{k+1,k} is known (src/gmres.cpp:190-217),
+// the older rotations having been applied already; on trigger, back
+// substitution for y (src/gmres.cpp:47-59).
 __device__ void givens_step(SysState &st, double *H, double *cs, double *sn, double *g, double *y,
                             int k, int cap, double hnorm) {
     double *col = H + hcol_offset(k);
     col[k + 1] = hnorm;
     st.hnorm = hnorm;
     const bool breakdown = hnorm == 0;
-    bool bad = !isfinite(hnorm);
-    for (int r = 0; r < k; ++r) {
-        const double up = col[r], lo = col[r + 1];
-        col[r] = cs[r] * up + sn[r] * lo;
-        col[r + 1] = -sn[r] * up + cs[r] * lo;
-        bad = bad || !isfinite(col[r]) || !isfinite(col[r + 1]);
-    }
+    bool bad = !isfinite(hnorm) || st.rot_bad;
     const double upper = col[k];
     const double radius = hypot_ref(upper, col[k + 1]);
     bad = bad || !isfinite(radius);
@@ -171,74 +195,177 @@ __device__ void givens_step(SysState &st, double *H, double *cs, double *sn, dou
     }
 }
 
-// w = M2 t for the tile's rows (group of LANES lanes per row), fused with the
-// first Gram-Schmidt partial <V_0, w>.
-template <int LANES, typename T>
-__global__ void __launch_bounds__(kTileThreads)
-spmv_dot0_kernel(Tiles tiles, KrylovArgs<T> a, const int *__restrict__ ptr,
-                 const int *__restrict__ col, const T *__restrict__ val, const T *__restrict__ x) {
-    __shared__ double lds[kTileThreads / kWave];
+// ------------------------------------------------------- compact MGS passes
+constexpr int kProjWaves = 8;                  // waves per projection block
+constexpr int kProjVecs = 2 * kProjWaves;      // granule of basis vectors per block
+constexpr int kProjBlocks = 2048;              // blocks wanted per projection launch
+constexpr int kProjPer = kTileRows / kWave;    // rows per lane
+constexpr int kReduceWaves = 16;               // one value per wave
+constexpr int kUpdateThreads = 1024;
+constexpr int kUpdatePer = kTileRows / kUpdateThreads;
+
+// Pass 1, grid (tile, group of kProjVecs vectors): w and V_k of the tile are
+// staged in LDS; each wave streams two basis vectors of the group and forms
+// <V_i, w> and <V_i, V_k> (FP64 accumulation, each lane in row order, then a
+// fixed butterfly), written to gp[i][tile] and gp[cap+1+i][tile].
+template <typename T>
+__global__ void __launch_bounds__(kProjWaves *kWave)
+project_kernel(Tiles tiles, KrylovArgs<T> a, int k, int group) {
+    __shared__ T ws[kTileRows], vs[kTileRows];
+    const int t = blockIdx.x, s = tiles.sys[t];
+    const int i0 = blockIdx.y * group;
+    if (!a.st[s].active || i0 > k)
+        return;
+    const int r0 = tiles.row0[t], nr = tiles.row1[t] - r0;
+    const T *__restrict__ w = a.w + r0;
+    const T *__restrict__ vk = a.V[k] + r0;
+    for (int r = threadIdx.x; r < kTileRows; r += kProjWaves * kWave) {
+        ws[r] = r < nr ? w[r] : T(0);
+        vs[r] = r < nr ? vk[r] : T(0);
+    }
+    __syncthreads();
+    const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+    const int iend = min(k, i0 + group - 1);
+    const std::int64_t nt = tiles.ntiles, off = a.cap + 1;
+    for (int ia = i0 + 2 * wave; ia <= iend; ia += 2 * kProjWaves) {
+        const int ib = ia + 1;
+        const bool two = ib <= iend;
+        const T *__restrict__ va = a.V[ia] + r0;
+        const T *__restrict__ vb = a.V[two ? ib : ia] + r0;
+        double dwa = 0, dva = 0, dwb = 0, dvb = 0;
+#pragma unroll 8
+        for (int m = 0; m < kProjPer; ++m) {
+            const int r = lane + m * kWave;
+            if (r < nr) {
+                const double xa = (double)va[r], xb = (double)vb[r];
+                const double wr = (double)ws[r], vr = (double)vs[r];
+                dwa += xa * wr;
+                dva += xa * vr;
+                dwb += xb * wr;
+                dvb += xb * vr;
+            }
+        }
+        dwa = wave_sum(dwa);
+        dva = wave_sum(dva);
+        dwb = wave_sum(dwb);
+        dvb = wave_sum(dvb);
+        if (lane == 0) {
+            a.gp[ia * nt + t] = dwa;
+            if (ia < k)
+                a.gp[(off + ia) * nt + t] = dva;
+            if (two) {
+                a.gp[ib * nt + t] = dwb;
+                if (ib < k)
+                    a.gp[(off + ib) * nt + t] = dvb;
+            }
+        }
+    }
+}
+
+// Pass 1 reduction, grid (system, group of kReduceWaves values): value
+// v <= k is q_v = <V_v, w>, value k < v <= 2k is the Gram entry
+// <V_k, V_{v-k-1}>.  Each wave sums the system's tiles (lane l: tiles
+// t0+l, t0+l+64, ... in order, then a fixed butterfly).
+template <typename T>
+__global__ void __launch_bounds__(kReduceWaves *kWave)
+gram_reduce_kernel(Tiles tiles, KrylovArgs<T> a, int k) {
+    const int s = blockIdx.x;
+    if (!a.st[s].active)
+        return;
+    const int v = blockIdx.y * kReduceWaves + (int)(threadIdx.x / kWave);
+    const int lane = threadIdx.x % kWave;
+    if (v > 2 * k)
+        return;
+    const std::int64_t nt = tiles.ntiles;
+    const double *row = a.gp + (v <= k ? v : a.cap + 1 + (v - k - 1)) * nt;
+    double sum = 0;
+    for (int t = tiles.sys_tile0[s] + lane; t < tiles.sys_tile1[s]; t += kWave)
+        sum += row[t];
+    sum = wave_sum(sum);
+    if (lane == 0) {
+        if (v <= k)
+            a.q[(std::int64_t)s * (a.cap + 1) + v] = sum;
+        else
+            a.gram[(std::int64_t)s * a.gstride + grow_offset(k) + (v - k - 1)] = sum;
+    }
+}
+
+// (I + L) h = q by forward substitution, one wave per system: lane j owns
+// unknown j of the current block of 64, first subtracts the finished blocks
+// (i ascending), then the block is solved with the pivots broadcast lane by
+// lane -- every h_j sees its terms in the order i = 0, 1, ..., j-1.  h is the
+// MGS column H[0..k][k].
+template <typename T>
+__global__ void __launch_bounds__(kWave) gram_solve_kernel(Tiles tiles, KrylovArgs<T> a, int k) {
+    const int s = blockIdx.x;
+    if (!a.st[s].active)
+        return;
+    const int lane = threadIdx.x;
+    const double *q = a.q + (std::int64_t)s * (a.cap + 1);
+    const double *L = a.gram + (std::int64_t)s * a.gstride;
+    double *h = a.h + (std::int64_t)s * (a.cap + 1);
+    double *col = a.H + (std::int64_t)s * a.hstride + hcol_offset(k);
+    for (int b0 = 0; b0 <= k; b0 += kWave) {
+        const int j = b0 + lane;
+        const bool mine = j <= k;
+        const double *Lj = L + grow_offset(j);
+        double acc = mine ? q[j] : 0.0;
+        if (mine)
+            for (int i = 0; i < b0; ++i)
+                acc = acc - Lj[i] * h[i];
+        const int last = min(k, b0 + kWave - 1);
+        for (int i = b0; i <= last; ++i) {
+            const double hi = __shfl(acc, i - b0, kWave);
+            if (mine && j > i)
+                acc = acc - Lj[i] * hi;
+        }
+        if (mine) {
+            h[j] = acc;
+            col[j] = acc;
+        }
+        __syncthreads();
+    }
+    if (lane == 0)
+        a.st[s].rot_bad = apply_old_rotations(col, a.cs + (std::int64_t)s * a.cap,
+                                              a.sn + (std::int64_t)s * a.cap, k);
+}
+
+// Pass 2: w <- w - sum_i h_i V_i (i ascending, in T as the reference's
+// w -= h V_j), and the norm partial of the new w into kSlotNorm.
+template <typename T>
+__global__ void __launch_bounds__(kUpdateThreads)
+gram_update_kernel(Tiles tiles, KrylovArgs<T> a, int k) {
+    __shared__ double lds[kUpdateThreads / kWave];
     BSPAI_SYS_TILES
     if (!a.st[s].active)
         return;
-    const int groups = kTileThreads / LANES;
-    const int g = threadIdx.x / LANES, lane = threadIdx.x % LANES;
-    const T *v0 = a.V[0];
-    double dot = 0;
-    for (int row = tiles.row0[t] + g; row < tiles.row1[t]; row += groups) {
-        T acc = 0;
-        const int e1 = ptr[row + 1];
-        for (int e = ptr[row] + lane; e < e1; e += LANES)
-            acc += val[e] * x[col[e]];
-        if (LANES > 1)
-            acc = group_sum<LANES>(acc);
-        if (lane == 0) {
-            a.w[row] = acc;
-            dot += (double)v0[row] * (double)acc;
-        }
-    }
-    store_dot_partial(dot, slot_ptr(a.part, tiles.ntiles, 0), t, lds);
-}
-
-// MGS round j of iteration k (1 <= j <= k+1):
-//   h_{j-1,k} = reduce(slot (j-1)%2);  w -= h_{j-1,k} V_{j-1}
-//   j <= k:   partial <V_j, w>  -> slot j%2
-//   j == k+1: partial ||w||^2   -> slot j%2
-template <typename T>
-__global__ void __launch_bounds__(kTileThreads)
-mgs_round_kernel(Tiles tiles, KrylovArgs<T> a, int j, int k) {
-    __shared__ double lds[kTileThreads / kWave];
-    BSPAI_SYS_TILES
-    SysState &st = a.st[s];
-    if (!st.active)
-        return;
-    const double h = reduce_dot(slot_ptr(a.part, tiles.ntiles, (j - 1) & 1), t0, t1, lds);
-    if (writer)
-        a.H[(std::int64_t)s * a.hstride + hcol_offset(k) + j - 1] = h;
-    const T hp = (T)h;
-    const T *vprev = a.V[j - 1];
-    T *w = a.w;
-    double *out = slot_ptr(a.part, tiles.ntiles, j & 1);
+    const double *h = a.h + (std::int64_t)s * (a.cap + 1);
     const int r0 = tiles.row0[t], r1 = tiles.row1[t];
-    if (j <= k) {
-        const T *vcur = a.V[j];
-        double dot = 0;
-        for (int r = r0 + threadIdx.x; r < r1; r += kTileThreads) {
-            const T x = w[r] - hp * vprev[r];
-            w[r] = x;
-            dot += (double)vcur[r] * (double)x;
-        }
-        store_dot_partial(dot, out, t, lds);
-    } else {
-        double vals[kTilePer];
-        int cnt = 0;
-        for (int r = r0 + threadIdx.x; r < r1; r += kTileThreads) {
-            const T x = w[r] - hp * vprev[r];
-            w[r] = x;
-            vals[cnt++] = (double)x;
-        }
-        store_norm_partial(vals, cnt, out, t, lds);
+    T x[kUpdatePer];
+    int rows[kUpdatePer];
+#pragma unroll
+    for (int u = 0; u < kUpdatePer; ++u) {
+        rows[u] = r0 + (int)threadIdx.x + u * kUpdateThreads;
+        x[u] = rows[u] < r1 ? a.w[rows[u]] : T(0);
     }
+#pragma unroll 4
+    for (int i = 0; i <= k; ++i) {
+        const T *__restrict__ v = a.V[i];
+        const T hi = (T)h[i];
+#pragma unroll
+        for (int u = 0; u < kUpdatePer; ++u)
+            if (rows[u] < r1)
+                x[u] = x[u] - hi * v[rows[u]];
+    }
+    double vals[kUpdatePer];
+    int cnt = 0;
+#pragma unroll
+    for (int u = 0; u < kUpdatePer; ++u)
+        if (rows[u] < r1) {
+            a.w[rows[u]] = x[u];
+            vals[cnt++] = (double)x[u];
+        }
+    store_norm_partial(vals, cnt, slot_ptr(a.part, tiles.ntiles, kSlotNorm), t, lds);
 }
 
 // h_{k+1,k} = ||w||; V_{k+1} = w / h (unless breakdown); the writer runs the
@@ -251,7 +378,7 @@ normalize_givens_kernel(Tiles tiles, KrylovArgs<T> a, int k) {
     SysState &st = a.st[s];
     if (!st.active)
         return;
-    const double hn = reduce_norm(slot_ptr(a.part, tiles.ntiles, (k + 1) & 1), t0, t1, lds);
+    const double hn = reduce_norm(slot_ptr(a.part, tiles.ntiles, kSlotNorm), t0, t1, lds);
     if (hn != 0 && isfinite(hn) && k + 1 <= a.cap) {
         const T h = (T)hn;
         T *v = a.V[k + 1];
@@ -467,30 +594,27 @@ __global__ void gmres_reset_kernel(int nsys, KrylovArgs<T> k, const IrSysState *
 // ------------------------------------------------------------------ launchers
 #define BSPAI_GRID dim3(tl.ntiles), dim3(kTileThreads), 0, s
 
-template <typename T>
-void gmres_spmv_dot0(hipStream_t s, int lanes, const Tiles &tl, const KrylovArgs<T> &a,
-                     const int *ptr, const int *col, const T *val, const T *x) {
-#define BSPAI_D0(L)                                                                               \
-    case L:                                                                                      \
-        hipLaunchKernelGGL((spmv_dot0_kernel<L, T>), BSPAI_GRID, tl, a, ptr, col, val, x);       \
-        break;
-    switch (lanes) {
-        BSPAI_D0(1)
-        BSPAI_D0(2)
-        BSPAI_D0(4)
-        BSPAI_D0(8)
-        BSPAI_D0(16)
-        BSPAI_D0(32)
-        BSPAI_D0(64)
-    default:
-        throw std::invalid_argument("lanes must be a power of two <= 64");
-    }
-#undef BSPAI_D0
+template <typename T> void gmres_project(hipStream_t s, const Tiles &tl, const KrylovArgs<T> &a, int k) {
+    // Vectors per block: enough groups to fill the chip, but each group
+    // restages w and V_k, so no more groups than that.
+    const int groups = std::max(1, kProjBlocks / std::max(1, tl.ntiles));
+    const int group = kProjVecs * (int)ceil_div(ceil_div(k + 1, groups), kProjVecs);
+    hipLaunchKernelGGL(project_kernel<T>, dim3(tl.ntiles, ceil_div(k + 1, group)),
+                       dim3(kProjWaves * kWave), 0, s, tl, a, k, group);
     HIP_LAUNCH_CHECK();
 }
-template <typename T>
-void gmres_mgs_round(hipStream_t s, const Tiles &tl, const KrylovArgs<T> &a, int j, int k) {
-    hipLaunchKernelGGL(mgs_round_kernel<T>, BSPAI_GRID, tl, a, j, k);
+template <typename T> void gmres_gram_reduce(hipStream_t s, const Tiles &tl, const KrylovArgs<T> &a, int k) {
+    hipLaunchKernelGGL(gram_reduce_kernel<T>, dim3(tl.nsys, ceil_div(2 * k + 1, kReduceWaves)),
+                       dim3(kReduceWaves * kWave), 0, s, tl, a, k);
+    HIP_LAUNCH_CHECK();
+}
+template <typename T> void gmres_gram_solve(hipStream_t s, const Tiles &tl, const KrylovArgs<T> &a, int k) {
+    hipLaunchKernelGGL(gram_solve_kernel<T>, dim3(tl.nsys), dim3(kWave), 0, s, tl, a, k);
+    HIP_LAUNCH_CHECK();
+}
+template <typename T> void gmres_gram_update(hipStream_t s, const Tiles &tl, const KrylovArgs<T> &a, int k) {
+    hipLaunchKernelGGL(gram_update_kernel<T>, dim3(tl.ntiles), dim3(kUpdateThreads), 0, s, tl, a,
+                       k);
     HIP_LAUNCH_CHECK();
 }
 template <typename T>
@@ -541,9 +665,10 @@ void gmres_diff_partial(hipStream_t s, const Tiles &tl, SysState *st, double *pa
 }
 
 #define BSPAI_KRYLOV_INST(T)                                                                      \
-    template void gmres_spmv_dot0<T>(hipStream_t, int, const Tiles &, const KrylovArgs<T> &,      \
-                                     const int *, const int *, const T *, const T *);             \
-    template void gmres_mgs_round<T>(hipStream_t, const Tiles &, const KrylovArgs<T> &, int, int); \
+    template void gmres_project<T>(hipStream_t, const Tiles &, const KrylovArgs<T> &, int);       \
+    template void gmres_gram_reduce<T>(hipStream_t, const Tiles &, const KrylovArgs<T> &, int);   \
+    template void gmres_gram_solve<T>(hipStream_t, const Tiles &, const KrylovArgs<T> &, int);    \
+    template void gmres_gram_update<T>(hipStream_t, const Tiles &, const KrylovArgs<T> &, int);   \
     template void gmres_normalize_givens<T>(hipStream_t, const Tiles &, const KrylovArgs<T> &,    \
                                             int);                                                 \
     template void gmres_candidate<T>(hipStream_t, const Tiles &, const KrylovArgs<T> &);          \

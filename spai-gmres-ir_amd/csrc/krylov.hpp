@@ -29,15 +29,14 @@ enum : int {
 };
 // Partial-sum slots of the GMRES reductions (each 2 doubles per tile).
 enum : int {
-    kSlotMgs0 = 0, // Gram-Schmidt rounds alternate between slots 0 and 1
-    kSlotMgs1 = 1,
-    kSlotRhs = 2,
-    kSlotOrig = 3,
-    kSlotDen = 4,
-    kSlotBeta = 5,
-    kSlotChkOrig = 6,
-    kSlotChkRel = 7,
-    kNumSlots = 8,
+    kSlotNorm = 0, // ||w|| after orthogonalization
+    kSlotRhs = 1,
+    kSlotOrig = 2,
+    kSlotDen = 3,
+    kSlotBeta = 4,
+    kSlotChkOrig = 5,
+    kSlotChkRel = 6,
+    kNumSlots = 7,
 };
 enum : int { kPhaseInit = 0, kPhaseDen = 1, kPhaseBeta = 2, kPhaseCheck = 3 };
 enum : int { kIrSlotR2 = 0, kIrSlotB = 1, kIrNumSlots = 2 };
@@ -51,6 +50,13 @@ __host__ __device__ inline std::int64_t hcol_offset(int k) {
 inline std::int64_t hessenberg_size(int cap) {
     return hcol_offset(cap) + 2;
 }
+// Row j of the strictly lower Gram triangle (entries <V_j, V_i>, i < j), packed.
+__host__ __device__ inline std::int64_t grow_offset(int j) {
+    return (std::int64_t)j * (j - 1) / 2;
+}
+inline std::int64_t gram_size(int cap) {
+    return grow_offset(cap + 1);
+}
 
 struct SysState {
     double tol;
@@ -59,6 +65,7 @@ struct SysState {
     int nrows;
     int participating, active, status, converged, iterations;
     int triggered, ccount, zero_pivot, breakdown;
+    int rot_bad; // a previous rotation made column k non-finite
     int rhs_zero, x0_is_solution, has_solution;
 };
 
@@ -72,6 +79,11 @@ template <typename T> struct KrylovArgs {
     double *part; // kNumSlots slots x 2 doubles per tile
     double *x;    // candidate / solution (FP64)
     const double *x0;
+    // Compact Gram-Schmidt workspace: gp[v][tile] per-tile partials of
+    // <V_i, w> (v = i) and <V_i, V_k> (v = cap+1+i); per system q = V^T w,
+    // h = MGS coefficients ([cap+1] each) and the packed Gram triangle.
+    double *gp, *q, *h, *gram;
+    std::int64_t gstride;
 };
 
 struct IrSysState {
@@ -92,11 +104,12 @@ struct IrArgs {
     double be_tol, relres_target;
 };
 
-template <typename T>
-void gmres_spmv_dot0(hipStream_t s, int lanes, const Tiles &tl, const KrylovArgs<T> &a,
-                     const int *ptr, const int *col, const T *val, const T *x);
-template <typename T>
-void gmres_mgs_round(hipStream_t s, const Tiles &tl, const KrylovArgs<T> &a, int j, int k);
+// Compact MGS of iteration k: project (V^T w and the Gram row), reduce over
+// tiles, triangular solve for h (+ older rotations), update w -= V h.
+template <typename T> void gmres_project(hipStream_t s, const Tiles &tl, const KrylovArgs<T> &a, int k);
+template <typename T> void gmres_gram_reduce(hipStream_t s, const Tiles &tl, const KrylovArgs<T> &a, int k);
+template <typename T> void gmres_gram_solve(hipStream_t s, const Tiles &tl, const KrylovArgs<T> &a, int k);
+template <typename T> void gmres_gram_update(hipStream_t s, const Tiles &tl, const KrylovArgs<T> &a, int k);
 template <typename T>
 void gmres_normalize_givens(hipStream_t s, const Tiles &tl, const KrylovArgs<T> &a, int k);
 template <typename T> void gmres_candidate(hipStream_t s, const Tiles &tl, const KrylovArgs<T> &a);

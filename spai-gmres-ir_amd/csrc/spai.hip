@@ -111,6 +111,7 @@ spai_columns_kernel(int n, const int *__restrict__ aptr, const int *__restrict__
         const int k = worklist[slot];
         int status = kSpaiNotAttempted;
         int fits = 0, growths = 0, clamps = 0, q = 1, qdone = 0;
+        double qr_flops = 0;
         double last_res = __builtin_nan(""), last_piv = __builtin_nan("");
         L.I(oJ) = k;
         int *jout = out_j + (std::int64_t)slot * out_stride;
@@ -217,26 +218,59 @@ spai_columns_kernel(int n, const int *__restrict__ aptr, const int *__restrict__
                 }
                 for (int r = 0; r < len; ++r)
                     L.D(oV + r) = L.D(oV + r) / vnorm;
-                // Trailing columns c > j (column j itself is overwritten below).
-                for (int c = j + 1; c < q; ++c) {
-                    T dot = 0;
-                    for (int r = 0; r < len; ++r)
-                        dot += L.D(oV + r) * L.D(oR + c * M + j + r);
-                    for (int r = 0; r < len; ++r) {
-                        T &x = L.D(oR + c * M + j + r);
-                        x -= T(2) * L.D(oV + r) * dot;
-                        if (!isfinite(x))
-                            fit_status = kSpaiNumericalFailure;
+                // Reflect the trailing columns c > j and the right side
+                // (column j itself is overwritten below).  Columns are
+                // independent, so groups of four share each load of v while
+                // every column keeps its own sequential dot product and
+                // update, in the reference's order (src/spai.cpp:59-75).
+                const int ntrail = q - j; // q-1-j columns + the right side
+                for (int c0 = 0; c0 < ntrail && fit_status == kSpaiSuccess; c0 += 4) {
+                    const int nb = ntrail - c0 < 4 ? ntrail - c0 : 4;
+                    int base[4];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        const int c = j + 1 + c0 + (u < nb ? u : 0);
+                        base[u] = (c < q ? oR + c * M : oRhs) + j;
                     }
-                }
-                T dot = 0;
-                for (int r = 0; r < len; ++r)
-                    dot += L.D(oV + r) * L.D(oRhs + j + r);
-                for (int r = 0; r < len; ++r) {
-                    T &x = L.D(oRhs + j + r);
-                    x -= T(2) * L.D(oV + r) * dot;
-                    if (!isfinite(x))
-                        fit_status = kSpaiNumericalFailure;
+                    if (nb == 4) {
+                        T d0 = 0, d1 = 0, d2 = 0, d3 = 0;
+                        for (int r = 0; r < len; ++r) {
+                            const T vr = L.D(oV + r);
+                            d0 += vr * L.D(base[0] + r);
+                            d1 += vr * L.D(base[1] + r);
+                            d2 += vr * L.D(base[2] + r);
+                            d3 += vr * L.D(base[3] + r);
+                        }
+                        bool ok = true;
+                        for (int r = 0; r < len; ++r) {
+                            const T v2 = T(2) * L.D(oV + r);
+                            T &x0 = L.D(base[0] + r);
+                            T &x1 = L.D(base[1] + r);
+                            T &x2 = L.D(base[2] + r);
+                            T &x3 = L.D(base[3] + r);
+                            x0 -= v2 * d0;
+                            x1 -= v2 * d1;
+                            x2 -= v2 * d2;
+                            x3 -= v2 * d3;
+                            ok = ok && isfinite(x0) && isfinite(x1) && isfinite(x2) && isfinite(x3);
+                        }
+                        if (!ok)
+                            fit_status = kSpaiNumericalFailure;
+                    } else {
+                        for (int u = 0; u < nb; ++u) {
+                            T dot = 0;
+                            for (int r = 0; r < len; ++r)
+                                dot += L.D(oV + r) * L.D(base[u] + r);
+                            bool ok = true;
+                            for (int r = 0; r < len; ++r) {
+                                T &x = L.D(base[u] + r);
+                                x -= T(2) * L.D(oV + r) * dot;
+                                ok = ok && isfinite(x);
+                            }
+                            if (!ok)
+                                fit_status = kSpaiNumericalFailure;
+                        }
+                    }
                 }
                 L.D(oR + j * M + j) = diagonal;
                 const T ad = fabs(diagonal);
@@ -298,6 +332,10 @@ spai_columns_kernel(int n, const int *__restrict__ aptr, const int *__restrict__
             if (fits != 0)
                 ++growths;
             ++fits;
+            {
+                const double fm = m, fq = q;
+                qr_flops += 2 * fm * fq * fq - 2 * fq * fq * fq / 3 + 4 * fm * fq + fq * fq;
+            }
             for (int c = 0; c < q; ++c) {
                 jout[c] = L.I(oJ + c);
                 cout_[c] = L.D(oCoef + c);
@@ -474,6 +512,7 @@ spai_columns_kernel(int n, const int *__restrict__ aptr, const int *__restrict__
         out.q[k] = status == kSpaiOverflow ? 0 : qdone;
         out.residual[k] = last_res;
         out.pivot[k] = last_piv;
+        out.flops[k] = qr_flops;
         out.off[k] = (std::int64_t)slot * out_stride;
         out.pass[k] = out.pass_id;
     }

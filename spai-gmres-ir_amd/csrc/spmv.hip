@@ -12,27 +12,70 @@
 
 namespace bspai_hip {
 
+// Row-group kernel: LANES lanes per row stream the row with unit stride and
+// reduce with a fixed butterfly (for long rows).
 template <int LANES, typename TV, typename TX, typename TA, typename TY>
-__global__ void __launch_bounds__(256)
-spmv_kernel(Tiles tiles, const int *__restrict__ flag, int flag_stride, const int *__restrict__ ptr,
-            const int *__restrict__ col, const TV *__restrict__ val, const TX *__restrict__ x,
-            TY *__restrict__ y) {
-    const int t = blockIdx.x;
+__global__ void __launch_bounds__(kSpmvRows)
+spmv_group_kernel(Tiles tiles, const int *__restrict__ flag, int flag_stride,
+                  const int *__restrict__ ptr, const int *__restrict__ col,
+                  const TV *__restrict__ val, const TX *__restrict__ x, TY *__restrict__ y) {
+    const int t = blockIdx.x / kSpmvSub, sb = blockIdx.x % kSpmvSub;
     if (flag && !flag[(std::int64_t)tiles.sys[t] * flag_stride])
         return;
-    const int groups = blockDim.x / LANES;
+    const int rbeg = tiles.row0[t] + sb * kSpmvRows;
+    const int rend = min(tiles.row1[t], rbeg + kSpmvRows);
+    const int groups = kSpmvRows / LANES;
     const int g = threadIdx.x / LANES, lane = threadIdx.x % LANES;
-    const int r1 = tiles.row1[t];
-    for (int row = tiles.row0[t] + g; row < r1; row += groups) {
+    for (int row = rbeg + g; row < rend; row += groups) {
         TA acc = 0;
         const int e1 = ptr[row + 1];
         for (int e = ptr[row] + lane; e < e1; e += LANES)
             acc += (TA)val[e] * (TA)x[col[e]];
-        if (LANES > 1)
-            acc = group_sum<LANES>(acc);
+        acc = group_sum<LANES>(acc);
         if (lane == 0)
             y[row] = (TY)acc;
     }
+}
+
+// Streaming kernel for short rows: the block's 256 rows own one contiguous
+// range of (val, col); all threads load it with unit stride, form the products
+// in LDS, then each thread adds its row's products in CSR order -- the
+// reference's sequential row sum (src/cpu.cpp:88-89), bit for bit.
+constexpr int kStreamChunk = 2048;
+template <typename TV, typename TX, typename TA, typename TY>
+__global__ void __launch_bounds__(kSpmvRows)
+spmv_stream_kernel(Tiles tiles, const int *__restrict__ flag, int flag_stride,
+                   const int *__restrict__ ptr, const int *__restrict__ col,
+                   const TV *__restrict__ val, const TX *__restrict__ x, TY *__restrict__ y) {
+    __shared__ int rp[kSpmvRows + 1];
+    __shared__ TA prod[kStreamChunk];
+    const int t = blockIdx.x / kSpmvSub, sb = blockIdx.x % kSpmvSub;
+    if (flag && !flag[(std::int64_t)tiles.sys[t] * flag_stride])
+        return;
+    const int rbeg = tiles.row0[t] + sb * kSpmvRows;
+    const int rend = min(tiles.row1[t], rbeg + kSpmvRows);
+    if (rbeg >= rend)
+        return;
+    const int nr = rend - rbeg;
+    for (int i = threadIdx.x; i <= nr; i += kSpmvRows)
+        rp[i] = ptr[rbeg + i];
+    __syncthreads();
+    const int e0 = rp[0], e1 = rp[nr];
+    const int tid = threadIdx.x;
+    const int mb = tid < nr ? rp[tid] : 0, me = tid < nr ? rp[tid + 1] : 0;
+    TA acc = 0;
+    for (int c0 = e0; c0 < e1; c0 += kStreamChunk) {
+        const int c1 = min(e1, c0 + kStreamChunk);
+        for (int e = c0 + tid; e < c1; e += kSpmvRows)
+            prod[e - c0] = (TA)val[e] * (TA)x[col[e]];
+        __syncthreads();
+        const int lo = max(mb, c0), hi = min(me, c1);
+        for (int e = lo; e < hi; ++e)
+            acc += prod[e - c0];
+        __syncthreads();
+    }
+    if (tid < nr)
+        y[rbeg + tid] = (TY)acc;
 }
 
 template <typename TV, typename TX, typename TA, typename TY>
@@ -40,14 +83,17 @@ void launch_spmv(hipStream_t s, int lanes, const Tiles &tiles, const int *flag, 
                  const int *ptr, const int *col, const TV *val, const TX *x, TY *y) {
     if (tiles.ntiles == 0)
         return;
-    const dim3 grid(tiles.ntiles), block(kTileThreads);
+    const dim3 grid((unsigned)tiles.ntiles * kSpmvSub), block(kSpmvRows);
 #define BSPAI_SPMV_CASE(L)                                                                        \
     case L:                                                                                      \
-        hipLaunchKernelGGL((spmv_kernel<L, TV, TX, TA, TY>), grid, block, 0, s, tiles, flag,     \
-                           flag_stride, ptr, col, val, x, y);                                    \
+        hipLaunchKernelGGL((spmv_group_kernel<L, TV, TX, TA, TY>), grid, block, 0, s, tiles,     \
+                           flag, flag_stride, ptr, col, val, x, y);                              \
         break;
     switch (lanes) {
-        BSPAI_SPMV_CASE(1)
+    case 1:
+        hipLaunchKernelGGL((spmv_stream_kernel<TV, TX, TA, TY>), grid, block, 0, s, tiles, flag,
+                           flag_stride, ptr, col, val, x, y);
+        break;
         BSPAI_SPMV_CASE(2)
         BSPAI_SPMV_CASE(4)
         BSPAI_SPMV_CASE(8)

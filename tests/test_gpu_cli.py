@@ -31,14 +31,20 @@ def parse(out):
 @pytest.mark.parametrize("exe", ["hip_fp64_ir", "hip_mixed_ir"])
 def test_cli_tiny_fixture(exe):
     """[[4,1,0],[2,5,1],[0,1,3]] at eps 0.5: x = [12,2,16]/(50 sqrt 3)
-    (reference tests/check_cuda_ir_cli.py:44), exit 0, backward error <= 10u."""
+    (reference tests/check_cuda_ir_cli.py:44), exit 0, backward error <= 10u.
+    The FP64 flavour takes the reference CLI's outer steps; the mixed flavour's
+    FP32 corrections (tau 1e-4) may need one more refinement step."""
     r = run(exe, os.path.join(GOLDEN, "tiny_nonsymmetric.mtx"), "0.5")
     assert r.returncode == 0, r.stderr
     kv, x = parse(r.stdout)
     ref_kv, ref_x = parse(open(os.path.join(GOLDEN, "cli_tiny.txt")).read())
     assert kv["status"] == ref_kv["status"] == "success" and kv["converged"] == "1"
     assert kv["n"] == "3" and kv["nnz"] == "7" and kv["epsilon"] == "0.5"
-    assert kv["outer_steps"] == ref_kv["outer_steps"]
+    if exe == "hip_fp64_ir":
+        assert kv["outer_steps"] == ref_kv["outer_steps"] and kv["tau"] == ref_kv["tau"]
+    else:
+        assert kv["tau"] == "0.0001"
+        assert int(ref_kv["outer_steps"]) <= int(kv["outer_steps"]) <= int(ref_kv["outer_steps"]) + 2
     assert float(kv["backward_error"]) <= 10 * 2.0 ** -53
     exact = np.array([12, 2, 16]) / (50 * math.sqrt(3))
     assert np.max(np.abs(x - exact)) <= 1e-15
