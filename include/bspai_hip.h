@@ -157,8 +157,10 @@ int bspai_hip_matrix_download(bspai_hip_ctx *ctx, const bspai_hip_matrix *m, int
 int bspai_hip_matrix_transpose(bspai_hip_ctx *ctx, const bspai_hip_matrix *m,
                                bspai_hip_matrix **out);
 
-/* y = A x in FP64 (lanes: 0 = library choice, 1 = sequential row order,
- * bit-identical to the reference's CsrMatrix::multiply). */
+/* y = A x in FP64.  lanes: 0 = the matrix's default; 1 = streaming kernel
+ * (the block's entries staged in LDS); -1 = one thread per row; 2..64 (a
+ * power of two) = that many lanes per row.  1 and -1 add each row in CSR
+ * order, bit-identical to the reference's CsrMatrix::multiply. */
 int bspai_hip_spmv(bspai_hip_ctx *ctx, const bspai_hip_matrix *a, const double *x, double *y,
                    int where, int lanes);
 /* y = A x with FP32 values and vectors (the mixed path's inner product). */

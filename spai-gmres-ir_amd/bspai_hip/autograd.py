@@ -23,7 +23,8 @@ class SparseSolve(torch.autograd.Function):
     def forward(ctx, values, b, handle, opts):
         m = handle.matrix
         m.set_values(values.detach().contiguous())
-        res = api.spai_gmres_ir(m, b.detach().contiguous(), **opts)
+        fwd = {k: v for k, v in opts.items() if k != "adjoint"}
+        res = api.spai_gmres_ir(m, b.detach().contiguous(), **fwd)
         if res.status != "success":
             raise RuntimeError(f"forward solve failed: {res.status}: {res.message}")
         handle.last_forward = res
@@ -38,17 +39,19 @@ class SparseSolve(torch.autograd.Function):
         (x,) = ctx.saved_tensors
         h = ctx.handle
         m = h.matrix
+        opts = {k: v for k, v in ctx.opts.items() if k != "adjoint"}
+        opts.update(ctx.opts.get("adjoint") or {})
         # P of the current A (the forward's construction is rebuilt here so the
         # transposed preconditioner matches the values used in forward).
-        spai = api.construct_spai(m, epsilon=ctx.opts.get("epsilon", 0.3),
-                                  alpha=ctx.opts.get("alpha", 0) or -(-m.n // 8),
-                                  beta=ctx.opts.get("beta", 8),
-                                  precision=ctx.opts.get("precision", "fp64"))
+        spai = api.construct_spai(m, epsilon=opts.get("epsilon", 0.3),
+                                  alpha=opts.get("alpha", 0) or -(-m.n // 8),
+                                  beta=opts.get("beta", 8),
+                                  precision=opts.get("precision", "fp64"))
         if spai.status != "success":
             raise RuntimeError(f"backward SPAI failed: {spai.status}")
         at = m.transpose()
         pt = spai.matrix(transpose=True)
-        kw = {k: v for k, v in ctx.opts.items() if k not in ("epsilon", "alpha", "beta")}
+        kw = {k: v for k, v in opts.items() if k not in ("epsilon", "alpha", "beta")}
         adj = api.adjoint_solve(at, pt, grad_x.contiguous(), **kw)
         if adj.status != "success":
             raise RuntimeError(f"adjoint solve failed: {adj.status}: {adj.message}")
@@ -71,7 +74,12 @@ def solve(values: torch.Tensor, b: torch.Tensor, handle: SolverHandle, **opts) -
     """x = A(values)^-1 b, differentiable in values and b.
 
     opts: spai_gmres_ir keywords (epsilon, beta, tolerance, max_iterations,
-    max_outer_steps, precision, relres_target)."""
+    max_outer_steps, precision, relres_target), used by both solves, plus
+    `adjoint`: a dict of keywords that override them for the adjoint solve.
+    The adjoint's stopping test is on the unpreconditioned residual
+    ||g - A^T lambda|| / ||g||, whose FP32 floor is u32 ||A|| ||lambda|| / ||g||:
+    ill-conditioned systems may need the adjoint in FP64, or a looser FP32
+    tolerance, where the forward solve runs mixed."""
     return SparseSolve.apply(values, b, handle, opts)
 
 

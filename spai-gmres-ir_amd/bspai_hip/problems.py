@@ -221,6 +221,12 @@ class SystemBatch:
     offsets: np.ndarray  # int64[nsys+1], row range of each system
     rhs: np.ndarray      # float64[sum n], each block equal-component unit norm
 
+    def block(self, s: int) -> Csr:
+        """System s as its own CSR (rows and columns renumbered from 0)."""
+        a, r0, r1 = self.matrix, int(self.offsets[s]), int(self.offsets[s + 1])
+        p0, p1 = int(a.ptr[r0]), int(a.ptr[r1])
+        return Csr(r1 - r0, a.ptr[r0:r1 + 1] - p0, a.col[p0:p1] - r0, a.val[p0:p1])
+
 
 def many_systems(count: int = 256, seed: int = 0xBA7C4, lmin: int = 32, lmax: int = 256,
                  sigma: float = 1.0, pemax: float = 10.0) -> SystemBatch:

@@ -451,7 +451,8 @@ int bspai_hip_spmv(bspai_hip_ctx *ctx, const bspai_hip_matrix *a, const double *
                    int where, int lanes) {
     return guard(ctx, [&] {
         require(a && x && y, "null argument");
-        require(lanes == 0 || (lanes <= 64 && (lanes & (lanes - 1)) == 0), "lanes: 0 or 2^k<=64");
+        require(lanes == 0 || lanes == -1 || (lanes > 0 && lanes <= 64 && (lanes & (lanes - 1)) == 0),
+                "lanes: 0, -1 or 2^k <= 64");
         Context &c = ctx->c;
         In<double> xi(c, x, a->a.n, where);
         Out<double> yo(c, y, a->a.n, where);
@@ -464,7 +465,8 @@ int bspai_hip_spmv_f32(bspai_hip_ctx *ctx, const bspai_hip_matrix *a, const floa
                        int where, int lanes) {
     return guard(ctx, [&] {
         require(a && x && y, "null argument");
-        require(lanes == 0 || (lanes <= 64 && (lanes & (lanes - 1)) == 0), "lanes: 0 or 2^k<=64");
+        require(lanes == 0 || lanes == -1 || (lanes > 0 && lanes <= 64 && (lanes & (lanes - 1)) == 0),
+                "lanes: 0, -1 or 2^k <= 64");
         Context &c = ctx->c;
         DevCsr &A = const_cast<DevCsr &>(a->a);
         A.ensure32(c);

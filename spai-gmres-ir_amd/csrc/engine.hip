@@ -557,11 +557,14 @@ template <typename T> void GmresEngine<T>::ensure_basis(int count) {
     const int chunk = 16;
     if (count <= vcount_)
         return;
+    // Vector stride padded to 64 elements: every basis vector starts 256-byte
+    // aligned, which the 2-wide loads of the Gram-Schmidt passes rely on.
+    const std::size_t ld = (std::size_t)((L_.n + 63) / 64 * 64);
     while ((int)vptrs_.size() < count) {
         const int take = std::min(chunk, cap_ + 1 - (int)vptrs_.size());
-        chunks_.emplace_back((std::size_t)take * L_.n);
+        chunks_.emplace_back((std::size_t)take * ld);
         for (int i = 0; i < take; ++i)
-            vptrs_.push_back(chunks_.back().data() + (std::size_t)i * L_.n);
+            vptrs_.push_back(chunks_.back().data() + (std::size_t)i * ld);
     }
     upload(c_, vdev_.data(), vptrs_.data(), vptrs_.size());
     vcount_ = (int)vptrs_.size();

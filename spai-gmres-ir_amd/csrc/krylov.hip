@@ -118,29 +118,12 @@ __device__ inline double reduce_max(const double *p, int t0, int t1, double *lds
     (void)writer;
 
 // ------------------------------------------------------------------ GMRES
-// The previous rotations 0..k-1 applied to rows 0..k of column k
-// (src/gmres.cpp:183-189).  They never touch row k+1 = ||w||, so they run as
-// soon as the MGS coefficients are known; the carried row stays in a register.
-__device__ bool apply_old_rotations(double *__restrict__ col, const double *__restrict__ cs,
-                                    const double *__restrict__ sn, int k) {
-    bool bad = false;
-    double up = col[0];
-    for (int r = 0; r < k; ++r) {
-        const double lo = col[r + 1], c = cs[r], z = sn[r];
-        const double top = c * up + z * lo;
-        up = -z * up + c * lo;
-        col[r] = top;
-        bad = bad || !isfinite(top) || !isfinite(up);
-    }
-    col[k] = up;
-    return bad;
-}
-
 // Givens update of column k once h_{k+1,k} is known (src/gmres.cpp:190-217),
-// the older rotations having been applied already; on trigger, back
-// substitution for y (src/gmres.cpp:47-59).
-__device__ void givens_step(SysState &st, double *H, double *cs, double *sn, double *g, double *y,
-                            int k, int cap, double hnorm) {
+// the older rotations having been applied already.  Sets st.triggered when
+// the candidate must be formed and verified; y is then solved by
+// back_substitute.
+__device__ void givens_step(SysState &st, double *H, double *cs, double *sn, double *g, int k,
+                            int cap, double hnorm) {
     double *col = H + hcol_offset(k);
     col[k + 1] = hnorm;
     st.hnorm = hnorm;
@@ -179,30 +162,55 @@ __device__ void givens_step(SysState &st, double *H, double *cs, double *sn, dou
         return;
     }
     st.ccount = count;
-    if (st.triggered) {
-        for (int row = count - 1; row >= 0; --row) {
-            double value = g[row];
-            for (int c = row + 1; c < count; ++c)
-                value = value - H[hcol_offset(c) + row] * y[c];
-            y[row] = value / H[hcol_offset(row) + row];
-            if (!isfinite(y[row])) {
-                st.status = kGmNumericalFailure;
-                st.active = 0;
-                st.triggered = 0;
-                return;
-            }
+}
+
+// y = R^{-1} g for the leading count x count block (src/gmres.cpp:47-59), by
+// one wave: row by row from the bottom, the sum over c > row is split by
+// c mod 64 across the lanes (each lane ascending) and combined by a fixed
+// butterfly; lane row mod 64 owns y[row], so every y entry is written and
+// read by one lane only.  Returns false on a non-finite y.
+__device__ bool back_substitute(const double *__restrict__ H, const double *__restrict__ g,
+                                double *__restrict__ y, int count) {
+    const int lane = threadIdx.x % kWave;
+    bool bad = false;
+    for (int row = count - 1; row >= 0; --row) {
+        double part = 0;
+        for (int c = row + 1 + ((lane - row - 1) % kWave + kWave) % kWave; c < count; c += kWave)
+            part += H[hcol_offset(c) + row] * y[c];
+        part = wave_sum(part);
+        if (lane == row % kWave) {
+            const double v = (g[row] - part) / H[hcol_offset(row) + row];
+            y[row] = v;
+            bad = bad || !isfinite(v);
         }
     }
+    return !__any(bad);
 }
 
 // ------------------------------------------------------- compact MGS passes
 constexpr int kProjWaves = 8;                  // waves per projection block
 constexpr int kProjVecs = 2 * kProjWaves;      // granule of basis vectors per block
 constexpr int kProjBlocks = 2048;              // blocks wanted per projection launch
-constexpr int kProjPer = kTileRows / kWave;    // rows per lane
+constexpr int kProjPairs = kTileRows / (2 * kWave); // row pairs per lane
 constexpr int kReduceWaves = 16;               // one value per wave
-constexpr int kUpdateThreads = 1024;
-constexpr int kUpdatePer = kTileRows / kUpdateThreads;
+constexpr int kUpdateThreads = 1024;            // one row pair per thread
+
+// Rows r, r+1 of a vector in one 2-wide load when the pair is aligned (basis
+// strides are even and a tile's pairs share the parity of the tile start,
+// so the test is block-uniform), else in two loads.
+template <typename T> struct Vec2;
+template <> struct Vec2<float> { using type = float2; };
+template <> struct Vec2<double> { using type = double2; };
+template <typename T> __device__ inline void load_pair(const T *p, bool aligned, T &x, T &y) {
+    if (aligned) {
+        const typename Vec2<T>::type v = *reinterpret_cast<const typename Vec2<T>::type *>(p);
+        x = v.x;
+        y = v.y;
+    } else {
+        x = p[0];
+        y = p[1];
+    }
+}
 
 // Pass 1, grid (tile, group of kProjVecs vectors): w and V_k of the tile are
 // staged in LDS; each wave streams two basis vectors of the group and forms
@@ -227,22 +235,36 @@ project_kernel(Tiles tiles, KrylovArgs<T> a, int k, int group) {
     const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
     const int iend = min(k, i0 + group - 1);
     const std::int64_t nt = tiles.ntiles, off = a.cap + 1;
+    const bool aligned = (r0 & 1) == 0;
     for (int ia = i0 + 2 * wave; ia <= iend; ia += 2 * kProjWaves) {
         const int ib = ia + 1;
         const bool two = ib <= iend;
         const T *__restrict__ va = a.V[ia] + r0;
         const T *__restrict__ vb = a.V[two ? ib : ia] + r0;
         double dwa = 0, dva = 0, dwb = 0, dvb = 0;
-#pragma unroll 8
-        for (int m = 0; m < kProjPer; ++m) {
-            const int r = lane + m * kWave;
-            if (r < nr) {
+#pragma unroll 4
+        for (int m = 0; m < kProjPairs; ++m) {
+            const int r = 2 * (lane + m * kWave);
+            if (r + 1 < nr) {
+                T a0, a1, b0, b1;
+                load_pair(va + r, aligned, a0, a1);
+                load_pair(vb + r, aligned, b0, b1);
+                const double w0 = (double)ws[r], w1 = (double)ws[r + 1];
+                const double v0 = (double)vs[r], v1 = (double)vs[r + 1];
+                dwa += (double)a0 * w0;
+                dva += (double)a0 * v0;
+                dwb += (double)b0 * w0;
+                dvb += (double)b0 * v0;
+                dwa += (double)a1 * w1;
+                dva += (double)a1 * v1;
+                dwb += (double)b1 * w1;
+                dvb += (double)b1 * v1;
+            } else if (r < nr) {
                 const double xa = (double)va[r], xb = (double)vb[r];
-                const double wr = (double)ws[r], vr = (double)vs[r];
-                dwa += xa * wr;
-                dva += xa * vr;
-                dwb += xb * wr;
-                dvb += xb * vr;
+                dwa += xa * (double)ws[r];
+                dva += xa * (double)vs[r];
+                dwb += xb * (double)ws[r];
+                dvb += xb * (double)vs[r];
             }
         }
         dwa = wave_sum(dwa);
@@ -286,7 +308,7 @@ gram_reduce_kernel(Tiles tiles, KrylovArgs<T> a, int k) {
         if (v <= k)
             a.q[(std::int64_t)s * (a.cap + 1) + v] = sum;
         else
-            a.gram[(std::int64_t)s * a.gstride + grow_offset(k) + (v - k - 1)] = sum;
+            a.gram[(std::int64_t)s * a.gstride + gram_index(k, v - k - 1, a.cap)] = sum;
     }
 }
 
@@ -294,30 +316,57 @@ gram_reduce_kernel(Tiles tiles, KrylovArgs<T> a, int k) {
 // unknown j of the current block of 64, first subtracts the finished blocks
 // (i ascending), then the block is solved with the pivots broadcast lane by
 // lane -- every h_j sees its terms in the order i = 0, 1, ..., j-1.  h is the
-// MGS column H[0..k][k].
+// MGS column H[0..k][k].  Lane 0 then applies the older Givens rotations to
+// it, staged through LDS in chunks so the serial chain reads no global memory.
+constexpr int kRotChunk = 512;
+constexpr int kLoadBatch = 16; // divides kWave
 template <typename T>
 __global__ void __launch_bounds__(kWave) gram_solve_kernel(Tiles tiles, KrylovArgs<T> a, int k) {
+    __shared__ double lcs[kRotChunk], lsn[kRotChunk], lrow[kRotChunk];
+    __shared__ double tile[kWave][kWave + 1]; // the block's own triangle, [i][lane]
     const int s = blockIdx.x;
     if (!a.st[s].active)
         return;
-    const int lane = threadIdx.x;
-    const double *q = a.q + (std::int64_t)s * (a.cap + 1);
+    const int lane = threadIdx.x, cap = a.cap;
+    const double *q = a.q + (std::int64_t)s * (cap + 1);
     const double *L = a.gram + (std::int64_t)s * a.gstride;
-    double *h = a.h + (std::int64_t)s * (a.cap + 1);
+    double *h = a.h + (std::int64_t)s * (cap + 1);
     double *col = a.H + (std::int64_t)s * a.hstride + hcol_offset(k);
     for (int b0 = 0; b0 <= k; b0 += kWave) {
         const int j = b0 + lane;
         const bool mine = j <= k;
-        const double *Lj = L + grow_offset(j);
-        double acc = mine ? q[j] : 0.0;
-        if (mine)
-            for (int i = 0; i < b0; ++i)
-                acc = acc - Lj[i] * h[i];
         const int last = min(k, b0 + kWave - 1);
+        // Stage L(j, b0..j-1) first: these loads do not depend on the chain,
+        // and are issued in batches so their latencies overlap.
+        for (int i0 = b0; i0 <= last; i0 += kLoadBatch) {
+            double v[kLoadBatch];
+#pragma unroll
+            for (int u = 0; u < kLoadBatch; ++u) {
+                const int i = i0 + u;
+                v[u] = i <= last && mine && j > i ? L[gram_index(j, i, cap)] : 0.0;
+            }
+#pragma unroll
+            for (int u = 0; u < kLoadBatch; ++u)
+                if (i0 + u <= last)
+                    tile[i0 + u - b0][lane] = v[u];
+        }
+        double acc = mine ? q[j] : 0.0;
+        for (int i0 = 0; i0 < b0; i0 += kLoadBatch) { // b0 is a multiple of the batch
+            double lv[kLoadBatch], hv[kLoadBatch];
+#pragma unroll
+            for (int u = 0; u < kLoadBatch; ++u) {
+                lv[u] = mine ? L[gram_index(j, i0 + u, cap)] : 0.0;
+                hv[u] = h[i0 + u];
+            }
+#pragma unroll
+            for (int u = 0; u < kLoadBatch; ++u)
+                acc = acc - lv[u] * hv[u];
+        }
+        __syncthreads();
         for (int i = b0; i <= last; ++i) {
             const double hi = __shfl(acc, i - b0, kWave);
             if (mine && j > i)
-                acc = acc - Lj[i] * hi;
+                acc = acc - tile[i - b0][lane] * hi;
         }
         if (mine) {
             h[j] = acc;
@@ -325,9 +374,41 @@ __global__ void __launch_bounds__(kWave) gram_solve_kernel(Tiles tiles, KrylovAr
         }
         __syncthreads();
     }
-    if (lane == 0)
-        a.st[s].rot_bad = apply_old_rotations(col, a.cs + (std::int64_t)s * a.cap,
-                                              a.sn + (std::int64_t)s * a.cap, k);
+    // Rotations 0..k-1 on rows 0..k (src/gmres.cpp:183-189); the carried row
+    // stays in a register of lane 0.
+    const double *cs = a.cs + (std::int64_t)s * cap, *sn = a.sn + (std::int64_t)s * cap;
+    double up = col[0];
+    bool bad = false;
+    for (int c0 = 0; c0 < k; c0 += kRotChunk) {
+        const int cnt = min(kRotChunk, k - c0);
+#pragma unroll
+        for (int u = 0; u < kRotChunk / kWave; ++u) {
+            const int r = lane + u * kWave;
+            if (r < cnt) {
+                lcs[r] = cs[c0 + r];
+                lsn[r] = sn[c0 + r];
+                lrow[r] = col[c0 + 1 + r];
+            }
+        }
+        __syncthreads();
+        if (lane == 0) {
+            for (int r = 0; r < cnt; ++r) {
+                const double lo = lrow[r], c = lcs[r], z = lsn[r];
+                const double top = c * up + z * lo;
+                up = -z * up + c * lo;
+                lrow[r] = top; // now row c0 + r
+                bad = bad || !isfinite(top) || !isfinite(up);
+            }
+        }
+        __syncthreads();
+        for (int r = lane; r < cnt; r += kWave)
+            col[c0 + r] = lrow[r];
+        __syncthreads();
+    }
+    if (lane == 0) {
+        col[k] = up;
+        a.st[s].rot_bad = bad;
+    }
 }
 
 // Pass 2: w <- w - sum_i h_i V_i (i ascending, in T as the reference's
@@ -340,31 +421,34 @@ gram_update_kernel(Tiles tiles, KrylovArgs<T> a, int k) {
     if (!a.st[s].active)
         return;
     const double *h = a.h + (std::int64_t)s * (a.cap + 1);
-    const int r0 = tiles.row0[t], r1 = tiles.row1[t];
-    T x[kUpdatePer];
-    int rows[kUpdatePer];
-#pragma unroll
-    for (int u = 0; u < kUpdatePer; ++u) {
-        rows[u] = r0 + (int)threadIdx.x + u * kUpdateThreads;
-        x[u] = rows[u] < r1 ? a.w[rows[u]] : T(0);
-    }
+    const int r0 = tiles.row0[t], nr = tiles.row1[t] - r0;
+    const int r = 2 * (int)threadIdx.x;
+    const bool aligned = (r0 & 1) == 0;
+    const int cnt = r + 1 < nr ? 2 : (r < nr ? 1 : 0);
+    T *__restrict__ w = a.w + r0 + r;
+    T x0 = T(0), x1 = T(0);
+    if (cnt == 2)
+        load_pair(w, aligned, x0, x1);
+    else if (cnt == 1)
+        x0 = w[0];
 #pragma unroll 4
     for (int i = 0; i <= k; ++i) {
-        const T *__restrict__ v = a.V[i];
+        const T *__restrict__ v = a.V[i] + r0 + r;
         const T hi = (T)h[i];
-#pragma unroll
-        for (int u = 0; u < kUpdatePer; ++u)
-            if (rows[u] < r1)
-                x[u] = x[u] - hi * v[rows[u]];
-    }
-    double vals[kUpdatePer];
-    int cnt = 0;
-#pragma unroll
-    for (int u = 0; u < kUpdatePer; ++u)
-        if (rows[u] < r1) {
-            a.w[rows[u]] = x[u];
-            vals[cnt++] = (double)x[u];
+        if (cnt == 2) {
+            T v0, v1;
+            load_pair(v, aligned, v0, v1);
+            x0 = x0 - hi * v0;
+            x1 = x1 - hi * v1;
+        } else if (cnt == 1) {
+            x0 = x0 - hi * v[0];
         }
+    }
+    if (cnt >= 1)
+        w[0] = x0;
+    if (cnt == 2)
+        w[1] = x1;
+    const double vals[2] = {(double)x0, (double)x1};
     store_norm_partial(vals, cnt, slot_ptr(a.part, tiles.ntiles, kSlotNorm), t, lds);
 }
 
@@ -385,10 +469,21 @@ normalize_givens_kernel(Tiles tiles, KrylovArgs<T> a, int k) {
         for (int r = tiles.row0[t] + threadIdx.x; r < tiles.row1[t]; r += kTileThreads)
             v[r] = a.w[r] / h;
     }
+    if (t != t0)
+        return;
+    double *H = a.H + (std::int64_t)s * a.hstride, *g = a.g + (std::int64_t)s * (a.cap + 1);
     if (writer)
-        givens_step(st, a.H + (std::int64_t)s * a.hstride, a.cs + (std::int64_t)s * a.cap,
-                    a.sn + (std::int64_t)s * a.cap, a.g + (std::int64_t)s * (a.cap + 1),
-                    a.y + (std::int64_t)s * a.cap, k, a.cap, hn);
+        givens_step(st, H, a.cs + (std::int64_t)s * a.cap, a.sn + (std::int64_t)s * a.cap, g, k,
+                    a.cap, hn);
+    __syncthreads();
+    if (threadIdx.x < kWave && st.triggered) {
+        const bool ok = back_substitute(H, g, a.y + (std::int64_t)s * a.cap, st.ccount);
+        if (!ok && threadIdx.x == 0) {
+            st.status = kGmNumericalFailure;
+            st.active = 0;
+            st.triggered = 0;
+        }
+    }
 }
 
 // Candidate x = x0 + sum_{j<count} y_j V_j (FP64, j ascending, src/gmres.cpp:60-63).

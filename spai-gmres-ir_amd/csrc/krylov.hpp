@@ -50,12 +50,17 @@ __host__ __device__ inline std::int64_t hcol_offset(int k) {
 inline std::int64_t hessenberg_size(int cap) {
     return hcol_offset(cap) + 2;
 }
-// Row j of the strictly lower Gram triangle (entries <V_j, V_i>, i < j), packed.
-__host__ __device__ inline std::int64_t grow_offset(int j) {
-    return (std::int64_t)j * (j - 1) / 2;
+// Strictly lower Gram triangle L(j, i) = <V_j, V_i>, i < j <= cap, packed by
+// columns: column i holds rows i+1..cap, so the lanes of a wave that own
+// consecutive rows j read consecutive words.
+__host__ __device__ inline std::int64_t gcol_offset(int i, int cap) {
+    return (std::int64_t)i * cap - (std::int64_t)i * (i - 1) / 2;
+}
+__host__ __device__ inline std::int64_t gram_index(int j, int i, int cap) {
+    return gcol_offset(i, cap) + (j - i - 1);
 }
 inline std::int64_t gram_size(int cap) {
-    return grow_offset(cap + 1);
+    return (std::int64_t)cap * (cap + 1) / 2;
 }
 
 struct SysState {

@@ -64,6 +64,8 @@ struct Caps {
     int mcap, qcap, ccap;
 };
 
+constexpr int kFuse = 2; // trailing columns reflected together (register budget)
+
 template <typename T> struct Lane {
     T *d;          // float work area
     int *i;        // index work area
@@ -71,6 +73,25 @@ template <typename T> struct Lane {
     __device__ T &D(int e) const { return d[dbase + (std::int64_t)e * kWave]; }
     __device__ int &I(int e) const { return i[ibase + (std::int64_t)e * kWave]; }
 };
+
+// Householder vector from x (held in v[0..len), 2-norm `length` > 0):
+// v = x / length; v_0 += sign(v_0); v /= ||v|| (src/spai.cpp:48-56).  Returns
+// the diagonal entry -copysign(length, x_0).
+template <typename T>
+__device__ T householder_vector(const Lane<T> &L, int ov, int len, T length) {
+    const T diagonal = -copysign(length, L.D(ov));
+    T vnorm = 0;
+    for (int r = 0; r < len; ++r) {
+        T x = L.D(ov + r) / length;
+        if (r == 0)
+            x += copysign(T(1), x);
+        L.D(ov + r) = x;
+        vnorm = hypot_ref(vnorm, x);
+    }
+    for (int r = 0; r < len; ++r)
+        L.D(ov + r) = L.D(ov + r) / vnorm;
+    return diagonal;
+}
 
 } // namespace
 
@@ -190,91 +211,132 @@ spai_columns_kernel(int n, const int *__restrict__ aptr, const int *__restrict__
             }
             T min_pivot = T(INFINITY);
             int fit_status = kSpaiSuccess;
+            // Step j reflects column j+1 first: its updated rows j+1.. are the
+            // next Householder vector, so the next norm chain runs in the same
+            // pass.  The other trailing columns (and the right side, slot q)
+            // are then reflected in one pass that also accumulates their dot
+            // products with the next vector.  Every value sees the
+            // reference's operations in the reference's order
+            // (src/spai.cpp:41-78); only the number of passes over the work
+            // area changes.  Status precedence is kept: a failure while
+            // reflecting step j is reported before the next vector's norm.
+            int oVc = oV, oVn = oRes; // Householder vectors of steps j, j+1
+            const int oDot = oSc;     // per trailing slot: dot with the current v
+            auto slot_base = [&](int c) { return c < q ? oR + c * M : oRhs; };
+            T length = 0;
+            for (int r = 0; r < m; ++r) {
+                const T x = L.D(oR + r);
+                L.D(oVc + r) = x;
+                length = hypot_ref(length, x);
+            }
+            T diagonal = 0;
+            if (!isfinite(length))
+                fit_status = kSpaiNumericalFailure;
+            else if (length == 0)
+                fit_status = kSpaiRankDeficient;
+            else
+                diagonal = householder_vector(L, oVc, m, length);
+            if (fit_status == kSpaiSuccess) {
+                // dots of step 0: slots 1..q (q = right side), groups of four
+                for (int c0 = 1; c0 <= q; c0 += 4) {
+                    const int nb = q + 1 - c0 < 4 ? q + 1 - c0 : 4;
+                    T d[4] = {0, 0, 0, 0};
+                    for (int r = 0; r < m; ++r) {
+                        const T vr = L.D(oVc + r);
+#pragma unroll
+                        for (int u = 0; u < 4; ++u)
+                            if (u < nb)
+                                d[u] += vr * L.D(slot_base(c0 + u) + r);
+                    }
+#pragma unroll
+                    for (int u = 0; u < 4; ++u)
+                        if (u < nb)
+                            L.D(oDot + c0 + u) = d[u];
+                }
+            }
             for (int j = 0; j < q && fit_status == kSpaiSuccess; ++j) {
                 const int len = m - j;
-                T length = 0;
-                for (int r = 0; r < len; ++r) {
-                    const T x = L.D(oR + j * M + j + r);
-                    L.D(oV + r) = x;
-                    length = hypot_ref(length, x);
+                const bool next = j + 1 < q;
+                bool ok = true;
+                T nlength = 0, ndiagonal = 0;
+                int nstatus = kSpaiSuccess;
+                if (next) {
+                    const T dt = L.D(oDot + j + 1);
+                    const int base = oR + (j + 1) * M + j;
+                    T &x0 = L.D(base);
+                    x0 -= T(2) * L.D(oVc) * dt;
+                    ok = isfinite(x0);
+                    for (int r = 1; r < len; ++r) {
+                        T &x = L.D(base + r);
+                        x -= T(2) * L.D(oVc + r) * dt;
+                        ok = ok && isfinite(x);
+                        L.D(oVn + r - 1) = x;
+                        nlength = hypot_ref(nlength, x);
+                    }
+                    if (!isfinite(nlength))
+                        nstatus = kSpaiNumericalFailure;
+                    else if (nlength == 0)
+                        nstatus = kSpaiRankDeficient;
+                    else if (ok)
+                        ndiagonal = householder_vector(L, oVn, len - 1, nlength);
                 }
-                if (!isfinite(length)) {
+                const bool fuse = next && ok && nstatus == kSpaiSuccess;
+                // the remaining slots (j+2..q, or just the right side on the
+                // last step): reflect with v_j; fused dots with v_{j+1}, in
+                // groups of kFuse slots that share each load of v_j and v_{j+1}
+                for (int c0 = next ? j + 2 : j + 1; c0 <= q && ok; c0 += kFuse) {
+                    const int nb = q + 1 - c0 < kFuse ? q + 1 - c0 : kFuse;
+                    T dt[kFuse], dn[kFuse];
+                    int base[kFuse];
+#pragma unroll
+                    for (int u = 0; u < kFuse; ++u) {
+                        const int c = c0 + (u < nb ? u : 0);
+                        dt[u] = L.D(oDot + c);
+                        dn[u] = 0;
+                        base[u] = slot_base(c) + j;
+                    }
+                    {
+                        const T v2 = T(2) * L.D(oVc);
+#pragma unroll
+                        for (int u = 0; u < kFuse; ++u)
+                            if (u < nb) {
+                                T &y = L.D(base[u]);
+                                y -= v2 * dt[u];
+                                ok = ok && isfinite(y);
+                            }
+                    }
+                    for (int r = 1; r < len; ++r) {
+                        const T w2 = T(2) * L.D(oVc + r);
+                        const T vn = fuse ? L.D(oVn + r - 1) : T(0);
+#pragma unroll
+                        for (int u = 0; u < kFuse; ++u)
+                            if (u < nb) {
+                                T &x = L.D(base[u] + r);
+                                x -= w2 * dt[u];
+                                ok = ok && isfinite(x);
+                                dn[u] += vn * x;
+                            }
+                    }
+#pragma unroll
+                    for (int u = 0; u < kFuse; ++u)
+                        if (u < nb)
+                            L.D(oDot + c0 + u) = dn[u];
+                }
+                if (!ok) {
                     fit_status = kSpaiNumericalFailure;
                     break;
-                }
-                if (length == 0) {
-                    fit_status = kSpaiRankDeficient;
-                    break;
-                }
-                const T v0 = L.D(oV);
-                const T diagonal = -copysign(length, v0);
-                T vnorm = 0;
-                for (int r = 0; r < len; ++r) {
-                    T x = L.D(oV + r) / length;
-                    if (r == 0)
-                        x += copysign(T(1), x);
-                    L.D(oV + r) = x;
-                    vnorm = hypot_ref(vnorm, x);
-                }
-                for (int r = 0; r < len; ++r)
-                    L.D(oV + r) = L.D(oV + r) / vnorm;
-                // Reflect the trailing columns c > j and the right side
-                // (column j itself is overwritten below).  Columns are
-                // independent, so groups of four share each load of v while
-                // every column keeps its own sequential dot product and
-                // update, in the reference's order (src/spai.cpp:59-75).
-                const int ntrail = q - j; // q-1-j columns + the right side
-                for (int c0 = 0; c0 < ntrail && fit_status == kSpaiSuccess; c0 += 4) {
-                    const int nb = ntrail - c0 < 4 ? ntrail - c0 : 4;
-                    int base[4];
-#pragma unroll
-                    for (int u = 0; u < 4; ++u) {
-                        const int c = j + 1 + c0 + (u < nb ? u : 0);
-                        base[u] = (c < q ? oR + c * M : oRhs) + j;
-                    }
-                    if (nb == 4) {
-                        T d0 = 0, d1 = 0, d2 = 0, d3 = 0;
-                        for (int r = 0; r < len; ++r) {
-                            const T vr = L.D(oV + r);
-                            d0 += vr * L.D(base[0] + r);
-                            d1 += vr * L.D(base[1] + r);
-                            d2 += vr * L.D(base[2] + r);
-                            d3 += vr * L.D(base[3] + r);
-                        }
-                        bool ok = true;
-                        for (int r = 0; r < len; ++r) {
-                            const T v2 = T(2) * L.D(oV + r);
-                            T &x0 = L.D(base[0] + r);
-                            T &x1 = L.D(base[1] + r);
-                            T &x2 = L.D(base[2] + r);
-                            T &x3 = L.D(base[3] + r);
-                            x0 -= v2 * d0;
-                            x1 -= v2 * d1;
-                            x2 -= v2 * d2;
-                            x3 -= v2 * d3;
-                            ok = ok && isfinite(x0) && isfinite(x1) && isfinite(x2) && isfinite(x3);
-                        }
-                        if (!ok)
-                            fit_status = kSpaiNumericalFailure;
-                    } else {
-                        for (int u = 0; u < nb; ++u) {
-                            T dot = 0;
-                            for (int r = 0; r < len; ++r)
-                                dot += L.D(oV + r) * L.D(base[u] + r);
-                            bool ok = true;
-                            for (int r = 0; r < len; ++r) {
-                                T &x = L.D(base[u] + r);
-                                x -= T(2) * L.D(oV + r) * dot;
-                                ok = ok && isfinite(x);
-                            }
-                            if (!ok)
-                                fit_status = kSpaiNumericalFailure;
-                        }
-                    }
                 }
                 L.D(oR + j * M + j) = diagonal;
                 const T ad = fabs(diagonal);
                 min_pivot = ad < min_pivot ? ad : min_pivot;
+                if (next && nstatus != kSpaiSuccess) {
+                    fit_status = nstatus;
+                    break;
+                }
+                diagonal = ndiagonal;
+                const int t = oVc;
+                oVc = oVn;
+                oVn = t;
             }
             if (fit_status == kSpaiSuccess) {
                 for (int j = q - 1; j >= 0; --j) {

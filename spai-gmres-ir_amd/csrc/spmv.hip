@@ -12,6 +12,16 @@
 
 namespace bspai_hip {
 
+// Workgroups are dispatched to the 8 XCDs round-robin (block b on XCD b mod 8),
+// and each XCD has its own L2.  Remapping gives every XCD one contiguous run
+// of row blocks, so the x entries that neighbouring rows share (stencil
+// neighbours, the rows of one system) are reused inside one L2.
+constexpr int kXcds = 8;
+__device__ inline int xcd_block(int b, int nb) {
+    const int q = nb / kXcds, r = nb % kXcds, x = b % kXcds;
+    return x * q + min(x, r) + b / kXcds;
+}
+
 // Row-group kernel: LANES lanes per row stream the row with unit stride and
 // reduce with a fixed butterfly (for long rows).
 template <int LANES, typename TV, typename TX, typename TA, typename TY>
@@ -19,7 +29,8 @@ __global__ void __launch_bounds__(kSpmvRows)
 spmv_group_kernel(Tiles tiles, const int *__restrict__ flag, int flag_stride,
                   const int *__restrict__ ptr, const int *__restrict__ col,
                   const TV *__restrict__ val, const TX *__restrict__ x, TY *__restrict__ y) {
-    const int t = blockIdx.x / kSpmvSub, sb = blockIdx.x % kSpmvSub;
+    const int b = xcd_block(blockIdx.x, gridDim.x);
+    const int t = b / kSpmvSub, sb = b % kSpmvSub;
     if (flag && !flag[(std::int64_t)tiles.sys[t] * flag_stride])
         return;
     const int rbeg = tiles.row0[t] + sb * kSpmvRows;
@@ -49,7 +60,8 @@ spmv_stream_kernel(Tiles tiles, const int *__restrict__ flag, int flag_stride,
                    const TV *__restrict__ val, const TX *__restrict__ x, TY *__restrict__ y) {
     __shared__ int rp[kSpmvRows + 1];
     __shared__ TA prod[kStreamChunk];
-    const int t = blockIdx.x / kSpmvSub, sb = blockIdx.x % kSpmvSub;
+    const int b = xcd_block(blockIdx.x, gridDim.x);
+    const int t = b / kSpmvSub, sb = b % kSpmvSub;
     if (flag && !flag[(std::int64_t)tiles.sys[t] * flag_stride])
         return;
     const int rbeg = tiles.row0[t] + sb * kSpmvRows;
@@ -64,10 +76,29 @@ spmv_stream_kernel(Tiles tiles, const int *__restrict__ flag, int flag_stride,
     const int tid = threadIdx.x;
     const int mb = tid < nr ? rp[tid] : 0, me = tid < nr ? rp[tid + 1] : 0;
     TA acc = 0;
+    constexpr int per = kStreamChunk / kSpmvRows;
     for (int c0 = e0; c0 < e1; c0 += kStreamChunk) {
         const int c1 = min(e1, c0 + kStreamChunk);
-        for (int e = c0 + tid; e < c1; e += kSpmvRows)
-            prod[e - c0] = (TA)val[e] * (TA)x[col[e]];
+        // All column loads first, then the value loads and x gathers: every
+        // thread keeps `per` independent loads of each kind in flight.
+        int cj[per];
+#pragma unroll
+        for (int u = 0; u < per; ++u) {
+            const int e = c0 + tid + u * kSpmvRows;
+            cj[u] = e < c1 ? col[e] : 0;
+        }
+        TA pr[per];
+#pragma unroll
+        for (int u = 0; u < per; ++u) {
+            const int e = c0 + tid + u * kSpmvRows;
+            pr[u] = e < c1 ? (TA)val[e] * (TA)x[cj[u]] : TA(0);
+        }
+#pragma unroll
+        for (int u = 0; u < per; ++u) {
+            const int e = c0 + tid + u * kSpmvRows;
+            if (e < c1)
+                prod[e - c0] = pr[u];
+        }
         __syncthreads();
         const int lo = max(mb, c0), hi = min(me, c1);
         for (int e = lo; e < hi; ++e)
@@ -94,6 +125,10 @@ void launch_spmv(hipStream_t s, int lanes, const Tiles &tiles, const int *flag, 
         hipLaunchKernelGGL((spmv_stream_kernel<TV, TX, TA, TY>), grid, block, 0, s, tiles, flag,
                            flag_stride, ptr, col, val, x, y);
         break;
+    case -1:
+        hipLaunchKernelGGL((spmv_group_kernel<1, TV, TX, TA, TY>), grid, block, 0, s, tiles, flag,
+                           flag_stride, ptr, col, val, x, y);
+        break;
         BSPAI_SPMV_CASE(2)
         BSPAI_SPMV_CASE(4)
         BSPAI_SPMV_CASE(8)
@@ -101,7 +136,7 @@ void launch_spmv(hipStream_t s, int lanes, const Tiles &tiles, const int *flag, 
         BSPAI_SPMV_CASE(32)
         BSPAI_SPMV_CASE(64)
     default:
-        throw std::invalid_argument("spmv lanes must be a power of two <= 64");
+        throw std::invalid_argument("spmv lanes must be -1 or a power of two <= 64");
     }
 #undef BSPAI_SPMV_CASE
     HIP_LAUNCH_CHECK();

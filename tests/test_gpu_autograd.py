@@ -10,7 +10,9 @@ checked against
     noise of the finite difference itself: |fd - g| <= 5e-3 |g| + 5e-3 max|g|
     (with exact LU solves the h = 1e-6 differences already scatter by up to
     ~1e-2 relative on small-gradient edges, because L ~ 3e6);
-  * the closed form dL/dA = -lambda x^T on a tiny dense system.
+  * the closed form dL/dA = -lambda x^T on a tiny dense system;
+and the adjoint solve itself (both precisions) against a sparse LU solve of
+A^T lambda = g on the nonsymmetric 64x64 convection-diffusion system.
 
 Both solves stop on the reference's backward-error test (eta <= 10u).  A
 relative residual of 1e-12 lies below this system's FP64 floor: u ||(|A||x|)||
@@ -46,7 +48,10 @@ def _loss_and_grad(handle, model, theta_np, b_np, xs_np, **opts):
     return loss.item(), theta.grad.detach().cpu().numpy(), x.detach().cpu().numpy()
 
 
-OPTS = dict(precision="mixed", tolerance=1e-4, max_iterations=600)
+# Forward mixed; the adjoint in FP64: its unpreconditioned residual test has
+# an FP32 floor of u32 ||A|| ||lambda|| / ||x - x*|| ~ 1.5e-3 on this system.
+OPTS = dict(precision="mixed", tolerance=1e-3, max_iterations=600, relres_target=2e-12,
+            adjoint=dict(precision="fp64", tolerance=1e-8, relres_target=0.0))
 
 
 def test_adjoint_gradient_vs_fp64_reference(ctx, c5):
@@ -126,3 +131,29 @@ def test_sddmm_closed_form(ctx):
     expect = -lam[rows] * xe[a.col]
     assert np.allclose(vals.grad.cpu().numpy(), expect, rtol=1e-9, atol=1e-12)
     assert np.allclose(b.grad.cpu().numpy(), lam, rtol=1e-9, atol=1e-12)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_adjoint_solve_nonsymmetric(ctx, precision):
+    """A^T lambda = g right-preconditioned by P^T on the 64x64
+    convection-diffusion matrix (A != A^T), under IR to the reference's
+    backward-error test: lambda matches a sparse LU solve of A^T to 1e-10."""
+    import scipy.sparse.linalg as spla
+
+    from bspai_hip import api
+    from bspai_hip import problems as P
+
+    a = P.convection_diffusion_2d()
+    m = api.Matrix(a, ctx)
+    spai = api.construct_spai(m, 0.3, -(-a.n // 8), 8, precision=precision)
+    assert spai.status == "success"
+    g = np.random.default_rng(11).standard_normal(a.n)
+    tol = 1e-8 if precision == "fp64" else 1e-4
+    r = api.adjoint_solve(m.transpose(), spai.matrix(transpose=True), g, precision=precision,
+                          tolerance=tol)
+    assert r.status == "success" and r.converged, (r.status, r.message)
+    A = a.to_scipy()
+    ref = spla.spsolve(A.T.tocsc(), g)
+    lam = np.asarray(r.solution)
+    assert np.linalg.norm(lam - ref) <= 1e-10 * np.linalg.norm(ref)
+    assert np.linalg.norm(g - A.T @ lam) <= 1e-12 * np.linalg.norm(g)
