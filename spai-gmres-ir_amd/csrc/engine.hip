@@ -27,6 +27,8 @@ void sync(const Context &c) {
 // Timer labels: the kernel names rocprofv3 reports, plus the operand role.
 template <typename T> constexpr const char *kSpaiKernelName = "spai_columns_kernel<double>";
 template <> constexpr const char *kSpaiKernelName<float> = "spai_columns_kernel<float>";
+template <typename T> constexpr const char *kSpaiBlockKernelName = "spai_block_kernel<double>";
+template <> constexpr const char *kSpaiBlockKernelName<float> = "spai_block_kernel<float>";
 
 KernelTimer::~KernelTimer() {
     for (hipEvent_t e : pool_)
@@ -168,7 +170,8 @@ __global__ void collect_overflow_kernel(int nwork, const int *work, const int *s
 // Reduce per-column outcomes: first failing column, totals.
 __global__ void column_summary_kernel(int n, const int *status, const int *fits, const int *growths,
                                       const int *clamps, const int *q, const double *flops,
-                                      int *first_fail, unsigned long long *tot, double *ftot) {
+                                      const int *pass, int npass, int *first_fail,
+                                      unsigned long long *tot, double *ftot) {
     // Block sums first (exact: the counts stay far below 2^53), one atomic
     // per block and counter.
     __shared__ double lds[256 / kWave];
@@ -182,9 +185,12 @@ __global__ void column_summary_kernel(int n, const int *status, const int *fits,
         if (threadIdx.x == 0)
             atomicAdd(tot + c, (unsigned long long)v);
     }
-    const double f = block_sum(in ? flops[k] : 0.0, lds);
-    if (threadIdx.x == 0)
-        atomicAdd(ftot, f); // telemetry only: summation order does not matter
+    // QR flops per capacity pass (telemetry only: summation order does not matter)
+    for (int p = 0; p < npass; ++p) {
+        const double f = block_sum(in && pass[k] == p ? flops[k] : 0.0, lds);
+        if (threadIdx.x == 0)
+            atomicAdd(ftot + p, f);
+    }
 }
 // Column pool pointers: per column (pass, offset) -> the pools' base pointers.
 struct PoolTable {
@@ -328,10 +334,70 @@ void spai_build_typed(Context &c, const DevCsr &a, const T *aval, T *d, double e
     const std::int64_t area_cap =
         std::min<std::int64_t>(8LL << 30, std::max<std::int64_t>(1LL << 28, free_bytes / 4));
     DeviceBuffer dwork, iwork;
+    std::vector<std::string> pass_labels; // timer label of each capacity pass
     while (nwork > 0) {
         if (out.passes >= 8)
             throw std::runtime_error("SPAI column exceeds the largest capacity bucket");
         const int pass = out.passes++;
+        if (pass > 0) {
+            // Columns that outgrew a bucket: one workgroup per column, caps
+            // growing 4x per pass (never beyond n).
+            if (pass == 1) {
+                mcap = 2048;
+                qcap = 128;
+                ccap = 16384;
+            } else {
+                mcap *= 4;
+                qcap *= 4;
+                ccap *= 4;
+            }
+            mcap = (int)std::min<std::int64_t>(mcap, n);
+            qcap = (int)std::min<std::int64_t>(qcap, n);
+            ccap = (int)std::min<std::int64_t>(ccap, n);
+            const int words = (int)((n + 31) / 32);
+            const std::int64_t bd = spai_block_doubles(mcap, qcap, ccap);
+            const std::int64_t bi = spai_block_ints(mcap, qcap, ccap, words, n);
+            const std::int64_t per_block = bd * (std::int64_t)sizeof(T) + bi * 4;
+            std::int64_t blocks = std::min<std::int64_t>(nwork, prop.multiProcessorCount * 4LL);
+            // Large buckets hold tens of MB per workgroup: allow up to a quarter
+            // of free memory (48 GiB at most) so that a full wave of
+            // workgroups stays resident.
+            const std::int64_t block_cap =
+                std::min<std::int64_t>(48LL << 30, std::max<std::int64_t>(area_cap, free_bytes / 4));
+            blocks = std::max<std::int64_t>(1, std::min<std::int64_t>(blocks, block_cap / per_block));
+            dwork.resize((std::size_t)(blocks * bd * (std::int64_t)sizeof(T)));
+            iwork.resize((std::size_t)(blocks * bi * 4));
+            // bitmaps clear, position maps -1 (the kernel keeps them so)
+            HIP_CHECK(hipMemsetAsync(iwork.as<int>(), 0, (std::size_t)(blocks * bi * 4), s));
+            launch_fill_int(s, blocks, bi, bi - n, n, iwork.as<int>(), -1);
+            out.pool_j.emplace_back((std::size_t)nwork * qcap);
+            out.pool_c.emplace_back((std::size_t)nwork * qcap * sizeof(T));
+            HIP_CHECK(hipMemsetAsync(counter.data(), 0, 2 * sizeof(int), s));
+            SpaiColumnOut o{out.c_status.data(), out.c_fits.data(),  out.c_growths.data(),
+                            out.c_clamps.data(), out.c_q.data(),     out.c_res.data(),
+                            out.c_piv.data(),    out.c_flops.data(), out.c_off.data(),
+                            out.c_pass.data(),   pass};
+            const std::string label = std::string(kSpaiBlockKernelName<T>) + " pass " +
+                                      std::to_string(pass) + " (" + std::to_string(nwork) +
+                                      " columns)";
+            pass_labels.push_back(label);
+            timed(c, label.c_str(), 0.0, 0.0, [&] {
+                launch_spai_block<T>(s, (int)blocks, n, a.ptr.data(), a.col.data(), aval, d,
+                                     bptr.data(), bcol.data(), (T)eps, alpha, beta, work.data(),
+                                     nwork, counter.data(), mcap, qcap, ccap, dwork.as<T>(),
+                                     iwork.as<int>(), o, out.pool_j.back().data(),
+                                     out.pool_c.back().as<T>(), qcap);
+            });
+            hipLaunchKernelGGL(collect_overflow_kernel, dim3(ceil_div(nwork, 256)), dim3(256), 0,
+                               s, nwork, work.data(), out.c_status.data(), next.data(),
+                               counter.data() + 1);
+            int nover = 0;
+            download(c, &nover, counter.data() + 1, 1);
+            sync(c);
+            std::swap(work, next);
+            nwork = nover;
+            continue;
+        }
         const std::int64_t ld = spai_lane_doubles(mcap, qcap, ccap);
         const std::int64_t li = spai_lane_ints(mcap, qcap, ccap);
         // Lanes in flight: bounded by the device and by the work-area cap
@@ -352,6 +418,7 @@ void spai_build_typed(Context &c, const DevCsr &a, const T *aval, T *d, double e
                         out.c_clamps.data(), out.c_q.data(), out.c_res.data(),
                         out.c_piv.data(), out.c_flops.data(), out.c_off.data(),
                         out.c_pass.data(), pass};
+        pass_labels.push_back(kSpaiKernelName<T>);
         timed(c, kSpaiKernelName<T>, 0.0, 0.0, [&] {
             launch_spai_columns<T>(s, blocks, threads, n, a.ptr.data(), a.col.data(), aval, d,
                                    bptr.data(), bcol.data(), (T)eps, alpha, beta, work.data(),
@@ -374,17 +441,17 @@ void spai_build_typed(Context &c, const DevCsr &a, const T *aval, T *d, double e
     // ---- summary and P assembly (src/spai.cpp:352-372)
     DeviceArray<int> ff(2);
     DeviceArray<unsigned long long> tot(4);
-    DeviceArray<double> ftot(1);
+    DeviceArray<double> ftot(8);
     {
         const int init[2] = {INT_MAX, INT_MAX};
         upload(c, ff.data(), init, 2);
         HIP_CHECK(hipMemsetAsync(tot.data(), 0, 4 * sizeof(unsigned long long), s));
-        HIP_CHECK(hipMemsetAsync(ftot.data(), 0, sizeof(double), s));
+        HIP_CHECK(hipMemsetAsync(ftot.data(), 0, 8 * sizeof(double), s));
     }
     hipLaunchKernelGGL(column_summary_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, s, n,
                        out.c_status.data(), out.c_fits.data(), out.c_growths.data(),
-                       out.c_clamps.data(), out.c_q.data(), out.c_flops.data(), ff.data(),
-                       tot.data(), ftot.data());
+                       out.c_clamps.data(), out.c_q.data(), out.c_flops.data(),
+                       out.c_pass.data(), out.passes, ff.data(), tot.data(), ftot.data());
     PoolTable pt{};
     for (int p = 0; p < out.passes; ++p) {
         pt.j[p] = out.pool_j[p].data();
@@ -403,9 +470,14 @@ void spai_build_typed(Context &c, const DevCsr &a, const T *aval, T *d, double e
     download(c, hff, ff.data(), 2);
     download(c, htot, tot.data(), 4);
     download(c, &pnnz, out.p.ptr.data() + n, 1);
-    download(c, &out.qr_flops, ftot.data(), 1);
+    double pass_flops[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    download(c, pass_flops, ftot.data(), 8);
     sync(c);
-    c.timer.add_flops(kSpaiKernelName<T>, out.qr_flops);
+    out.qr_flops = 0;
+    for (int p = 0; p < out.passes; ++p) {
+        out.qr_flops += pass_flops[p];
+        c.timer.add_flops(pass_labels[p].c_str(), pass_flops[p]);
+    }
     out.fits = (std::int64_t)htot[0];
     out.growths = (std::int64_t)htot[1];
     out.clamps = (std::int64_t)htot[2];

@@ -44,6 +44,18 @@ void launch_spai_columns(hipStream_t s, int blocks, int threads, int n, const in
                          int nwork, int *cursor, int mcap, int qcap, int ccap, T *dwork,
                          int *iwork, SpaiColumnOut out, int *out_j, T *out_c,
                          std::int64_t out_stride);
+// Workgroup-per-column kernel for the columns that outgrow the lane kernel.
+std::int64_t spai_block_doubles(int mcap, int qcap, int ccap);
+std::int64_t spai_block_ints(int mcap, int qcap, int ccap, int words, int n);
+// p[b*stride + offset + i] = value for b < blocks, i < count.
+void launch_fill_int(hipStream_t s, std::int64_t blocks, std::int64_t stride, std::int64_t offset,
+                     int count, int *p, int value);
+template <typename T>
+void launch_spai_block(hipStream_t s, int blocks, int n, const int *aptr, const int *acol,
+                       const T *aval, const T *dscale, const int *bptr, const int *bcol, T eps,
+                       long long alpha, int beta, const int *worklist, int nwork, int *cursor,
+                       int mcap, int qcap, int ccap, T *dwork, int *iwork, SpaiColumnOut out,
+                       int *out_j, T *out_c, std::int64_t out_stride);
 template <typename T>
 void launch_scale_rows(hipStream_t s, int n, const int *ptr, const int *col, const T *val, T *d,
                        int *first_bad);

@@ -77,20 +77,56 @@ template <typename T> struct Lane {
 // Householder vector from x (held in v[0..len), 2-norm `length` > 0):
 // v = x / length; v_0 += sign(v_0); v /= ||v|| (src/spai.cpp:48-56).  Returns
 // the diagonal entry -copysign(length, x_0).
+// The sequential loops below load kLB elements ahead of their dependent
+// chains (hypot, accumulation), so a lane keeps several loads in flight.
+constexpr int kLB = 3;
+
 template <typename T>
 __device__ T householder_vector(const Lane<T> &L, int ov, int len, T length) {
     const T diagonal = -copysign(length, L.D(ov));
     T vnorm = 0;
-    for (int r = 0; r < len; ++r) {
-        T x = L.D(ov + r) / length;
-        if (r == 0)
-            x += copysign(T(1), x);
-        L.D(ov + r) = x;
-        vnorm = hypot_ref(vnorm, x);
+    for (int r0 = 0; r0 < len; r0 += kLB) {
+        T x[kLB];
+#pragma unroll
+        for (int u = 0; u < kLB; ++u)
+            x[u] = L.D(ov + min(r0 + u, len - 1));
+#pragma unroll
+        for (int u = 0; u < kLB; ++u)
+            if (r0 + u < len) {
+                T y = x[u] / length;
+                if (r0 + u == 0)
+                    y += copysign(T(1), y);
+                L.D(ov + r0 + u) = y;
+                vnorm = hypot_ref(vnorm, y);
+            }
     }
-    for (int r = 0; r < len; ++r)
-        L.D(ov + r) = L.D(ov + r) / vnorm;
+    for (int r0 = 0; r0 < len; r0 += kLB) {
+        T x[kLB];
+#pragma unroll
+        for (int u = 0; u < kLB; ++u)
+            x[u] = L.D(ov + min(r0 + u, len - 1));
+#pragma unroll
+        for (int u = 0; u < kLB; ++u)
+            if (r0 + u < len)
+                L.D(ov + r0 + u) = x[u] / vnorm;
+    }
     return diagonal;
+}
+
+// hypot chain over a lane array, loads batched ahead of the chain.
+template <typename T> __device__ T lane_hypot_chain(const Lane<T> &L, int o, int len) {
+    T acc = 0;
+    for (int r0 = 0; r0 < len; r0 += kLB) {
+        T x[kLB];
+#pragma unroll
+        for (int u = 0; u < kLB; ++u)
+            x[u] = L.D(o + min(r0 + u, len - 1));
+#pragma unroll
+        for (int u = 0; u < kLB; ++u)
+            if (r0 + u < len)
+                acc = hypot_ref(acc, x[u]);
+    }
+    return acc;
 }
 
 } // namespace
@@ -223,12 +259,9 @@ spai_columns_kernel(int n, const int *__restrict__ aptr, const int *__restrict__
             int oVc = oV, oVn = oRes; // Householder vectors of steps j, j+1
             const int oDot = oSc;     // per trailing slot: dot with the current v
             auto slot_base = [&](int c) { return c < q ? oR + c * M : oRhs; };
-            T length = 0;
-            for (int r = 0; r < m; ++r) {
-                const T x = L.D(oR + r);
-                L.D(oVc + r) = x;
-                length = hypot_ref(length, x);
-            }
+            for (int r = 0; r < m; ++r)
+                L.D(oVc + r) = L.D(oR + r);
+            T length = lane_hypot_chain(L, oVc, m);
             T diagonal = 0;
             if (!isfinite(length))
                 fit_status = kSpaiNumericalFailure;
@@ -266,12 +299,23 @@ spai_columns_kernel(int n, const int *__restrict__ aptr, const int *__restrict__
                     T &x0 = L.D(base);
                     x0 -= T(2) * L.D(oVc) * dt;
                     ok = isfinite(x0);
-                    for (int r = 1; r < len; ++r) {
-                        T &x = L.D(base + r);
-                        x -= T(2) * L.D(oVc + r) * dt;
-                        ok = ok && isfinite(x);
-                        L.D(oVn + r - 1) = x;
-                        nlength = hypot_ref(nlength, x);
+                    for (int r0 = 1; r0 < len; r0 += kLB) {
+                        T xb[kLB], vb[kLB];
+#pragma unroll
+                        for (int u = 0; u < kLB; ++u) {
+                            const int r = min(r0 + u, len - 1);
+                            xb[u] = L.D(base + r);
+                            vb[u] = L.D(oVc + r);
+                        }
+#pragma unroll
+                        for (int u = 0; u < kLB; ++u)
+                            if (r0 + u < len) {
+                                const T x = xb[u] - T(2) * vb[u] * dt;
+                                L.D(base + r0 + u) = x;
+                                ok = ok && isfinite(x);
+                                L.D(oVn + r0 + u - 1) = x;
+                                nlength = hypot_ref(nlength, x);
+                            }
                     }
                     if (!isfinite(nlength))
                         nstatus = kSpaiNumericalFailure;
@@ -380,14 +424,10 @@ spai_columns_kernel(int n, const int *__restrict__ aptr, const int *__restrict__
                 }
             }
             L.D(oRes + pk) -= T(1);
-            T rnorm = 0;
-            bool res_finite = true;
-            for (int r = 0; r < m; ++r) {
-                const T x = L.D(oRes + r);
-                res_finite = res_finite && isfinite(x);
-                rnorm = hypot_ref(rnorm, x);
-            }
-            if (!res_finite || !isfinite(rnorm)) {
+            // (a non-finite entry makes the chain non-finite: hypot returns
+            // inf for an infinite argument and NaN for a NaN one otherwise)
+            const T rnorm = lane_hypot_chain(L, oRes, m);
+            if (!isfinite(rnorm)) {
                 status = kSpaiNumericalFailure;
                 break;
             }
@@ -605,6 +645,829 @@ template void launch_spai_columns<float>(hipStream_t, int, int, int, const int *
                                          float, long long, int, const int *, int, int *, int,
                                          int, int, float *, int *, SpaiColumnOut, int *, float *,
                                          std::int64_t);
+
+// ================================================================ block kernel
+// Columns that outgrow the lane kernel's caps (long rows, wide patterns,
+// thousands of candidates: the power-law matrices) are fitted by one
+// 256-thread workgroup each.  The arithmetic is the lane kernel's, in the
+// same order, so FP64 stays bit-identical to the CPU:
+//   * every hypot chain, dot product, the back substitution and the mean of
+//     the scores run sequentially (one thread, or one thread per column);
+//   * elementwise work runs in parallel: gathers, Householder vector scaling,
+//     the reflection of every (row, column) entry, per-row residual sums in
+//     pattern order, per-candidate scores;
+//   * index sets are formed with a per-workgroup bitmap over the n columns
+//     (mark, then an ordered compaction that also clears it), replacing the
+//     repeated sorted merges of the lane kernel, and the beta smallest
+//     (score, index) pairs are found by block-wide argmin rounds.
+// Per workgroup: T area  R[Q][M] | G[Q][M] | rhs[M] | v[M] | res[M] | score[C] | coef[Q] |
+//                         dot[Q+1]
+//                int area I[M] | J[Q] | Jt[Q] | cand[C] | clamp[C] | sel[Q] | bitmap[W] |
+//                         pos[n]
+// G keeps the gathered B(I,J) for the residual; pos maps a column index to its
+// position in I (-1 elsewhere), replacing binary searches.
+constexpr int kBlk = 256;
+
+struct BlockCaps {
+    int mcap, qcap, ccap, words;
+};
+
+std::int64_t spai_block_doubles(int M, int Q, int C) {
+    return 2LL * Q * M + 3LL * M + C + 2LL * Q + 1;
+}
+std::int64_t spai_block_ints(int M, int Q, int C, int words, int n) {
+    return (std::int64_t)M + 3LL * Q + 2LL * C + words + n;
+}
+
+// Block-wide helpers (blockDim.x == kBlk).
+__device__ inline int block_scan_excl(int v, int *sh, int &total) {
+    // sh: kBlk + 1 ints
+    const int tid = threadIdx.x;
+    sh[tid] = v;
+    __syncthreads();
+    for (int off = 1; off < kBlk; off <<= 1) {
+        const int add = tid >= off ? sh[tid - off] : 0;
+        __syncthreads();
+        sh[tid] += add;
+        __syncthreads();
+    }
+    total = sh[kBlk - 1];
+    const int ex = sh[tid] - v;
+    __syncthreads();
+    return ex;
+}
+
+// Ordered compaction of the marked bits in words [w0, w1] into out[0..); the
+// words are cleared.  Returns the number of set bits (entries past `cap` are
+// counted, not written).
+__device__ int bitmap_compact(unsigned *bm, int w0, int w1, int *out, int cap, int *sh) {
+    constexpr int kW = 8; // words loaded together
+    const int tid = threadIdx.x;
+    const int nw = w1 >= w0 ? w1 - w0 + 1 : 0;
+    const int per = (nw + kBlk - 1) / kBlk;
+    const int a = w0 + tid * per, b = min(w0 + nw, a + per);
+    int cnt = 0;
+    for (int w0b = a; w0b < b; w0b += kW) {
+        unsigned x[kW];
+#pragma unroll
+        for (int u = 0; u < kW; ++u)
+            x[u] = w0b + u < b ? bm[w0b + u] : 0u;
+#pragma unroll
+        for (int u = 0; u < kW; ++u)
+            cnt += __popc(x[u]);
+    }
+    int total;
+    int pos = block_scan_excl(cnt, sh, total);
+    for (int w0b = a; w0b < b; w0b += kW) {
+        unsigned x[kW];
+#pragma unroll
+        for (int u = 0; u < kW; ++u)
+            x[u] = w0b + u < b ? bm[w0b + u] : 0u;
+#pragma unroll
+        for (int u = 0; u < kW; ++u) {
+            unsigned bits = x[u];
+            while (bits) {
+                const int bit = __ffs(bits) - 1;
+                bits &= bits - 1;
+                if (pos < cap)
+                    out[pos] = (w0b + u) * 32 + bit;
+                ++pos;
+            }
+            if (w0b + u < b && x[u])
+                bm[w0b + u] = 0u;
+        }
+    }
+    __syncthreads();
+    return total;
+}
+
+// Marks column x; the caller keeps the word range [lo, hi] it touched and
+// publishes it once with publish_range.
+__device__ inline void mark(unsigned *bm, int x, int &lo, int &hi) {
+    atomicOr(bm + (x >> 5), 1u << (x & 31));
+    lo = min(lo, x >> 5);
+    hi = max(hi, x >> 5);
+}
+__device__ inline void publish_range(int lo, int hi, int *wlo, int *whi) {
+    if (hi >= 0) {
+        atomicMin(wlo, lo);
+        atomicMax(whi, hi);
+    }
+}
+
+// Block sum in a fixed tree order (every thread gets the same bits).
+__device__ inline double block_sum_fixed(double v, double *sh) {
+    const int tid = threadIdx.x;
+    sh[tid] = v;
+    __syncthreads();
+    for (int off = kBlk / 2; off > 0; off >>= 1) {
+        if (tid < off)
+            sh[tid] += sh[tid + off];
+        __syncthreads();
+    }
+    const double r = sh[0];
+    __syncthreads();
+    return r;
+}
+
+constexpr int kBatch = 16; // loads issued together in the sequential loops
+constexpr int kRowB = 4;   // rows (or entries) per lane loaded together
+
+// hypot_ref(...hypot_ref(hypot_ref(0, x0), x1)..., x_{len-1}): the reference's
+// norm2 chain, with the loads issued in batches ahead of the chain.
+template <typename T> __device__ T hypot_chain(const T *x, int len) {
+    T acc = 0;
+    for (int r0 = 0; r0 < len; r0 += kBatch) {
+        T b[kBatch];
+#pragma unroll
+        for (int u = 0; u < kBatch; ++u)
+            b[u] = x[min(r0 + u, len - 1)];
+#pragma unroll
+        for (int u = 0; u < kBatch; ++u)
+            if (r0 + u < len)
+                acc = hypot_ref(acc, b[u]);
+    }
+    return acc;
+}
+
+// First position p in sorted a[0..m) with a[p] >= x.
+__device__ inline int lower_bound(const int *a, int m, int x) {
+    int lo = 0, hi = m;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (a[mid] < x)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlk)
+spai_block_kernel(int n, const int *__restrict__ aptr, const int *__restrict__ acol,
+                  const T *__restrict__ aval, const T *__restrict__ dscale,
+                  const int *__restrict__ bptr, const int *__restrict__ bcol, T eps,
+                  long long alpha, int beta, const int *__restrict__ worklist, int nwork,
+                  int *__restrict__ cursor, BlockCaps caps, T *__restrict__ dwork,
+                  int *__restrict__ iwork, std::int64_t blk_d, std::int64_t blk_i,
+                  SpaiColumnOut out, int *__restrict__ out_j, T *__restrict__ out_c,
+                  std::int64_t out_stride) {
+    __shared__ int sh[kBlk + 1];
+    __shared__ int s_slot, s_pk, s_flag, s_wlo, s_whi, s_fail;
+    __shared__ T s_red[kBlk];
+    __shared__ int s_redi[kBlk];
+    __shared__ double s_dred[kBlk];
+    __shared__ T s_scal[4];
+    constexpr bool kExact = sizeof(T) == sizeof(double);
+    const int tid = threadIdx.x, wave = tid / kWave, lane = tid % kWave;
+    const int M = caps.mcap, Q = caps.qcap, C = caps.ccap;
+    T *D = dwork + (std::int64_t)blockIdx.x * blk_d;
+    int *Iw = iwork + (std::int64_t)blockIdx.x * blk_i;
+    T *R = D, *G = D + (std::int64_t)Q * M, *rhs = G + (std::int64_t)Q * M, *v = rhs + M,
+      *res = v + M, *sc = res + M, *coef = sc + C, *dot = coef + Q;
+    int *I = Iw, *J = I + M, *Jt = J + Q, *cand = Jt + Q, *clampf = cand + C, *sel = clampf + C;
+    unsigned *bm = reinterpret_cast<unsigned *>(sel + Q);
+    int *pos = reinterpret_cast<int *>(bm + caps.words); // all -1 between columns
+    int mapped = 0; // entries of I currently recorded in pos
+
+    for (;;) {
+        if (tid == 0)
+            s_slot = atomicAdd(cursor, 1);
+        __syncthreads();
+        const int slot = s_slot;
+        __syncthreads();
+        if (slot >= nwork)
+            break;
+        const int k = worklist[slot];
+        int status = kSpaiNotAttempted;
+        int fits = 0, growths = 0, clamps = 0, q = 1, qdone = 0;
+        double last_res = __builtin_nan(""), last_piv = __builtin_nan(""), qr_flops = 0;
+        if (tid == 0)
+            J[0] = k;
+        int *jout = out_j + (std::int64_t)slot * out_stride;
+        T *cout_ = out_c + (std::int64_t)slot * out_stride;
+        __syncthreads();
+        for (;;) {
+            // ---- shadow I = union of cols(A row j), j in J (bitmap, ordered)
+            if (tid == 0) {
+                s_wlo = INT_MAX;
+                s_whi = -1;
+            }
+            __syncthreads();
+            {
+                int lo = INT_MAX, hi = -1;
+                for (int jj = wave; jj < q; jj += kBlk / kWave) {
+                    const int j = J[jj];
+                    for (int e = aptr[j] + lane; e < aptr[j + 1]; e += kWave)
+                        mark(bm, acol[e], lo, hi);
+                }
+                publish_range(lo, hi, &s_wlo, &s_whi);
+            }
+            __syncthreads();
+            for (int r = tid; r < mapped; r += kBlk) // the previous fit's shadow
+                pos[I[r]] = -1;
+            __syncthreads();
+            mapped = 0;
+            const int m = bitmap_compact(bm, s_wlo, s_whi, I, M, sh);
+            if (m > M) {
+                status = kSpaiOverflow;
+                break;
+            }
+            if (m < q) {
+                status = kSpaiRankDeficient;
+                break;
+            }
+            for (int r = tid; r < m; r += kBlk)
+                pos[I[r]] = r;
+            mapped = m;
+            __syncthreads();
+            // ---- gather R(i, c) = a_{J[c], I[i]} d_{J[c]}, rhs = e_k(I)
+            if (tid == 0)
+                s_pk = -1;
+            for (int c = wave; c < q; c += kBlk / kWave)
+                for (int r = lane; r < m; r += kWave) {
+                    R[(std::int64_t)c * M + r] = T(0);
+                    G[(std::int64_t)c * M + r] = T(0);
+                }
+            __syncthreads();
+            for (int r = tid; r < m; r += kBlk) {
+                rhs[r] = I[r] == k ? T(1) : T(0);
+                if (I[r] == k)
+                    s_pk = r;
+            }
+            for (int c = wave; c < q; c += kBlk / kWave) {
+                const int j = J[c];
+                const T dj = dscale[j];
+                for (int e = aptr[j] + lane; e < aptr[j + 1]; e += kWave) {
+                    const std::int64_t at = (std::int64_t)c * M + pos[acol[e]];
+                    const T x = aval[e] * dj;
+                    R[at] = x;
+                    G[at] = x;
+                }
+            }
+            __syncthreads();
+            const int pk = s_pk;
+            if (pk < 0) {
+                status = kSpaiNumericalFailure;
+                break;
+            }
+            // ---- Householder least squares (src/spai.cpp:30-98)
+            T rmax = 0;
+            for (int r = tid; r < m; r += kBlk) {
+                T sum = 0;
+                for (int c0 = 0; c0 < q; c0 += kBatch) {
+                    T xb[kBatch];
+#pragma unroll
+                    for (int u = 0; u < kBatch; ++u)
+                        xb[u] = R[(std::int64_t)min(c0 + u, q - 1) * M + r];
+#pragma unroll
+                    for (int u = 0; u < kBatch; ++u)
+                        if (c0 + u < q)
+                            sum += fabs(xb[u]);
+                }
+                rmax = sum > rmax ? sum : rmax;
+            }
+            s_red[tid] = rmax;
+            __syncthreads();
+            if (tid == 0) {
+                T mx = 0;
+                for (int t = 0; t < kBlk; ++t)
+                    mx = s_red[t] > mx ? s_red[t] : mx;
+                s_scal[3] = mx;
+            }
+            __syncthreads();
+            const T inf_norm = s_scal[3];
+            T min_pivot = T(INFINITY);
+            int fit_status = kSpaiSuccess;
+            for (int j = 0; j < q; ++j) {
+                const int len = m - j;
+                T *colj = R + (std::int64_t)j * M + j;
+                T length;
+                if (kExact) { // the reference's hypot chain (norm2)
+                    for (int r = tid; r < len; r += kBlk)
+                        v[r] = colj[r];
+                    __syncthreads();
+                    if (tid == 0)
+                        s_scal[0] = hypot_chain(v, len);
+                    __syncthreads();
+                    length = s_scal[0];
+                } else { // FP32: sum of squares in FP64, fixed-order tree
+                    double acc = 0;
+                    for (int r = tid; r < len; r += kBlk) {
+                        const T x = colj[r];
+                        v[r] = x;
+                        acc += (double)x * (double)x;
+                    }
+                    length = (T)sqrt(block_sum_fixed(acc, s_dred));
+                }
+                if (!isfinite(length)) {
+                    fit_status = kSpaiNumericalFailure;
+                    break;
+                }
+                if (length == 0) {
+                    fit_status = kSpaiRankDeficient;
+                    break;
+                }
+                const T diagonal = -copysign(length, v[0]);
+                __syncthreads(); // v[0] read before it is rescaled
+                for (int r = tid; r < len; r += kBlk) {
+                    T x = v[r] / length;
+                    if (r == 0)
+                        x += copysign(T(1), x);
+                    v[r] = x;
+                }
+                __syncthreads();
+                T vnorm;
+                if (kExact) {
+                    if (tid == 0)
+                        s_scal[1] = hypot_chain(v, len);
+                    __syncthreads();
+                    vnorm = s_scal[1];
+                } else {
+                    double acc = 0;
+                    for (int r = tid; r < len; r += kBlk)
+                        acc += (double)v[r] * (double)v[r];
+                    vnorm = (T)sqrt(block_sum_fixed(acc, s_dred));
+                }
+                for (int r = tid; r < len; r += kBlk)
+                    v[r] = v[r] / vnorm;
+                __syncthreads();
+                // slots j+1..q (q = right side)
+                if (kExact) {
+                    // one thread per slot, rows in order; loads in batches
+                    for (int c = j + 1 + tid; c <= q; c += kBlk) {
+                        const T *x = (c < q ? R + (std::int64_t)c * M : rhs) + j;
+                        T d = 0;
+                        for (int r0 = 0; r0 < len; r0 += kBatch) {
+                            T vb[kBatch], xb[kBatch];
+#pragma unroll
+                            for (int u = 0; u < kBatch; ++u) {
+                                const int r = min(r0 + u, len - 1);
+                                vb[u] = v[r];
+                                xb[u] = x[r];
+                            }
+#pragma unroll
+                            for (int u = 0; u < kBatch; ++u)
+                                if (r0 + u < len)
+                                    d += vb[u] * xb[u];
+                        }
+                        dot[c] = d;
+                    }
+                } else {
+                    // FP32: one wave per slot, lanes over rows (loads batched),
+                    // FP64 partials combined by a fixed butterfly
+                    for (int c = j + 1 + wave; c <= q; c += kBlk / kWave) {
+                        const T *x = (c < q ? R + (std::int64_t)c * M : rhs) + j;
+                        double d = 0;
+                        for (int r0 = lane; r0 < len; r0 += kWave * kRowB) {
+                            T vb[kRowB], xb[kRowB];
+#pragma unroll
+                            for (int u = 0; u < kRowB; ++u) {
+                                const int r = min(r0 + u * kWave, len - 1);
+                                vb[u] = v[r];
+                                xb[u] = x[r];
+                            }
+#pragma unroll
+                            for (int u = 0; u < kRowB; ++u)
+                                if (r0 + u * kWave < len)
+                                    d += (double)vb[u] * (double)xb[u];
+                        }
+                        d = wave_sum(d);
+                        if (lane == 0)
+                            dot[c] = (T)d;
+                    }
+                }
+                if (tid == 0)
+                    s_flag = 0;
+                __syncthreads();
+                bool bad = false;
+                for (int c = j + 1 + wave; c <= q; c += kBlk / kWave) {
+                    T *x = (c < q ? R + (std::int64_t)c * M : rhs) + j;
+                    const T dc = dot[c];
+                    for (int r0 = lane; r0 < len; r0 += kWave * kRowB) {
+                        T vb[kRowB], xb[kRowB];
+#pragma unroll
+                        for (int u = 0; u < kRowB; ++u) {
+                            const int r = min(r0 + u * kWave, len - 1);
+                            vb[u] = v[r];
+                            xb[u] = x[r];
+                        }
+#pragma unroll
+                        for (int u = 0; u < kRowB; ++u)
+                            if (r0 + u * kWave < len) {
+                                const T y = xb[u] - T(2) * vb[u] * dc;
+                                x[r0 + u * kWave] = y;
+                                bad = bad || !isfinite(y);
+                            }
+                    }
+                }
+                if (bad)
+                    s_flag = 1;
+                __syncthreads();
+                if (s_flag) {
+                    fit_status = kSpaiNumericalFailure;
+                    break;
+                }
+                if (tid == 0)
+                    colj[0] = diagonal;
+                const T ad = fabs(diagonal);
+                min_pivot = ad < min_pivot ? ad : min_pivot;
+                __syncthreads();
+            }
+            if (fit_status == kSpaiSuccess) {
+                if (tid == 0) {
+                    int st = kSpaiSuccess;
+                    for (int j = q - 1; j >= 0; --j) {
+                        T value = rhs[j];
+                        for (int c0 = j + 1; c0 < q; c0 += kBatch) {
+                            T rb[kBatch], cb[kBatch];
+#pragma unroll
+                            for (int u = 0; u < kBatch; ++u) {
+                                const int c = min(c0 + u, q - 1);
+                                rb[u] = R[(std::int64_t)c * M + j];
+                                cb[u] = coef[c];
+                            }
+#pragma unroll
+                            for (int u = 0; u < kBatch; ++u)
+                                if (c0 + u < q)
+                                    value -= rb[u] * cb[u];
+                        }
+                        const T dg = R[(std::int64_t)j * M + j];
+                        if (!isfinite(dg)) {
+                            st = kSpaiNumericalFailure;
+                            break;
+                        }
+                        if (dg == 0) {
+                            st = kSpaiRankDeficient;
+                            break;
+                        }
+                        const T sj = value / dg;
+                        if (!isfinite(sj)) {
+                            st = kSpaiNumericalFailure;
+                            break;
+                        }
+                        coef[j] = sj;
+                    }
+                    s_fail = st;
+                }
+                __syncthreads();
+                fit_status = s_fail;
+            }
+            if (fit_status != kSpaiSuccess) {
+                status = fit_status;
+                break;
+            }
+            const T pivot_ratio = inf_norm == 0 ? T(0) : min_pivot / inf_norm;
+            // ---- fitted residual s = B(I,J) m - e_k(I): per row, the stored
+            // entries in pattern order (as the lane kernel's scatter)
+            // (stored entries of B are nonzero: the scaling step rejects zero
+            // scaled entries, so G == 0 marks exactly the absent ones)
+            for (int r = tid; r < m; r += kBlk) {
+                T acc = 0;
+                for (int c0 = 0; c0 < q; c0 += kBatch) {
+                    T gb[kBatch];
+#pragma unroll
+                    for (int u = 0; u < kBatch; ++u)
+                        gb[u] = G[(std::int64_t)min(c0 + u, q - 1) * M + r];
+#pragma unroll
+                    for (int u = 0; u < kBatch; ++u)
+                        if (c0 + u < q && gb[u] != T(0))
+                            acc += gb[u] * coef[c0 + u];
+                }
+                if (r == pk)
+                    acc -= T(1);
+                res[r] = acc;
+            }
+            __syncthreads();
+            T rnorm;
+            if (kExact) {
+                if (tid == 0) {
+                    bool fin = true;
+                    for (int r = 0; r < m; ++r)
+                        fin = fin && isfinite(res[r]);
+                    const T rn = hypot_chain(res, m);
+                    s_scal[2] = rn;
+                    s_fail = fin && isfinite(rn) ? 0 : 1;
+                }
+                __syncthreads();
+                rnorm = s_scal[2];
+            } else {
+                double acc = 0;
+                for (int r = tid; r < m; r += kBlk)
+                    acc += (double)res[r] * (double)res[r];
+                const double ss = block_sum_fixed(acc, s_dred);
+                rnorm = (T)sqrt(ss);
+                if (tid == 0)
+                    s_fail = isfinite(ss) && isfinite(rnorm) ? 0 : 1;
+                __syncthreads();
+            }
+            if (s_fail) {
+                status = kSpaiNumericalFailure;
+                break;
+            }
+            if (fits != 0)
+                ++growths;
+            ++fits;
+            {
+                const double fm = m, fq = q;
+                qr_flops += 2 * fm * fq * fq - 2 * fq * fq * fq / 3 + 4 * fm * fq + fq * fq;
+            }
+            for (int c = tid; c < q; c += kBlk) {
+                jout[c] = J[c];
+                cout_[c] = coef[c];
+            }
+            qdone = q;
+            last_res = (double)rnorm;
+            last_piv = (double)pivot_ratio;
+            if (rnorm <= eps) {
+                status = kSpaiSuccess;
+                break;
+            }
+            if ((long long)growths == alpha) {
+                status = kSpaiIterationLimit;
+                break;
+            }
+            // ---- candidates: union of B rows i in I (and k), minus J
+            if (tid == 0) {
+                s_wlo = INT_MAX;
+                s_whi = -1;
+            }
+            __syncthreads();
+            {
+                int lo = INT_MAX, hi = -1;
+                for (int r = tid; r < m; r += kBlk) {
+                    const int i = I[r];
+                    const int e1 = bptr[i + 1];
+                    for (int e0 = bptr[i]; e0 < e1; e0 += kBatch) {
+                        int cb[kBatch];
+#pragma unroll
+                        for (int u = 0; u < kBatch; ++u)
+                            cb[u] = bcol[min(e0 + u, e1 - 1)];
+#pragma unroll
+                        for (int u = 0; u < kBatch; ++u)
+                            if (e0 + u < e1)
+                                mark(bm, cb[u], lo, hi);
+                    }
+                }
+                for (int e = bptr[k] + tid; e < bptr[k + 1]; e += kBlk)
+                    mark(bm, bcol[e], lo, hi);
+                publish_range(lo, hi, &s_wlo, &s_whi);
+            }
+            __syncthreads();
+            for (int jj = tid; jj < q; jj += kBlk)
+                atomicAnd(bm + (J[jj] >> 5), ~(1u << (J[jj] & 31)));
+            __syncthreads();
+            const int c = bitmap_compact(bm, s_wlo, s_whi, cand, C, sh);
+            if (c > C) {
+                status = kSpaiOverflow;
+                break;
+            }
+            if (c == 0) {
+                status = kSpaiEmptyCandidates;
+                break;
+            }
+            // ---- restricted scores (src/spai.cpp:198-215), one thread per
+            // candidate; the first failing candidate decides the status
+            const T rr = rnorm * rnorm;
+            if (tid == 0)
+                s_fail = INT_MAX;
+            __syncthreads();
+            for (int t = tid; t < c; t += kBlk) {
+                const int j = cand[t];
+                const T dj = dscale[j];
+                T dt = 0, den = 0;
+                const int e1 = aptr[j + 1];
+                for (int e0 = aptr[j]; e0 < e1; e0 += kRowB) {
+                    int pb[kRowB];
+                    T ab[kRowB], rb[kRowB];
+#pragma unroll
+                    for (int u = 0; u < kRowB; ++u)
+                        pb[u] = pos[acol[min(e0 + u, e1 - 1)]];
+#pragma unroll
+                    for (int u = 0; u < kRowB; ++u) {
+                        ab[u] = aval[min(e0 + u, e1 - 1)];
+                        rb[u] = res[max(pb[u], 0)];
+                    }
+#pragma unroll
+                    for (int u = 0; u < kRowB; ++u)
+                        if (e0 + u < e1 && pb[u] >= 0) {
+                            const T value = ab[u] * dj;
+                            dt += rb[u] * value;
+                            den += value * value;
+                        }
+                }
+                T sv = 0;
+                bool cl = false;
+                const int st = score_of(rr, dt, den, sv, cl);
+                sc[t] = sv;
+                clampf[t] = st == kSpaiSuccess ? (cl ? 1 : 0) : -st;
+                if (st != kSpaiSuccess)
+                    atomicMin(&s_fail, t);
+            }
+            __syncthreads();
+            const int first_bad = s_fail;
+            {
+                // clamps are counted up to the first failing candidate, as the
+                // reference's sequential loop does
+                const int lim = first_bad != INT_MAX ? first_bad : c;
+                int cl = 0;
+                for (int t = tid; t < lim; t += kBlk)
+                    cl += clampf[t];
+                sh[tid] = cl;
+                __syncthreads();
+                int total = 0;
+                for (int t = 0; t < kBlk; ++t)
+                    total += sh[t];
+                clamps += total;
+                __syncthreads();
+            }
+            if (first_bad != INT_MAX) {
+                status = -clampf[first_bad];
+                break;
+            }
+            T mean;
+            if (kExact) { // std::accumulate order (src/spai.cpp:216-217)
+                if (tid == 0) {
+                    T acc = 0;
+                    for (int t0 = 0; t0 < c; t0 += kBatch) {
+                        T xb[kBatch];
+#pragma unroll
+                        for (int u = 0; u < kBatch; ++u)
+                            xb[u] = sc[min(t0 + u, c - 1)];
+#pragma unroll
+                        for (int u = 0; u < kBatch; ++u)
+                            if (t0 + u < c)
+                                acc += xb[u];
+                    }
+                    s_scal[0] = acc / T(c);
+                }
+                __syncthreads();
+                mean = s_scal[0];
+            } else {
+                double acc = 0;
+                for (int t = tid; t < c; t += kBlk)
+                    acc += (double)sc[t];
+                mean = (T)(block_sum_fixed(acc, s_dred) / (double)c);
+            }
+            if (!isfinite(mean)) {
+                status = kSpaiNumericalFailure;
+                break;
+            }
+            // ---- select <= beta smallest (score, index) with score <= mean
+            int nsel = 0;
+            T prev_s = T(-INFINITY);
+            int prev_j = -1;
+            for (; nsel < beta; ++nsel) {
+                T best_s = T(INFINITY);
+                int best_j = -1;
+                for (int t0 = tid; t0 < c; t0 += kBlk * kRowB) {
+                    T sb[kRowB];
+                    int jb[kRowB];
+#pragma unroll
+                    for (int u = 0; u < kRowB; ++u) {
+                        const int t = min(t0 + u * kBlk, c - 1);
+                        sb[u] = sc[t];
+                        jb[u] = cand[t];
+                    }
+#pragma unroll
+                    for (int u = 0; u < kRowB; ++u) {
+                        const T sv = sb[u];
+                        const int j = jb[u];
+                        if (t0 + u * kBlk >= c || sv > mean)
+                            continue;
+                        const bool after = sv > prev_s || (sv == prev_s && j > prev_j);
+                        const bool before =
+                            best_j < 0 || sv < best_s || (sv == best_s && j < best_j);
+                        if (after && before) {
+                            best_s = sv;
+                            best_j = j;
+                        }
+                    }
+                }
+                s_red[tid] = best_s;
+                s_redi[tid] = best_j;
+                __syncthreads();
+                for (int off = kBlk / 2; off > 0; off >>= 1) {
+                    if (tid < off) {
+                        const T os = s_red[tid + off];
+                        const int oj = s_redi[tid + off];
+                        const T ms = s_red[tid];
+                        const int mj = s_redi[tid];
+                        if (oj >= 0 && (mj < 0 || os < ms || (os == ms && oj < mj))) {
+                            s_red[tid] = os;
+                            s_redi[tid] = oj;
+                        }
+                    }
+                    __syncthreads();
+                }
+                best_s = s_red[0];
+                best_j = s_redi[0];
+                __syncthreads();
+                if (best_j < 0)
+                    break;
+                if (q + nsel + 1 > Q) { // the grown pattern would not fit
+                    nsel = -1;
+                    break;
+                }
+                if (tid == 0)
+                    sel[nsel] = best_j;
+                prev_s = best_s;
+                prev_j = best_j;
+            }
+            if (nsel < 0) {
+                status = kSpaiOverflow;
+                break;
+            }
+            if (nsel == 0) {
+                status = kSpaiNoGrowth;
+                break;
+            }
+            // sort the selection by index, then merge into J
+            if (tid == 0) {
+                for (int a = 1; a < nsel; ++a) {
+                    const int x = sel[a];
+                    int b = a - 1;
+                    while (b >= 0 && sel[b] > x) {
+                        sel[b + 1] = sel[b];
+                        --b;
+                    }
+                    sel[b + 1] = x;
+                }
+                int a = 0, b = 0, t = 0;
+                while (a < q || b < nsel) {
+                    if (b >= nsel || (a < q && J[a] < sel[b]))
+                        Jt[t++] = J[a++];
+                    else
+                        Jt[t++] = sel[b++];
+                }
+                for (int r = 0; r < t; ++r)
+                    J[r] = Jt[r];
+            }
+            q += nsel;
+            __syncthreads();
+        }
+        __syncthreads();
+        for (int r = tid; r < mapped; r += kBlk)
+            pos[I[r]] = -1;
+        mapped = 0;
+        __syncthreads();
+        if (tid == 0) {
+            out.status[k] = status;
+            out.fits[k] = fits;
+            out.growths[k] = growths;
+            out.clamps[k] = clamps;
+            out.q[k] = status == kSpaiOverflow ? 0 : qdone;
+            out.residual[k] = last_res;
+            out.pivot[k] = last_piv;
+            out.flops[k] = qr_flops;
+            out.off[k] = (std::int64_t)slot * out_stride;
+            out.pass[k] = out.pass_id;
+        }
+        __syncthreads();
+    }
+}
+
+template <typename T>
+void launch_spai_block(hipStream_t s, int blocks, int n, const int *aptr, const int *acol,
+                       const T *aval, const T *dscale, const int *bptr, const int *bcol, T eps,
+                       long long alpha, int beta, const int *worklist, int nwork, int *cursor,
+                       int mcap, int qcap, int ccap, T *dwork, int *iwork, SpaiColumnOut out,
+                       int *out_j, T *out_c, std::int64_t out_stride) {
+    const int words = (n + 31) / 32;
+    BlockCaps caps{mcap, qcap, ccap, words};
+    hipLaunchKernelGGL(spai_block_kernel<T>, dim3(blocks), dim3(kBlk), 0, s, n, aptr, acol, aval,
+                       dscale, bptr, bcol, eps, alpha, beta, worklist, nwork, cursor, caps, dwork,
+                       iwork, spai_block_doubles(mcap, qcap, ccap),
+                       spai_block_ints(mcap, qcap, ccap, words, n), out, out_j, out_c,
+                       out_stride);
+    HIP_LAUNCH_CHECK();
+}
+
+__global__ void fill_int_kernel(std::int64_t stride, std::int64_t offset, int count, int *p,
+                                int value) {
+    int *q = p + (std::int64_t)blockIdx.y * stride + offset;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x)
+        q[i] = value;
+}
+void launch_fill_int(hipStream_t s, std::int64_t blocks, std::int64_t stride, std::int64_t offset,
+                     int count, int *p, int value) {
+    if (blocks == 0 || count == 0)
+        return;
+    hipLaunchKernelGGL(fill_int_kernel, dim3(std::min<unsigned>(ceil_div(count, 256), 1024u),
+                                             (unsigned)blocks),
+                       dim3(256), 0, s, stride, offset, count, p, value);
+    HIP_LAUNCH_CHECK();
+}
+
+#define BSPAI_BLOCK_INST(T)                                                                       \
+    template void launch_spai_block<T>(hipStream_t, int, int, const int *, const int *,          \
+                                       const T *, const T *, const int *, const int *, T,        \
+                                       long long, int, const int *, int, int *, int, int, int,    \
+                                       T *, int *, SpaiColumnOut, int *, T *, std::int64_t);
+BSPAI_BLOCK_INST(double)
+BSPAI_BLOCK_INST(float)
+#undef BSPAI_BLOCK_INST
 
 // Single score on the device (unit-test hook for the restricted score).
 __global__ void score_kernel(double rr, double dot, double den, int *out_i, double *out_d) {

@@ -79,25 +79,25 @@ spmv_stream_kernel(Tiles tiles, const int *__restrict__ flag, int flag_stride,
     constexpr int per = kStreamChunk / kSpmvRows;
     for (int c0 = e0; c0 < e1; c0 += kStreamChunk) {
         const int c1 = min(e1, c0 + kStreamChunk);
-        // All column loads first, then the value loads and x gathers: every
-        // thread keeps `per` independent loads of each kind in flight.
+        // Unconditional loads (indices clamped into the chunk) so that all of
+        // them are in flight at once: columns and values, then the gathers.
         int cj[per];
+        TV vv[per];
 #pragma unroll
         for (int u = 0; u < per; ++u) {
-            const int e = c0 + tid + u * kSpmvRows;
-            cj[u] = e < c1 ? col[e] : 0;
+            const int e = min(c0 + tid + u * kSpmvRows, c1 - 1);
+            cj[u] = col[e];
+            vv[u] = val[e];
         }
-        TA pr[per];
+        TX xv[per];
 #pragma unroll
-        for (int u = 0; u < per; ++u) {
-            const int e = c0 + tid + u * kSpmvRows;
-            pr[u] = e < c1 ? (TA)val[e] * (TA)x[cj[u]] : TA(0);
-        }
+        for (int u = 0; u < per; ++u)
+            xv[u] = x[cj[u]];
 #pragma unroll
         for (int u = 0; u < per; ++u) {
             const int e = c0 + tid + u * kSpmvRows;
             if (e < c1)
-                prod[e - c0] = pr[u];
+                prod[e - c0] = (TA)vv[u] * (TA)xv[u];
         }
         __syncthreads();
         const int lo = max(mb, c0), hi = min(me, c1);
