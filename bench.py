@@ -8,7 +8,7 @@ stop (outercap 20, innercap 200).  `value` is the mixed FP32/FP64 path's
 solves/s; the plain FP64 path is timed in the same run and reported beside it.
 Inputs (matrix, right side) are resident in HBM before the timed region.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--extra many,powerlaw]
 """
 from __future__ import annotations
 
@@ -35,6 +35,11 @@ def parse():
                     help="SPAI columns in the CPU-baseline sample")
     ap.add_argument("--cpu-iters", type=int, default=32,
                     help="GMRES iterations in the CPU-baseline sample")
+    ap.add_argument("--extra", default="",
+                    help="comma list of extra workloads to time beside the headline: "
+                         "'many' (256 systems batched vs looped), 'powerlaw' (uneven columns)")
+    ap.add_argument("--powerlaw-n", type=int, default=50000,
+                    help="rows of the power-law matrix timed by --extra powerlaw")
     return ap.parse_args()
 
 
@@ -140,7 +145,72 @@ def main():
                            for r in rows[:8]] for name, rows in roof.items()},
         "cpu_baseline": cpu,
     }
+    extras = [e for e in args.extra.split(",") if e]
+    if extras:
+        line["extra"] = {}
+    if "many" in extras:
+        line["extra"]["many_systems"] = bench_many(ctx)
+    if "powerlaw" in extras:
+        line["extra"]["power_law"] = bench_power_law(ctx, args.powerlaw_n)
     print(json.dumps(line))
+
+
+def bench_many(ctx):
+    """256 independent variable-coefficient systems (L in 32..256): one batched
+    SPAI + IR solve of all of them against the single-system plain path looped
+    over the systems; aggregate solves/s."""
+    import numpy as np
+
+    import bspai_hip
+    from bspai_hip import problems as P
+
+    sb = P.many_systems()
+    ns = len(sb.offsets) - 1
+    mb = bspai_hip.Matrix(sb.matrix, ctx, systems=sb.offsets)
+    blocks = [bspai_hip.Matrix(sb.block(s), ctx) for s in range(ns)]
+    out = {"systems": ns, "rows": int(sb.matrix.n), "nnz": int(sb.matrix.nnz)}
+    for prec, tol in (("fp64", 1e-8), ("mixed", 1e-4)):
+        kw = dict(precision=prec, tolerance=tol, max_iterations=400)
+        bspai_hip.spai_gmres_ir(mb, sb.rhs, **kw)
+        t0 = time.perf_counter()
+        res = bspai_hip.spai_gmres_ir(mb, sb.rhs, **kw)
+        el = time.perf_counter() - t0
+        out[prec] = {"batched_solves_per_s": round(ns / el, 2),
+                     "succeeded": sum(r.status == "success" for r in res),
+                     "spai_ms": round(res[0].spai_ms, 1), "solve_ms": round(res[0].solve_ms, 1)}
+    kw = dict(precision="fp64", tolerance=1e-8, max_iterations=400)
+    t0 = time.perf_counter()
+    ok = 0
+    for s in range(ns):
+        r = bspai_hip.spai_gmres_ir(blocks[s], sb.rhs[sb.offsets[s]:sb.offsets[s + 1]], **kw)
+        ok += r.status == "success"
+    el = time.perf_counter() - t0
+    out["fp64"]["looped_solves_per_s"] = round(ns / el, 2)
+    out["fp64"]["batched_over_looped"] = round(out["fp64"]["batched_solves_per_s"] * el / ns, 2)
+    return out
+
+
+def bench_power_law(ctx, n):
+    """Uneven SPAI columns: power-law matrix (exponent 2.2, 4..200 off-diagonals
+    per row), eps 0.2, mixed IR to rel-res 1e-10; build and solve times."""
+    import numpy as np
+
+    import bspai_hip
+    from bspai_hip import problems as P
+
+    a = P.power_law(n=n)
+    b = P.equal_rhs(n)
+    m = bspai_hip.Matrix(a, ctx)
+    t0 = time.perf_counter()
+    s = bspai_hip.construct_spai(m, 0.2, -(-n // 8), 8, precision="mixed")
+    r = bspai_hip.refine_with_preconditioner(m, s, b, precision="mixed", tolerance=1e-4,
+                                             relres_target=1e-10)
+    el = time.perf_counter() - t0
+    x = np.asarray(r.solution)
+    relres = float(np.linalg.norm(b - a.to_scipy() @ x) / np.linalg.norm(b))
+    return {"n": n, "nnz": int(a.nnz), "status": r.status, "relres": relres,
+            "spai_ms": round(s.build_ms, 1), "passes": s.passes, "fits": s.fits,
+            "solve_ms": round(r.solve_ms, 1), "wall_s": round(el, 3)}
 
 
 PEAK_TFLOPS = {"double": 78.6, "float": 157.3}
