@@ -71,25 +71,33 @@ def main():
     common = dict(epsilon=0.3, alpha=0, beta=8, max_iterations=200, max_outer_steps=20,
                   relres_target=1e-12)
     # Inner tolerances: the FP32 correction GMRES stagnates between 1e-5 and
-    # 1e-6 on this system, so 1e-5 is its tightest reliable setting; FP64
-    # uses the reference's default 1e-8 (include/bspai/gmres.hpp:24).
-    paths = {"mixed": dict(precision="mixed", tolerance=1e-5),
+    # 1e-6 on this system; 3e-5 keeps a margin above that floor and still
+    # needs three refinement steps (1e-5: the same three steps, 5e-5: four).
+    # FP64 uses the reference's default 1e-8 (include/bspai/gmres.hpp:24).
+    paths = {"mixed": dict(precision="mixed", tolerance=3e-5),
              "fp64": dict(precision="fp64", tolerance=1e-8)}
-    results, kernels = {}, {}
+    results, kernels, live = {}, {}, {}
     for name, kw in paths.items():
         last = None
         for _ in range(args.warmup):
             last = bspai_hip.spai_gmres_ir(m, b, **common, **kw)
         torch.cuda.synchronize()
-        # Per-kernel HIP-event timing over the timed region (events on the
-        # library's stream, which is this stream).
-        ctx.set_timing(True)
+        # HIP-event timing of the SPAI construction kernels over the timed
+        # region (events on the library's stream, which is this stream; two
+        # events per launch cost nothing measurable).
+        ctx.set_timing("spai")
         t0 = time.perf_counter()
         runs = []
         for _ in range(args.steps):
             runs.append(bspai_hip.spai_gmres_ir(m, b, **common, **kw))
         torch.cuda.synchronize()
         el = time.perf_counter() - t0
+        live[name] = ctx.kernel_times()
+        # Every hot kernel timed, on one more solve after the timed region
+        # (bracketing each of the ~10 launches per Arnoldi step with events
+        # costs a few percent, which stays out of the headline).
+        ctx.set_timing(True)
+        bspai_hip.spai_gmres_ir(m, b, **common, **kw)
         kernels[name] = ctx.kernel_times()
         ctx.set_timing(False)
         last = runs[-1]
@@ -102,13 +110,23 @@ def main():
             outer_steps=last.outer_steps, inner_iterations=last.inner_iterations,
             spai_ms=last.spai_ms, solve_ms=last.solve_ms,
             gmres_iterations_per_s=sum(last.inner_iterations) / (el / args.steps))
-    # ---- rooflines from the live per-kernel timings of the headline (mixed) path
+    # ---- rooflines: the per-kernel table of the headline (mixed) path names
+    # the dominant kernel; when that is the SPAI construction, its figures
+    # are the ones timed live over the timed region.
     roof = {name: kernel_rooflines(kt) for name, kt in kernels.items()}
     dominant = max(roof["mixed"], key=lambda r: r["total_ms"])
+    live_rows = {r["kernel"]: r for r in kernel_rooflines(live["mixed"])}
+    timed_live = dominant["kernel"] in live_rows
+    if timed_live:
+        dominant = live_rows[dominant["kernel"]]
     roofline = {k: dominant[k] for k in ("kernel", "bound", "achieved", "peak", "unit", "frac")}
     roofline.update(traffic=None, per_launch=dominant["per_launch"], avg_launch_us=dominant["avg_us"],
-                    launches=dominant["launches"], share_of_timed_region=round(
-                        dominant["total_ms"] / (results["mixed"]["ms_per_solve"] * args.steps), 4))
+                    launches=dominant["launches"],
+                    measured="HIP events over the timed region" if timed_live else
+                    "HIP events, one extra solve after the timed region")
+    if timed_live:
+        roofline["share_of_timed_region"] = round(
+            dominant["total_ms"] / (results["mixed"]["ms_per_solve"] * args.steps), 4)
     # ---- CPU baseline: the oracle (scalar port) on a bounded sample
     cpu = None
     if not args.no_cpu:

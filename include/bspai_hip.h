@@ -126,7 +126,10 @@ int bspai_hip_synchronize(bspai_hip_ctx *ctx);
 
 /* Kernel timing telemetry.  While enabled, each launch of the hot kernels
  * (SPAI columns, the two SpMVs and the Gram-Schmidt passes of every Arnoldi
- * step) is bracketed by two HIP events on the context's stream.
+ * step) is bracketed by two HIP events on the context's stream; enable = 2
+ * times the SPAI construction kernels only (two events per capacity pass,
+ * negligible overhead), enable = 1 every hot kernel (a few percent of a
+ * GMRES step), 0 stops.
  * bspai_hip_ctx_kernel_times synchronizes and returns the totals per kernel
  * label since timing was last enabled: it writes at most `capacity` entries
  * and sets *count to the number of labels.  Enabling or disabling clears

@@ -79,9 +79,11 @@ class Context:
     def synchronize(self):
         check(_lib.load().bspai_hip_synchronize(self.h), self.h)
 
-    def set_timing(self, enable: bool = True):
-        """Start (clearing earlier records) or stop per-kernel HIP-event timing."""
-        check(_lib.load().bspai_hip_ctx_set_timing(self.h, int(bool(enable))), self.h)
+    def set_timing(self, enable=True):
+        """Start (clearing earlier records) or stop per-kernel HIP-event timing:
+        True / 1 every hot kernel, "spai" / 2 the SPAI construction only."""
+        mode = 2 if enable == "spai" else int(enable)
+        check(_lib.load().bspai_hip_ctx_set_timing(self.h, mode), self.h)
 
     def kernel_times(self):
         """Per-kernel totals since timing was enabled:

@@ -304,6 +304,7 @@ int bspai_hip_ctx_set_timing(bspai_hip_ctx *ctx, int enable) {
         HIP_CHECK(hipStreamSynchronize(ctx->c.stream));
         ctx->c.timer.clear();
         ctx->c.timer.on = enable != 0;
+        ctx->c.timer.spai_only = enable == 2;
     });
 }
 

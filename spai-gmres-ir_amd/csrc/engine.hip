@@ -387,7 +387,7 @@ void spai_build_typed(Context &c, const DevCsr &a, const T *aval, T *d, double e
                                      nwork, counter.data(), mcap, qcap, ccap, dwork.as<T>(),
                                      iwork.as<int>(), o, out.pool_j.back().data(),
                                      out.pool_c.back().as<T>(), qcap);
-            });
+            }, true);
             hipLaunchKernelGGL(collect_overflow_kernel, dim3(ceil_div(nwork, 256)), dim3(256), 0,
                                s, nwork, work.data(), out.c_status.data(), next.data(),
                                counter.data() + 1);
@@ -425,7 +425,7 @@ void spai_build_typed(Context &c, const DevCsr &a, const T *aval, T *d, double e
                                    nwork, counter.data(), mcap, qcap, ccap, dwork.as<T>(),
                                    iwork.as<int>(), o, out.pool_j.back().data(),
                                    out.pool_c.back().as<T>(), qcap);
-        });
+        }, true);
         hipLaunchKernelGGL(collect_overflow_kernel, dim3(ceil_div(nwork, 256)), dim3(256), 0, s,
                            nwork, work.data(), out.c_status.data(), next.data(),
                            counter.data() + 1);

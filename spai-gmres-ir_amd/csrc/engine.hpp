@@ -89,6 +89,7 @@ class KernelTimer {
     KernelTimer &operator=(const KernelTimer &) = delete;
     ~KernelTimer();
     bool on = false;
+    bool spai_only = false; // time only the SPAI construction kernels
     int begin(hipStream_t s);
     void end(hipStream_t s, int first, const char *name, double bytes, double flops);
     // Adds flops known only after the launch (device-side counts) to the
@@ -125,8 +126,9 @@ struct Context {
 // Runs `launch` (one kernel launch on c.stream), timed when the context's
 // timer is on.
 template <class F>
-inline void timed(Context &c, const char *name, double bytes, double flops, F &&launch) {
-    if (!c.timer.on) {
+inline void timed(Context &c, const char *name, double bytes, double flops, F &&launch,
+                  bool spai = false) {
+    if (!c.timer.on || (c.timer.spai_only && !spai)) {
         launch();
         return;
     }
