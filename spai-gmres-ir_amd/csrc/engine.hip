@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstddef>
 #include <cstdlib>
+#include <type_traits>
 
 namespace bspai_hip {
 
@@ -297,15 +298,26 @@ void spai_build_typed(Context &c, const DevCsr &a, const T *aval, T *d, double e
         return;
     }
     // ---- B structure = transpose of A's pattern (row i lists rows j with a_ji != 0)
+    // FP32 first bucket: one wavefront per column while the columns are too
+    // few to fill the device with one lane per column (about 160 per CU: the
+    // measured crossover lies between 40k and 110k columns on 256 CUs), the
+    // lane kernel above that.  BSPAI_SPAI_FP32_KERNEL=lane|wave forces one.
+    hipDeviceProp_t prop;
+    HIP_CHECK(hipGetDeviceProperties(&prop, c.device));
+    bool wave_fp32 = std::is_same<T, float>::value && n <= 160LL * prop.multiProcessorCount;
+    if (const char *env = std::getenv("BSPAI_SPAI_FP32_KERNEL"))
+        wave_fp32 = std::is_same<T, float>::value && std::string(env) == "wave";
     DeviceArray<int> bptr(n + 1), bcol(a.nnz > 0 ? a.nnz : 1), cursor(n + 1);
+    DeviceArray<T> bval(wave_fp32 && a.nnz > 0 ? a.nnz : 1);
     DeviceArray<int> scan_tmp(scan_tmp_ints(n + 1));
     HIP_CHECK(hipMemsetAsync(bptr.data(), 0, (n + 1) * sizeof(int), s));
     launch_count_cols(s, a.nnz, a.col.data(), bptr.data());
     scan_inclusive_int(s, bptr.data(), n + 1, scan_tmp.data());
     HIP_CHECK(hipMemcpyAsync(cursor.data(), bptr.data(), (n + 1) * sizeof(int),
                              hipMemcpyDeviceToDevice, s));
-    launch_transpose_fill<T>(s, n, a.ptr.data(), a.col.data(), (const T *)nullptr, bptr.data(),
-                             cursor.data(), bcol.data(), (T *)nullptr);
+    launch_transpose_fill<T>(s, n, a.ptr.data(), a.col.data(), wave_fp32 ? aval : nullptr,
+                             bptr.data(), cursor.data(), bcol.data(),
+                             wave_fp32 ? bval.data() : nullptr);
     // ---- columns, in capacity passes
     out.c_status.resize(n);
     out.c_fits.resize(n);
@@ -321,8 +333,6 @@ void spai_build_typed(Context &c, const DevCsr &a, const T *aval, T *d, double e
     hipLaunchKernelGGL(iota_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, s, n, work.data());
     int nwork = n;
     int mcap = 64, qcap = 16, ccap = 128;
-    hipDeviceProp_t prop;
-    HIP_CHECK(hipGetDeviceProperties(&prop, c.device));
     // Columns in flight: the column kernel is latency-bound (dependent loads
     // of the lane's work area), so occupancy pays.  Overridable for tuning.
     int waves_per_cu = 32;
@@ -387,6 +397,36 @@ void spai_build_typed(Context &c, const DevCsr &a, const T *aval, T *d, double e
                                      nwork, counter.data(), mcap, qcap, ccap, dwork.as<T>(),
                                      iwork.as<int>(), o, out.pool_j.back().data(),
                                      out.pool_c.back().as<T>(), qcap);
+            }, true);
+            hipLaunchKernelGGL(collect_overflow_kernel, dim3(ceil_div(nwork, 256)), dim3(256), 0,
+                               s, nwork, work.data(), out.c_status.data(), next.data(),
+                               counter.data() + 1);
+            int nover = 0;
+            download(c, &nover, counter.data() + 1, 1);
+            sync(c);
+            std::swap(work, next);
+            nwork = nover;
+            continue;
+        }
+        if (wave_fp32) {
+            // FP32: one wavefront per column (a one-wave workgroup).
+            const int wq = spai_wave_qcap();
+            const int blocks = (int)std::max<std::int64_t>(
+                1, std::min<std::int64_t>(nwork, prop.multiProcessorCount * 16LL));
+            out.pool_j.emplace_back((std::size_t)nwork * wq);
+            out.pool_c.emplace_back((std::size_t)nwork * wq * sizeof(T));
+            HIP_CHECK(hipMemsetAsync(counter.data(), 0, 2 * sizeof(int), s));
+            SpaiColumnOut o{out.c_status.data(), out.c_fits.data(), out.c_growths.data(),
+                            out.c_clamps.data(), out.c_q.data(), out.c_res.data(),
+                            out.c_piv.data(), out.c_flops.data(), out.c_off.data(),
+                            out.c_pass.data(), pass};
+            pass_labels.push_back("spai_wave_kernel<float>");
+            timed(c, "spai_wave_kernel<float>", 0.0, 0.0, [&] {
+                launch_spai_wave(s, blocks, a.ptr.data(), a.col.data(), (const float *)aval,
+                                 (const float *)d, bptr.data(), bcol.data(),
+                                 (const float *)bval.data(), (float)eps, alpha, beta,
+                                 work.data(), nwork, counter.data(), o,
+                                 out.pool_j.back().data(), out.pool_c.back().as<float>(), wq);
             }, true);
             hipLaunchKernelGGL(collect_overflow_kernel, dim3(ceil_div(nwork, 256)), dim3(256), 0,
                                s, nwork, work.data(), out.c_status.data(), next.data(),

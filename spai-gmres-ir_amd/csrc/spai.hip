@@ -646,6 +646,629 @@ template void launch_spai_columns<float>(hipStream_t, int, int, int, const int *
                                          int, int, float *, int *, SpaiColumnOut, int *, float *,
                                          std::int64_t);
 
+// ================================================================ wave kernel
+// FP32 fits of the first-bucket columns (the mixed path), one wavefront per
+// column, the dense problem in registers and LDS: lane r owns row r of the
+// least-squares matrix (m <= 64).
+//   * index sets: the rows that make up a set (rows of A for the shadow,
+//     rows of B = A^T for the candidates) are loaded cooperatively, then
+//     (column, entry) keys are bitonic-sorted across the wave; adjacent
+//     compares give the sorted, duplicate-free set and every entry's
+//     position in it;
+//   * QR: each lane keeps its row in registers; the dot products of the
+//     Householder vector with the trailing columns come from one
+//     reduce-scatter over the wave (lane 4c ends with column c);
+//   * candidate scores: the candidate stage keeps B's values, so the terms
+//     res_i * b_ij d_j and (b_ij d_j)^2 are formed per entry and summed per
+//     candidate in sorted order (ascending i, as the lane kernel sums A's
+//     row j), with no further loads;
+//   * the beta smallest (score, index) pairs are found by ranking.
+// The method and its decisions are the lane kernel's; the FP32 summation
+// order of the QR and norms differs, which the FP32 build allows (its
+// patterns are compared with the FP64 oracle's and differences counted).
+// Columns beyond the caps stop with kOverflow and go to the workgroup kernel.
+constexpr int kWM = 64;      // rows (one per lane)
+constexpr int kWQ = 16;      // pattern columns
+constexpr int kWC = 128;     // distinct candidate rows including J
+constexpr int kWS = 512;     // staged entries (9-bit entry numbers in the sort keys)
+constexpr int kWW = kWQ + 1; // LDS row stride of R
+
+struct WaveLds {
+    union {
+        float R[kWM][kWW]; // gathered B(I, J) (unchanged by the QR, which runs in registers)
+        float sden[kWS];   // candidate stage: (b_ij d_j)^2 in sorted order
+    };
+    float sval[kWS];     // staged values by entry; candidate stage: res_i b_ij d_j, sorted
+    int skey[kWS];       // candidate stage: keys in sorted order
+    signed char sc[kWS]; // entry -> staging row
+    int sbeg[kWM];       // staging row r holds entries e of source index sbeg[r] + e
+    float sd[kWM];       // staging row scale
+    int I[kWM];
+    int J[kWQ];
+    int sel[kWQ];
+    int cand[kWC];
+    short rbeg[kWC]; // first sorted entry of each candidate
+    float score[kWC];
+    float res[kWM];
+};
+
+__device__ inline void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+__device__ inline int wave_scan_excl(int v, int lane) {
+    int x = v;
+#pragma unroll
+    for (int off = 1; off < kWave; off <<= 1) {
+        const int y = __shfl_up(x, off, kWave);
+        if (lane >= off)
+            x += y;
+    }
+    return x - v;
+}
+__device__ inline int wave_sum_int(int v) {
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1)
+        v += __shfl_xor(v, off, kWave);
+    return v;
+}
+__device__ inline float lane_value(float v, int src) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), src));
+}
+
+// Lane r < nr stages row `row` of a CSR structure: entries [off_r, off_r +
+// len_r) map to row r (sc) with source index sbeg[r] + e and scale sd[r]
+// (dscale[row], or 1).  Returns the entry count, or -1 beyond kWS.
+__device__ int wave_rows(WaveLds &W, const int *__restrict__ ptr,
+                         const float *__restrict__ dscale, int row, int nr, int lane) {
+    const bool has = lane < nr;
+    const int e0 = has ? ptr[row] : 0;
+    const int e1 = has ? ptr[row + 1] : 0;
+    const float d = has && dscale ? dscale[row] : 1.f;
+    const int len = e1 - e0;
+    const int off = wave_scan_excl(len, lane);
+    const int total = __shfl(off + len, kWave - 1, kWave);
+    if (total > kWS)
+        return -1;
+    W.sbeg[lane] = e0 - off;
+    W.sd[lane] = d;
+    for (int t = 0; t < len; ++t)
+        W.sc[off + t] = (signed char)lane;
+    wave_sync();
+    return total;
+}
+
+// Bitonic sort of 64 P keys, element e = lane * P + i held in key[i].
+template <int P> __device__ inline void wave_bitonic(unsigned long long (&key)[P], int lane) {
+#pragma unroll
+    for (int k = 2; k <= kWave * P; k <<= 1) {
+#pragma unroll
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            if (j < P) {
+#pragma unroll
+                for (int i = 0; i < P; ++i) {
+                    const int p = i ^ j;
+                    if (p > i) {
+                        const bool up = ((lane * P + i) & k) == 0;
+                        const unsigned long long x = key[i], y = key[p];
+                        const bool sw = up ? x > y : x < y;
+                        key[i] = sw ? y : x;
+                        key[p] = sw ? x : y;
+                    }
+                }
+            } else {
+                const int lj = j / P;
+                const bool lower = (lane & lj) == 0;
+#pragma unroll
+                for (int i = 0; i < P; ++i) {
+                    const unsigned long long o = __shfl_xor(key[i], lj, kWave);
+                    const bool up = ((lane * P + i) & k) == 0;
+                    const bool take_min = lower == up;
+                    const bool less = o < key[i];
+                    key[i] = take_min == less ? o : key[i];
+                }
+            }
+        }
+    }
+}
+
+// The staged entries e < total of (col, val), sorted by (col, e) and
+// numbered by distinct column.  Shadow (kCand false): I <- the sorted set and
+// R(position, staging row) <- val * sd.  Candidates (kCand true): the set
+// minus J -> cand and rbeg, with each entry's score terms stored in sorted
+// order.  Returns the set size; outputs are written only when it is within
+// the cap (kWM, or kWC - q), which the caller checks.
+template <int P, bool kCand>
+__device__ int wave_sort_set(WaveLds &W, const int *__restrict__ col,
+                             const float *__restrict__ val, const float *__restrict__ dscale,
+                             int total, int q, int lane) {
+    unsigned long long key[P];
+    {
+        int src[P];
+#pragma unroll
+        for (int i = 0; i < P; ++i) {
+            const int e = lane * P + i;
+            const int ec = e < total ? e : 0;
+            src[i] = W.sbeg[W.sc[ec]] + ec;
+        }
+        int c[P];
+        float v[P];
+#pragma unroll
+        for (int i = 0; i < P; ++i) {
+            c[i] = col[src[i]];
+            v[i] = val[src[i]];
+        }
+#pragma unroll
+        for (int i = 0; i < P; ++i) {
+            const int e = lane * P + i;
+            if (e < total)
+                W.sval[e] = v[i] * W.sd[W.sc[e]];
+            key[i] = e < total ? ((unsigned long long)(unsigned)c[i] << 9) | (unsigned)e : ~0ull;
+        }
+    }
+    wave_bitonic<P>(key, lane);
+    wave_sync(); // sval of every entry written
+    int kv[P], ev[P], pos[P];
+    bool keep[P], first[P];
+#pragma unroll
+    for (int i = 0; i < P; ++i) {
+        kv[i] = key[i] == ~0ull ? -1 : (int)(key[i] >> 9);
+        ev[i] = (int)(key[i] & 511u);
+    }
+    int prev = __shfl_up(kv[P - 1], 1, kWave);
+    if (lane == 0)
+        prev = -2;
+    int cnt = 0;
+#pragma unroll
+    for (int i = 0; i < P; ++i) {
+        const int pv = i ? kv[i - 1] : prev;
+        first[i] = kv[i] >= 0 && kv[i] != pv;
+        bool in_j = false;
+        if (kCand && kv[i] >= 0) {
+            int lo = 0, hi = q;
+            while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if (W.J[mid] < kv[i])
+                    lo = mid + 1;
+                else
+                    hi = mid;
+            }
+            in_j = lo < q && W.J[lo] == kv[i];
+        }
+        keep[i] = kv[i] >= 0 && !in_j;
+        cnt += first[i] && keep[i] ? 1 : 0;
+    }
+    const int nset = wave_sum_int(cnt);
+    {
+        int run = wave_scan_excl(cnt, lane) - 1;
+#pragma unroll
+        for (int i = 0; i < P; ++i) {
+            if (first[i] && keep[i])
+                ++run;
+            pos[i] = run;
+        }
+    }
+    if (!kCand) {
+        if (nset > kWM)
+            return nset;
+#pragma unroll
+        for (int i = 0; i < P; ++i) {
+            if (first[i])
+                W.I[pos[i]] = kv[i];
+            if (keep[i])
+                W.R[pos[i]][W.sc[ev[i]]] = W.sval[ev[i]];
+        }
+        wave_sync();
+        return nset;
+    }
+    if (nset + q > kWC)
+        return nset;
+    float dj[P], sv[P], rs[P];
+#pragma unroll
+    for (int i = 0; i < P; ++i)
+        dj[i] = dscale[keep[i] ? kv[i] : 0];
+#pragma unroll
+    for (int i = 0; i < P; ++i) {
+        const int e = keep[i] ? ev[i] : 0;
+        sv[i] = W.sval[e];
+        rs[i] = W.res[W.sc[e]];
+    }
+    wave_sync(); // the entry-ordered values are read before the sorted ones replace them
+#pragma unroll
+    for (int i = 0; i < P; ++i) {
+        const int s = lane * P + i;
+        if (kv[i] >= 0)
+            W.skey[s] = kv[i];
+        if (keep[i]) {
+            const float v = sv[i] * dj[i];
+            W.sval[s] = rs[i] * v;
+            W.sden[s] = v * v;
+        }
+        if (first[i] && keep[i]) {
+            W.cand[pos[i]] = kv[i];
+            W.rbeg[pos[i]] = (short)s;
+        }
+    }
+    wave_sync();
+    return nset;
+}
+
+template <bool kCand>
+__device__ int wave_set(WaveLds &W, const int *col, const float *val, const float *dscale,
+                        int total, int q, int lane) {
+    if (total <= 0) // (rows of A and B in these stages always hold an entry)
+        return 0;
+    if (total <= kWave)
+        return wave_sort_set<1, kCand>(W, col, val, dscale, total, q, lane);
+    if (total <= 2 * kWave)
+        return wave_sort_set<2, kCand>(W, col, val, dscale, total, q, lane);
+    if (total <= 4 * kWave)
+        return wave_sort_set<4, kCand>(W, col, val, dscale, total, q, lane);
+    return wave_sort_set<8, kCand>(W, col, val, dscale, total, q, lane);
+}
+
+// Sums a[c] (c < 16) over the wave: lane l ends with the sum of column
+// (l >> 2) & 15 (halving exchanges at distances 32, 16, 8, 4, then 2 and 1).
+__device__ inline float wave_reduce_scatter16(float (&a)[kWQ], int lane) {
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+        const int half = 8 >> s;
+        const int mask = 32 >> s;
+        const bool hi = (lane & mask) != 0;
+#pragma unroll
+        for (int i = 0; i < half; ++i) {
+            const float keep = hi ? a[i + half] : a[i];
+            const float send = hi ? a[i] : a[i + half];
+            a[i] = keep + __shfl_xor(send, mask, kWave);
+        }
+    }
+    float x = a[0];
+    x += __shfl_xor(x, 2, kWave);
+    x += __shfl_xor(x, 1, kWave);
+    return x;
+}
+
+__global__ void __launch_bounds__(kWave)
+spai_wave_kernel(const int *__restrict__ aptr, const int *__restrict__ acol,
+                 const float *__restrict__ aval, const float *__restrict__ dscale,
+                 const int *__restrict__ bptr, const int *__restrict__ bcol,
+                 const float *__restrict__ bval, float eps, long long alpha, int beta,
+                 const int *__restrict__ worklist, int nwork, int *__restrict__ cursor,
+                 SpaiColumnOut out, int *__restrict__ out_j, float *__restrict__ out_c,
+                 std::int64_t out_stride) {
+    __shared__ WaveLds W;
+    const int lane = threadIdx.x;
+    for (;;) {
+        int slot = 0;
+        if (lane == 0)
+            slot = atomicAdd(cursor, 1);
+        slot = __shfl(slot, 0, kWave);
+        if (slot >= nwork)
+            break;
+        const int k = worklist[slot];
+        int status = kSpaiNotAttempted;
+        int fits = 0, growths = 0, clamps = 0, q = 1, qdone = 0;
+        double last_res = __builtin_nan(""), last_piv = __builtin_nan(""), qr_flops = 0;
+        if (lane == 0)
+            W.J[0] = k;
+        wave_sync();
+        int *jout = out_j + (std::int64_t)slot * out_stride;
+        float *cout_ = out_c + (std::int64_t)slot * out_stride;
+        for (;;) {
+            // ---- shadow I = union of cols(A row j), j in J, and the gather
+#pragma unroll
+            for (int c = 0; c < kWW; ++c)
+                W.R[lane][c] = 0.f;
+            const int na = wave_rows(W, aptr, dscale, lane < q ? W.J[lane] : 0, q, lane);
+            if (na < 0) {
+                status = kSpaiOverflow;
+                break;
+            }
+            const int m = wave_set<false>(W, acol, aval, nullptr, na, q, lane);
+            if (m > kWM) {
+                status = kSpaiOverflow;
+                break;
+            }
+            if (m < q) {
+                status = kSpaiRankDeficient;
+                break;
+            }
+            const int irow = lane < m ? W.I[lane] : -1;
+            const unsigned long long kmask = __ballot(irow == k);
+            if (kmask == 0ull) {
+                status = kSpaiNumericalFailure;
+                break;
+            }
+            const int pk = __ffsll((long long)kmask) - 1;
+            // ---- Householder least squares, lane r holding row r
+            float r[kWQ];
+#pragma unroll
+            for (int c = 0; c < kWQ; ++c)
+                r[c] = W.R[lane][c];
+            float rh = lane == pk ? 1.f : 0.f;
+            float rowsum = 0.f;
+#pragma unroll
+            for (int c = 0; c < kWQ; ++c)
+                rowsum += fabsf(r[c]);
+            const float inf_norm = wave_max(rowsum);
+            float min_pivot = INFINITY, dg = 0.f;
+            int fit_status = kSpaiSuccess;
+            // (every step is unrolled, so that r[] stays in registers; steps
+            // at or past q, or after a failure, are skipped as a whole)
+#pragma unroll
+            for (int j = 0; j < kWQ; ++j) {
+                if (j < q && fit_status == kSpaiSuccess) {
+                    const bool act = lane >= j && lane < m;
+                    const float x = act ? r[j] : 0.f;
+                    const float length = (float)sqrt(wave_sum((double)x * x));
+                    if (!isfinite(length)) {
+                        fit_status = kSpaiNumericalFailure;
+                    } else if (length == 0.f) {
+                        fit_status = kSpaiRankDeficient;
+                    } else {
+                        const float xj = lane_value(x, j);
+                        const float diagonal = -copysignf(length, xj);
+                        const float vnorm = sqrtf(2.f + 2.f * fabsf(xj) / length);
+                        float v = act ? x / length : 0.f;
+                        if (lane == j)
+                            v += copysignf(1.f, v);
+                        v = v / vnorm;
+                        float p[kWQ];
+#pragma unroll
+                        for (int c = 0; c < kWQ; ++c)
+                            p[c] = act && c > j ? v * r[c] : 0.f;
+                        const float d = wave_reduce_scatter16(p, lane);
+                        const float dr = wave_sum(v * rh);
+                        const float v2 = 2.f * v;
+                        bool bad = false;
+#pragma unroll
+                        for (int c = j + 1; c < kWQ; ++c) {
+                            if (c < q) {
+                                const float dc = lane_value(d, 4 * c);
+                                if (act) {
+                                    r[c] -= v2 * dc;
+                                    bad = bad || !isfinite(r[c]);
+                                }
+                            }
+                        }
+                        if (act) {
+                            rh -= v2 * dr;
+                            bad = bad || !isfinite(rh);
+                        }
+                        if (__any(bad)) {
+                            fit_status = kSpaiNumericalFailure;
+                        } else {
+                            if (lane == j)
+                                dg = diagonal;
+                            min_pivot = fminf(min_pivot, fabsf(diagonal));
+                        }
+                    }
+                }
+            }
+            // back substitution, one coefficient per lane (lane j: coef_j)
+            float coef = 0.f;
+            if (fit_status == kSpaiSuccess) {
+                float acc = rh;
+#pragma unroll
+                for (int j = kWQ - 1; j >= 0; --j) {
+                    if (j < q && fit_status == kSpaiSuccess) {
+                        int st = kSpaiSuccess;
+                        float cj = 0.f;
+                        if (lane == j) {
+                            if (!isfinite(dg))
+                                st = kSpaiNumericalFailure;
+                            else if (dg == 0.f)
+                                st = kSpaiRankDeficient;
+                            else {
+                                cj = acc / dg;
+                                if (!isfinite(cj))
+                                    st = kSpaiNumericalFailure;
+                            }
+                            coef = cj;
+                        }
+                        st = __builtin_amdgcn_readlane(st, j);
+                        if (st != kSpaiSuccess) {
+                            fit_status = st;
+                        } else {
+                            cj = lane_value(cj, j);
+                            if (lane < j)
+                                acc -= r[j] * cj;
+                        }
+                    }
+                }
+            }
+            if (fit_status != kSpaiSuccess) {
+                status = fit_status;
+                break;
+            }
+            const float pivot_ratio = inf_norm == 0.f ? 0.f : min_pivot / inf_norm;
+            // ---- residual s = B(I, J) m - e_k(I), from the gathered rows
+            float sres = 0.f;
+#pragma unroll
+            for (int c = 0; c < kWQ; ++c)
+                if (c < q)
+                    sres += W.R[lane][c] * lane_value(coef, c);
+            if (lane == pk)
+                sres -= 1.f;
+            if (lane >= m)
+                sres = 0.f;
+            W.res[lane] = sres;
+            const float rnorm = (float)sqrt(wave_sum((double)sres * sres));
+            if (!isfinite(rnorm)) {
+                status = kSpaiNumericalFailure;
+                break;
+            }
+            if (fits != 0)
+                ++growths;
+            ++fits;
+            {
+                const double fm = m, fq = q;
+                qr_flops += 2 * fm * fq * fq - 2 * fq * fq * fq / 3 + 4 * fm * fq + fq * fq;
+            }
+            if (lane < q) {
+                jout[lane] = W.J[lane];
+                cout_[lane] = coef;
+            }
+            qdone = q;
+            last_res = (double)rnorm;
+            last_piv = (double)pivot_ratio;
+            if (rnorm <= eps) {
+                status = kSpaiSuccess;
+                break;
+            }
+            if ((long long)growths == alpha) {
+                status = kSpaiIterationLimit;
+                break;
+            }
+            // ---- candidates: union of B rows i in I, minus J (the union
+            // holds J: row j of A lies in the shadow, so j is in B(I))
+            wave_sync(); // residual visible to the candidate stage
+            const int nb = wave_rows(W, bptr, nullptr, irow < 0 ? 0 : irow, m, lane);
+            if (nb < 0) {
+                status = kSpaiOverflow;
+                break;
+            }
+            const int nc = wave_set<true>(W, bcol, bval, dscale, nb, q, lane);
+            if (nc + q > kWC) {
+                status = kSpaiOverflow;
+                break;
+            }
+            if (nc == 0) {
+                status = kSpaiEmptyCandidates;
+                break;
+            }
+            // ---- restricted scores, from the sorted terms
+            const float rr = rnorm * rnorm;
+            int first_bad = INT_MAX, bad_status = kSpaiSuccess, cl_before[kWC / kWave];
+#pragma unroll
+            for (int u = 0; u < kWC / kWave; ++u) {
+                const int t = lane + u * kWave;
+                cl_before[u] = 0;
+                if (t < nc) {
+                    const int j = W.cand[t];
+                    int s = W.rbeg[t];
+                    float dt = 0.f, den = 0.f;
+                    do {
+                        dt += W.sval[s];
+                        den += W.sden[s];
+                        ++s;
+                    } while (s < nb && W.skey[s] == j);
+                    float sv = 0.f;
+                    bool cl = false;
+                    const int st = score_of(rr, dt, den, sv, cl);
+                    W.score[t] = sv;
+                    if (st != kSpaiSuccess) {
+                        if (t < first_bad) {
+                            first_bad = t;
+                            bad_status = st;
+                        }
+                    } else {
+                        cl_before[u] = cl ? 1 : 0;
+                    }
+                }
+            }
+            const int fb = -wave_max(-first_bad);
+            {
+                int cl = 0;
+#pragma unroll
+                for (int u = 0; u < kWC / kWave; ++u)
+                    if (lane + u * kWave < fb)
+                        cl += cl_before[u];
+                clamps += wave_sum_int(cl);
+            }
+            if (fb != INT_MAX) {
+                status = __shfl(bad_status, __ffsll((long long)__ballot(first_bad == fb)) - 1,
+                                kWave);
+                break;
+            }
+            wave_sync();
+            float msum = 0.f;
+            for (int t = 0; t < nc; ++t) // sequential, as the lane kernel
+                msum += W.score[t];
+            const float mean = msum / (float)nc;
+            if (!isfinite(mean)) {
+                status = kSpaiNumericalFailure;
+                break;
+            }
+            // ---- the beta smallest (score, index) with score <= mean: each
+            // eligible candidate counts the eligible ones before it
+            int nsel = 0;
+#pragma unroll
+            for (int u = 0; u < kWC / kWave; ++u) {
+                const int t = lane + u * kWave;
+                int rank = INT_MAX;
+                if (t < nc && W.score[t] <= mean) {
+                    const float st_ = W.score[t];
+                    const int jt = W.cand[t];
+                    rank = 0;
+                    for (int o = 0; o < nc; ++o) {
+                        const float so = W.score[o];
+                        if (so <= mean && (so < st_ || (so == st_ && W.cand[o] < jt)))
+                            ++rank;
+                    }
+                }
+                nsel += wave_sum_int(rank != INT_MAX ? 1 : 0);
+                if (rank < beta && rank < kWQ) // (more would overflow the pattern)
+                    W.sel[rank] = W.cand[t];
+            }
+            nsel = nsel < beta ? nsel : beta;
+            if (nsel == 0) {
+                status = kSpaiNoGrowth;
+                break;
+            }
+            if (q + nsel > kWQ) { // the grown pattern would not fit
+                status = kSpaiOverflow;
+                break;
+            }
+            wave_sync();
+            // J <- sorted(J u selection)
+            {
+                const int nq = q + nsel;
+                int mine = INT_MAX;
+                if (lane < q)
+                    mine = W.J[lane];
+                else if (lane < nq)
+                    mine = W.sel[lane - q];
+                int rank = 0;
+                for (int t = 0; t < nq; ++t)
+                    rank += __shfl(mine, t, kWave) < mine ? 1 : 0;
+                wave_sync();
+                if (lane < nq)
+                    W.J[rank] = mine;
+                q = nq;
+                wave_sync();
+            }
+        }
+        if (lane == 0) {
+            out.status[k] = status;
+            out.fits[k] = fits;
+            out.growths[k] = growths;
+            out.clamps[k] = clamps;
+            out.q[k] = status == kSpaiOverflow ? 0 : qdone;
+            out.residual[k] = last_res;
+            out.pivot[k] = last_piv;
+            out.flops[k] = qr_flops;
+            out.off[k] = (std::int64_t)slot * out_stride;
+            out.pass[k] = out.pass_id;
+        }
+        wave_sync();
+    }
+}
+
+void launch_spai_wave(hipStream_t s, int blocks, const int *aptr, const int *acol,
+                      const float *aval, const float *dscale, const int *bptr, const int *bcol,
+                      const float *bval, float eps, long long alpha, int beta,
+                      const int *worklist, int nwork, int *cursor, SpaiColumnOut out,
+                      int *out_j, float *out_c, std::int64_t out_stride) {
+    hipLaunchKernelGGL(spai_wave_kernel, dim3(blocks), dim3(kWave), 0, s, aptr, acol, aval,
+                       dscale, bptr, bcol, bval, eps, alpha, beta, worklist, nwork, cursor, out,
+                       out_j, out_c, out_stride);
+    HIP_LAUNCH_CHECK();
+}
+int spai_wave_qcap() {
+    return kWQ;
+}
+
 // ================================================================ block kernel
 // Columns that outgrow the lane kernel's caps (long rows, wide patterns,
 // thousands of candidates: the power-law matrices) are fitted by one

@@ -159,6 +159,39 @@ def test_spai_block_kernel_fp32_and_mixed_ir(ctx, power_law_small):
     assert np.linalg.norm(b - A @ x) <= 1e-10 * np.linalg.norm(b)
 
 
+@pytest.mark.parametrize("kernel", ["wave", "lane"])
+@pytest.mark.parametrize("side", [0, 32])
+def test_spai_fp32_kernels_vs_fp64(ctx, monkeypatch, kernel, side):
+    """Both FP32 first-bucket kernels (wavefront per column, lane per column),
+    forced in turn, on the 64x64 stencil and the 32^3 convection-diffusion
+    matrix: every column's adaptive pattern equals the FP64 build's (itself
+    bit-exact with the oracle) and the coefficients agree to 1e-5 (max-norm
+    per row of P, relative)."""
+    import bspai_hip
+
+    P = _problems()
+    a = P.convection_diffusion_2d() if side == 0 else P.convection_diffusion_3d(side)
+    m = bspai_hip.Matrix(a, ctx)
+    alpha = -(-a.n // 8)
+    ref = bspai_hip.construct_spai(m, epsilon=0.3, alpha=alpha, beta=8)
+    monkeypatch.setenv("BSPAI_SPAI_FP32_KERNEL", kernel)
+    s = bspai_hip.construct_spai(m, epsilon=0.3, alpha=alpha, beta=8, precision="mixed")
+    assert ref.status == "success" and s.status == "success"
+    assert s.fits == ref.fits and s.growths == ref.growths
+    gp, gc, gv = ref.preconditioner()
+    ptr, col, val = s.preconditioner()
+    worst, diff_rows = 0.0, 0
+    for r in range(a.n):
+        c32, v32 = col[ptr[r]:ptr[r + 1]], val[ptr[r]:ptr[r + 1]]
+        c64, v64 = gc[gp[r]:gp[r + 1]], gv[gp[r]:gp[r + 1]]
+        if not np.array_equal(c32, c64):
+            diff_rows += 1
+            continue
+        worst = max(worst, np.max(np.abs(v32 - v64)) / np.max(np.abs(v64)))
+    print(f"{kernel} kernel, n={a.n}: {diff_rows} differing patterns, worst {worst:.2e}")
+    assert diff_rows == 0 and worst <= 1e-5
+
+
 @pytest.mark.parametrize("n", [16, 64, 256, 1024])
 @pytest.mark.parametrize("diag", [8, 5])
 def test_spai_band_bit_exact(ctx, band_golden, n, diag):
