@@ -189,39 +189,44 @@ __device__ bool back_substitute(const double *__restrict__ H, const double *__re
 
 // ------------------------------------------------------- compact MGS passes
 constexpr int kProjWaves = 8;                  // waves per projection block
-constexpr int kProjVecs = 2 * kProjWaves;      // granule of basis vectors per block
-constexpr int kProjBlocks = 2048;              // blocks wanted per projection launch
-constexpr int kProjPairs = kTileRows / (2 * kWave); // row pairs per lane
+constexpr int kProjPerWave = 4;                // basis vectors streamed together by a wave
+constexpr int kProjVecs = kProjPerWave * kProjWaves; // basis vectors per block
 constexpr int kReduceWaves = 16;               // one value per wave
-constexpr int kUpdateThreads = 1024;            // one row pair per thread
+constexpr int kUpdateThreads = 256;             // 8 rows of a tile per thread
+// Basis vectors per load batch of the update: 8 16-byte loads per thread.
+template <typename T> constexpr int kUpdBatch = sizeof(T) == 8 ? 2 : 4;
 
-// Rows r, r+1 of a vector in one 2-wide load when the pair is aligned (basis
-// strides are even and a tile's pairs share the parity of the tile start,
-// so the test is block-uniform), else in two loads.
-template <typename T> struct Vec2;
-template <> struct Vec2<float> { using type = float2; };
-template <> struct Vec2<double> { using type = double2; };
-template <typename T> __device__ inline void load_pair(const T *p, bool aligned, T &x, T &y) {
-    if (aligned) {
-        const typename Vec2<T>::type v = *reinterpret_cast<const typename Vec2<T>::type *>(p);
-        x = v.x;
-        y = v.y;
-    } else {
-        x = p[0];
-        y = p[1];
-    }
+// 16-byte accesses: E rows of T per access.
+template <typename T> struct Vec16;
+template <> struct Vec16<float> { using type = float4; };
+template <> struct Vec16<double> { using type = double2; };
+template <typename T> __device__ inline T lane_of(const typename Vec16<T>::type &v, int e);
+template <> __device__ inline float lane_of<float>(const float4 &v, int e) {
+    return e == 0 ? v.x : e == 1 ? v.y : e == 2 ? v.z : v.w;
+}
+template <> __device__ inline double lane_of<double>(const double2 &v, int e) {
+    return e == 0 ? v.x : v.y;
 }
 
 // Pass 1, grid (tile, group of kProjVecs vectors): w and V_k of the tile are
-// staged in LDS; each wave streams two basis vectors of the group and forms
-// <V_i, w> and <V_i, V_k> (FP64 accumulation, each lane in row order, then a
-// fixed butterfly), written to gp[i][tile] and gp[cap+1+i][tile].
+// staged in LDS; each wave streams kProjPerWave basis vectors of the group
+// together (one 16-byte access of each per step, so that several are in
+// flight) and forms <V_i, w> and <V_i, V_k> (FP64 accumulation, each lane
+// over its rows, then a fixed butterfly), written to gp[i][tile] and
+// gp[cap+1+i][tile].  Full tiles whose start is 16-byte aligned (every tile
+// of a single system) use 16-byte accesses; the others read the same rows
+// per lane in the same order, one element at a time.
 template <typename T>
 __global__ void __launch_bounds__(kProjWaves *kWave)
-project_kernel(Tiles tiles, KrylovArgs<T> a, int k, int group) {
-    __shared__ T ws[kTileRows], vs[kTileRows];
+project_kernel(Tiles tiles, KrylovArgs<T> a, int k) {
+    using V16 = typename Vec16<T>::type;
+    constexpr int E = 16 / sizeof(T);
+    constexpr int G = kTileRows / (kWave * E); // 16-byte accesses per lane per vector
+    constexpr int P = kProjPerWave;
+    __shared__ V16 ws4[kTileRows / E], vs4[kTileRows / E];
+    T *ws = reinterpret_cast<T *>(ws4), *vs = reinterpret_cast<T *>(vs4);
     const int t = blockIdx.x, s = tiles.sys[t];
-    const int i0 = blockIdx.y * group;
+    const int i0 = blockIdx.y * kProjVecs;
     if (!a.st[s].active || i0 > k)
         return;
     const int r0 = tiles.row0[t], nr = tiles.row1[t] - r0;
@@ -233,53 +238,68 @@ project_kernel(Tiles tiles, KrylovArgs<T> a, int k, int group) {
     }
     __syncthreads();
     const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
-    const int iend = min(k, i0 + group - 1);
-    const std::int64_t nt = tiles.ntiles, off = a.cap + 1;
-    const bool aligned = (r0 & 1) == 0;
-    for (int ia = i0 + 2 * wave; ia <= iend; ia += 2 * kProjWaves) {
-        const int ib = ia + 1;
-        const bool two = ib <= iend;
-        const T *__restrict__ va = a.V[ia] + r0;
-        const T *__restrict__ vb = a.V[two ? ib : ia] + r0;
-        double dwa = 0, dva = 0, dwb = 0, dvb = 0;
-#pragma unroll 4
-        for (int m = 0; m < kProjPairs; ++m) {
-            const int r = 2 * (lane + m * kWave);
-            if (r + 1 < nr) {
-                T a0, a1, b0, b1;
-                load_pair(va + r, aligned, a0, a1);
-                load_pair(vb + r, aligned, b0, b1);
-                const double w0 = (double)ws[r], w1 = (double)ws[r + 1];
-                const double v0 = (double)vs[r], v1 = (double)vs[r + 1];
-                dwa += (double)a0 * w0;
-                dva += (double)a0 * v0;
-                dwb += (double)b0 * w0;
-                dvb += (double)b0 * v0;
-                dwa += (double)a1 * w1;
-                dva += (double)a1 * v1;
-                dwb += (double)b1 * w1;
-                dvb += (double)b1 * v1;
-            } else if (r < nr) {
-                const double xa = (double)va[r], xb = (double)vb[r];
-                dwa += xa * (double)ws[r];
-                dva += xa * (double)vs[r];
-                dwb += xb * (double)ws[r];
-                dvb += xb * (double)vs[r];
+    const int ib = i0 + P * wave; // this wave's vectors ib .. ib+P-1 (those <= k)
+    if (ib > k)
+        return;
+    const T *__restrict__ vp[P];
+#pragma unroll
+    for (int u = 0; u < P; ++u)
+        vp[u] = a.V[min(ib + u, k)] + r0;
+    double dw[P], dv[P];
+#pragma unroll
+    for (int u = 0; u < P; ++u)
+        dw[u] = dv[u] = 0;
+    if (nr == kTileRows && r0 % E == 0) {
+#pragma unroll 2
+        for (int g = 0; g < G; ++g) {
+            const int q = g * kWave + lane;
+            V16 x[P];
+#pragma unroll
+            for (int u = 0; u < P; ++u)
+                x[u] = reinterpret_cast<const V16 *>(vp[u])[q];
+            const V16 wq = ws4[q], vq = vs4[q];
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                const double we = (double)lane_of<T>(wq, e), ve = (double)lane_of<T>(vq, e);
+#pragma unroll
+                for (int u = 0; u < P; ++u) {
+                    const double xe = (double)lane_of<T>(x[u], e);
+                    dw[u] += xe * we;
+                    dv[u] += xe * ve;
+                }
             }
         }
-        dwa = wave_sum(dwa);
-        dva = wave_sum(dva);
-        dwb = wave_sum(dwb);
-        dvb = wave_sum(dvb);
-        if (lane == 0) {
-            a.gp[ia * nt + t] = dwa;
-            if (ia < k)
-                a.gp[(off + ia) * nt + t] = dva;
-            if (two) {
-                a.gp[ib * nt + t] = dwb;
-                if (ib < k)
-                    a.gp[(off + ib) * nt + t] = dvb;
+    } else {
+        for (int g = 0; g < G; ++g) {
+            const int q = g * kWave + lane;
+            for (int e = 0; e < E; ++e) {
+                const int r = q * E + e;
+                if (r < nr) {
+                    const double we = (double)ws[r], ve = (double)vs[r];
+#pragma unroll
+                    for (int u = 0; u < P; ++u) {
+                        const double xe = (double)vp[u][r];
+                        dw[u] += xe * we;
+                        dv[u] += xe * ve;
+                    }
+                }
             }
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < P; ++u) {
+        dw[u] = wave_sum(dw[u]);
+        dv[u] = wave_sum(dv[u]);
+    }
+    if (lane == 0) {
+        const std::int64_t nt = tiles.ntiles, off = a.cap + 1;
+#pragma unroll
+        for (int u = 0; u < P; ++u) {
+            const int i = ib + u;
+            if (i <= k)
+                a.gp[i * nt + t] = dw[u];
+            if (i < k)
+                a.gp[(off + i) * nt + t] = dv[u];
         }
     }
 }
@@ -301,6 +321,7 @@ gram_reduce_kernel(Tiles tiles, KrylovArgs<T> a, int k) {
     const std::int64_t nt = tiles.ntiles;
     const double *row = a.gp + (v <= k ? v : a.cap + 1 + (v - k - 1)) * nt;
     double sum = 0;
+#pragma unroll 8
     for (int t = tiles.sys_tile0[s] + lane; t < tiles.sys_tile1[s]; t += kWave)
         sum += row[t];
     sum = wave_sum(sum);
@@ -411,45 +432,271 @@ __global__ void __launch_bounds__(kWave) gram_solve_kernel(Tiles tiles, KrylovAr
     }
 }
 
+// The same solve for k < 256 with a 1024-thread block: all waves stage
+// column chunks of L in LDS (coalesced, every load of a chunk in flight at
+// once), then wave 0 runs the substitution from LDS, unknown j in lane
+// j % 64, slot j / 64, each pivot broadcast by readlane.  Every h_j again
+// sees its terms in the order i = 0, 1, ..., j-1 (bit-identical to the
+// kernel above); the rotations follow from LDS.
+constexpr int kSolveThreads = 1024;
+constexpr int kSolveSlots = 4;        // unknowns per lane: k < 256
+constexpr int kSolveChunk = 6144;     // L entries per LDS chunk (48 KiB)
+static_assert(kGramSolveLdsMaxK == kSolveSlots * kWave - 1, "LDS Gram solve capacity");
+
+__device__ inline double readlane_f64(double v, int l) {
+    const long long b = __double_as_longlong(v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, l);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), l);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kSolveThreads)
+gram_solve_lds_kernel(Tiles tiles, KrylovArgs<T> a, int k) {
+    __shared__ double Lc[kSolveChunk];
+    __shared__ double hs[kSolveSlots * kWave], lcs[kSolveSlots * kWave],
+        lsn[kSolveSlots * kWave];
+    const int s = blockIdx.x;
+    if (!a.st[s].active)
+        return;
+    const int tid = threadIdx.x, wave = tid / kWave, lane = tid % kWave, cap = a.cap;
+    const double *q = a.q + (std::int64_t)s * (cap + 1);
+    const double *L = a.gram + (std::int64_t)s * a.gstride;
+    double *h = a.h + (std::int64_t)s * (cap + 1);
+    double *col = a.H + (std::int64_t)s * a.hstride + hcol_offset(k);
+    const double *cs = a.cs + (std::int64_t)s * cap, *sn = a.sn + (std::int64_t)s * cap;
+    for (int r = tid; r < k; r += kSolveThreads) {
+        lcs[r] = cs[r];
+        lsn[r] = sn[r];
+    }
+    double acc[kSolveSlots];
+#pragma unroll
+    for (int u = 0; u < kSolveSlots; ++u) {
+        const int j = lane + u * kWave;
+        acc[u] = wave == 0 && j <= k ? q[j] : 0.0;
+    }
+    // Steps i = 0..k-1 (column i of L holds rows j = i+1..k), in chunks of
+    // whole columns.  Offset of column i in a chunk from column c0:
+    // sum_{c0 <= i' < i} (k - i').
+    for (int c0 = 0; c0 < k;) {
+        int c1 = c0, used = 0;
+        while (c1 < k && used + (k - c1) <= kSolveChunk) {
+            used += k - c1;
+            ++c1;
+        }
+        // flat entry e of the chunk -> column i (largest with off(i) <= e),
+        // row j = i + 1 + e - off(i); all of a thread's loads issued together
+        {
+            int src[kSolveChunk / kSolveThreads];
+#pragma unroll
+            for (int u = 0; u < kSolveChunk / kSolveThreads; ++u) {
+                const int e = min(tid + u * kSolveThreads, used - 1);
+                int lo = c0, hi = c1 - 1;
+                while (lo < hi) {
+                    const int mid = (lo + hi + 1) >> 1;
+                    if ((mid - c0) * k - ((mid - 1) * mid / 2 - (c0 - 1) * c0 / 2) <= e)
+                        lo = mid;
+                    else
+                        hi = mid - 1;
+                }
+                const int off = (lo - c0) * k - ((lo - 1) * lo / 2 - (c0 - 1) * c0 / 2);
+                src[u] = gram_index(lo + 1 + (e - off), lo, cap);
+            }
+            double v[kSolveChunk / kSolveThreads];
+#pragma unroll
+            for (int u = 0; u < kSolveChunk / kSolveThreads; ++u)
+                v[u] = L[src[u]];
+#pragma unroll
+            for (int u = 0; u < kSolveChunk / kSolveThreads; ++u)
+                if (tid + u * kSolveThreads < used)
+                    Lc[tid + u * kSolveThreads] = v[u];
+        }
+        __syncthreads();
+        if (wave == 0) {
+            for (int i = c0; i < c1; ++i) {
+                const int lo = (i - c0) * k - ((i - 1) * i / 2 - (c0 - 1) * c0 / 2);
+                // this step's L entries first (they do not wait for the chain)
+                double lv[kSolveSlots];
+#pragma unroll
+                for (int uu = 0; uu < kSolveSlots; ++uu) {
+                    const int j = lane + uu * kWave;
+                    lv[uu] = Lc[lo + min(max(j - i - 1, 0), k - i - 1)];
+                }
+                const int u = i / kWave;
+                const double mine = u == 0 ? acc[0] : u == 1 ? acc[1] : u == 2 ? acc[2] : acc[3];
+                const double hi = readlane_f64(mine, i % kWave);
+#pragma unroll
+                for (int uu = 0; uu < kSolveSlots; ++uu) {
+                    const int j = lane + uu * kWave;
+                    if (j > i && j <= k)
+                        acc[uu] = acc[uu] - lv[uu] * hi;
+                }
+            }
+        }
+        __syncthreads();
+        c0 = c1;
+    }
+    if (wave != 0)
+        return;
+#pragma unroll
+    for (int u = 0; u < kSolveSlots; ++u) {
+        const int j = lane + u * kWave;
+        if (j <= k) {
+            h[j] = acc[u];
+            hs[j] = acc[u];
+        }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    // Rotations 0..k-1 on rows 0..k (src/gmres.cpp:183-189), lane 0.
+    if (lane == 0) {
+        double up = hs[0];
+        bool bad = false;
+        constexpr int B = 8; // LDS reads batched ahead of the rotation chain
+        for (int r0 = 0; r0 < k; r0 += B) {
+            double lo[B], c[B], z[B];
+#pragma unroll
+            for (int u = 0; u < B; ++u) {
+                const int r = min(r0 + u, k - 1);
+                lo[u] = hs[r + 1];
+                c[u] = lcs[r];
+                z[u] = lsn[r];
+            }
+#pragma unroll
+            for (int u = 0; u < B; ++u) {
+                if (r0 + u < k) {
+                    const double top = c[u] * up + z[u] * lo[u];
+                    up = -z[u] * up + c[u] * lo[u];
+                    hs[r0 + u] = top;
+                    bad = bad || !isfinite(top) || !isfinite(up);
+                }
+            }
+        }
+        hs[k] = up;
+        a.st[s].rot_bad = bad;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    for (int r = lane; r <= k; r += kWave)
+        col[r] = hs[r];
+}
+
 // Pass 2: w <- w - sum_i h_i V_i (i ascending, in T as the reference's
-// w -= h V_j), and the norm partial of the new w into kSlotNorm.
+// w -= h V_j), and the norm partial of the new w into kSlotNorm.  Each thread
+// owns G 16-byte groups of the tile's rows; the basis is streamed in batches
+// of kUpdBatch<T> vectors, two batches in registers, so that one batch's loads
+// are in flight while the other is applied.  Tiles that are not full and
+// 16-byte aligned take the same rows per thread, element by element.
+template <typename T, int G>
+__device__ inline void upd_load(typename Vec16<T>::type (&buf)[kUpdBatch<T>][G],
+                                const KrylovArgs<T> &a, int base, int k, int r0, int tid) {
+    using V16 = typename Vec16<T>::type;
+#pragma unroll
+    for (int u = 0; u < kUpdBatch<T>; ++u) {
+        const V16 *__restrict__ v4 = reinterpret_cast<const V16 *>(a.V[min(base + u, k)] + r0);
+#pragma unroll
+        for (int g = 0; g < G; ++g)
+            buf[u][g] = v4[g * kUpdateThreads + tid];
+    }
+}
+template <typename T, int G, int E>
+__device__ inline void upd_apply(T (&x)[G][E], const typename Vec16<T>::type (&buf)[kUpdBatch<T>][G],
+                                 const double *h, int base, int k) {
+#pragma unroll
+    for (int u = 0; u < kUpdBatch<T>; ++u) {
+        if (base + u <= k) {
+            const T hi = (T)h[base + u];
+#pragma unroll
+            for (int g = 0; g < G; ++g)
+#pragma unroll
+                for (int e = 0; e < E; ++e)
+                    x[g][e] = x[g][e] - hi * lane_of<T>(buf[u][g], e);
+        }
+    }
+}
+
 template <typename T>
 __global__ void __launch_bounds__(kUpdateThreads)
 gram_update_kernel(Tiles tiles, KrylovArgs<T> a, int k) {
+    using V16 = typename Vec16<T>::type;
+    constexpr int E = 16 / sizeof(T);
+    constexpr int G = kTileRows / (kUpdateThreads * E);
     __shared__ double lds[kUpdateThreads / kWave];
     BSPAI_SYS_TILES
     if (!a.st[s].active)
         return;
     const double *h = a.h + (std::int64_t)s * (a.cap + 1);
     const int r0 = tiles.row0[t], nr = tiles.row1[t] - r0;
-    const int r = 2 * (int)threadIdx.x;
-    const bool aligned = (r0 & 1) == 0;
-    const int cnt = r + 1 < nr ? 2 : (r < nr ? 1 : 0);
-    T *__restrict__ w = a.w + r0 + r;
-    T x0 = T(0), x1 = T(0);
-    if (cnt == 2)
-        load_pair(w, aligned, x0, x1);
-    else if (cnt == 1)
-        x0 = w[0];
-#pragma unroll 4
-    for (int i = 0; i <= k; ++i) {
-        const T *__restrict__ v = a.V[i] + r0 + r;
-        const T hi = (T)h[i];
-        if (cnt == 2) {
-            T v0, v1;
-            load_pair(v, aligned, v0, v1);
-            x0 = x0 - hi * v0;
-            x1 = x1 - hi * v1;
-        } else if (cnt == 1) {
-            x0 = x0 - hi * v[0];
+    const int tid = threadIdx.x;
+    T *__restrict__ w = a.w + r0;
+    T x[G][E];
+    if (nr == kTileRows && r0 % E == 0) {
+        V16 *__restrict__ w4 = reinterpret_cast<V16 *>(w);
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            const V16 y = w4[g * kUpdateThreads + tid];
+#pragma unroll
+            for (int e = 0; e < E; ++e)
+                x[g][e] = lane_of<T>(y, e);
         }
+        constexpr int B = kUpdBatch<T>;
+        V16 b0[B][G], b1[B][G];
+        upd_load<T, G>(b0, a, 0, k, r0, tid);
+        for (int base = 0; base <= k; base += 2 * B) {
+            upd_load<T, G>(b1, a, base + B, k, r0, tid);
+            upd_apply<T, G, E>(x, b0, h, base, k);
+            if (base + B > k)
+                break;
+            upd_load<T, G>(b0, a, base + 2 * B, k, r0, tid);
+            upd_apply<T, G, E>(x, b1, h, base + B, k);
+        }
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            V16 y;
+            T *yp = reinterpret_cast<T *>(&y);
+#pragma unroll
+            for (int e = 0; e < E; ++e)
+                yp[e] = x[g][e];
+            w4[g * kUpdateThreads + tid] = y;
+        }
+    } else {
+#pragma unroll
+        for (int g = 0; g < G; ++g)
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                const int r = (g * kUpdateThreads + tid) * E + e;
+                x[g][e] = r < nr ? w[r] : T(0);
+            }
+        for (int i = 0; i <= k; ++i) {
+            const T *__restrict__ v = a.V[i] + r0;
+            const T hi = (T)h[i];
+#pragma unroll
+            for (int g = 0; g < G; ++g)
+#pragma unroll
+                for (int e = 0; e < E; ++e) {
+                    const int r = (g * kUpdateThreads + tid) * E + e;
+                    if (r < nr)
+                        x[g][e] = x[g][e] - hi * v[r];
+                }
+        }
+#pragma unroll
+        for (int g = 0; g < G; ++g)
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                const int r = (g * kUpdateThreads + tid) * E + e;
+                if (r < nr)
+                    w[r] = x[g][e];
+            }
     }
-    if (cnt >= 1)
-        w[0] = x0;
-    if (cnt == 2)
-        w[1] = x1;
-    const double vals[2] = {(double)x0, (double)x1};
-    store_norm_partial(vals, cnt, slot_ptr(a.part, tiles.ntiles, kSlotNorm), t, lds);
+    double vals[G * E];
+#pragma unroll
+    for (int g = 0; g < G; ++g)
+#pragma unroll
+        for (int e = 0; e < E; ++e)
+            vals[g * E + e] = (double)x[g][e];
+    store_norm_partial(vals, G * E, slot_ptr(a.part, tiles.ntiles, kSlotNorm), t, lds);
 }
 
 // h_{k+1,k} = ||w||; V_{k+1} = w / h (unless breakdown); the writer runs the
@@ -690,12 +937,8 @@ __global__ void gmres_reset_kernel(int nsys, KrylovArgs<T> k, const IrSysState *
 #define BSPAI_GRID dim3(tl.ntiles), dim3(kTileThreads), 0, s
 
 template <typename T> void gmres_project(hipStream_t s, const Tiles &tl, const KrylovArgs<T> &a, int k) {
-    // Vectors per block: enough groups to fill the chip, but each group
-    // restages w and V_k, so no more groups than that.
-    const int groups = std::max(1, kProjBlocks / std::max(1, tl.ntiles));
-    const int group = kProjVecs * (int)ceil_div(ceil_div(k + 1, groups), kProjVecs);
-    hipLaunchKernelGGL(project_kernel<T>, dim3(tl.ntiles, ceil_div(k + 1, group)),
-                       dim3(kProjWaves * kWave), 0, s, tl, a, k, group);
+    hipLaunchKernelGGL(project_kernel<T>, dim3(tl.ntiles, ceil_div(k + 1, kProjVecs)),
+                       dim3(kProjWaves * kWave), 0, s, tl, a, k);
     HIP_LAUNCH_CHECK();
 }
 template <typename T> void gmres_gram_reduce(hipStream_t s, const Tiles &tl, const KrylovArgs<T> &a, int k) {
@@ -704,7 +947,11 @@ template <typename T> void gmres_gram_reduce(hipStream_t s, const Tiles &tl, con
     HIP_LAUNCH_CHECK();
 }
 template <typename T> void gmres_gram_solve(hipStream_t s, const Tiles &tl, const KrylovArgs<T> &a, int k) {
-    hipLaunchKernelGGL(gram_solve_kernel<T>, dim3(tl.nsys), dim3(kWave), 0, s, tl, a, k);
+    if (k <= kGramSolveLdsMaxK)
+        hipLaunchKernelGGL(gram_solve_lds_kernel<T>, dim3(tl.nsys), dim3(kSolveThreads), 0, s, tl,
+                           a, k);
+    else
+        hipLaunchKernelGGL(gram_solve_kernel<T>, dim3(tl.nsys), dim3(kWave), 0, s, tl, a, k);
     HIP_LAUNCH_CHECK();
 }
 template <typename T> void gmres_gram_update(hipStream_t s, const Tiles &tl, const KrylovArgs<T> &a, int k) {

@@ -74,6 +74,9 @@ struct SysState {
     int rhs_zero, x0_is_solution, has_solution;
 };
 
+// Largest k solved by the LDS-staged Gram solve (beyond it, the one-wave kernel).
+constexpr int kGramSolveLdsMaxK = 255;
+
 template <typename T> struct KrylovArgs {
     SysState *st;
     T *const *V; // device array of basis-vector pointers (cap+1)
