@@ -9,6 +9,7 @@ solves/s; the plain FP64 path is timed in the same run and reported beside it.
 Inputs (matrix, right side) are resident in HBM before the timed region.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--extra many,powerlaw]
+       python bench.py --pmc-workload      (the short run profiled for HBM traffic)
 """
 from __future__ import annotations
 
@@ -40,7 +41,56 @@ def parse():
                          "'many' (256 systems batched vs looped), 'powerlaw' (uneven columns)")
     ap.add_argument("--powerlaw-n", type=int, default=50000,
                     help="rows of the power-law matrix timed by --extra powerlaw")
+    ap.add_argument("--pmc-workload", action="store_true",
+                    help="run the short workload profiled for HBM traffic counters instead of "
+                         "the benchmark: a 1 GiB device copy (counter calibration), one mixed "
+                         "SPAI build and 60 mixed GMRES iterations on the same system")
     return ap.parse_args()
+
+
+# HBM traffic per launch of a kernel on this workload, measured by rocprofv3
+# FETCH_SIZE / WRITE_SIZE passes over `bench.py --pmc-workload` and scaled by
+# the counters' reading for a copy of known size (the profile states both).
+TRAFFIC_PROFILE = os.path.join(REPO, "profiles", "spai_gmres_kernels_c2_128cube_hbm_traffic.md")
+
+
+def measured_traffic(kernel):
+    import re
+
+    try:
+        text = open(TRAFFIC_PROFILE).read()
+    except OSError:
+        return None
+    for name, val in re.findall(r"^traffic_bytes_per_launch (\S+) (\S+)$", text, re.M):
+        if name == kernel:
+            return float(val)
+    return None
+
+
+def pmc_workload(args):
+    import numpy as np
+    import torch
+
+    import bspai_hip
+    from bspai_hip import problems as P
+
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.Stream(dev)
+    torch.cuda.set_stream(stream)
+    ctx = bspai_hip.Context(0, stream=stream.cuda_stream)
+    src = torch.ones(1 << 27, dtype=torch.float64, device=dev)  # 1 GiB
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    a = P.convection_diffusion_3d(args.side)
+    m = bspai_hip.Matrix(a, ctx)
+    s = bspai_hip.construct_spai(m, 0.3, -(-a.n // 8), 8, precision="mixed")
+    b = torch.from_numpy(P.equal_rhs(a.n)).to(dev)
+    r = bspai_hip.refine_with_preconditioner(m, s, b, max_iterations=60, max_outer_steps=1,
+                                             precision="mixed", tolerance=1e-30)
+    torch.cuda.synchronize()
+    print(json.dumps({"pmc_workload": "done", "spai": s.status,
+                      "inner_iterations": r.inner_iterations}))
 
 
 def main():
@@ -49,6 +99,9 @@ def main():
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); the multi-GPU path "
               "is a design note, not a runnable configuration.", file=sys.stderr)
         sys.exit(2)
+    if args.pmc_workload:
+        pmc_workload(args)
+        return
     import numpy as np
     import torch
 
@@ -121,10 +174,14 @@ def main():
     if timed_live:
         dominant = live_rows[dominant["kernel"]]
     roofline = {k: dominant[k] for k in ("kernel", "bound", "achieved", "peak", "unit", "frac")}
-    roofline.update(traffic=None, per_launch=dominant["per_launch"], avg_launch_us=dominant["avg_us"],
+    traffic = measured_traffic(dominant["kernel"])
+    roofline.update(traffic=traffic, per_launch=dominant["per_launch"], avg_launch_us=dominant["avg_us"],
                     launches=dominant["launches"],
                     measured="HIP events over the timed region" if timed_live else
                     "HIP events, one extra solve after the timed region")
+    if traffic is not None:
+        roofline["traffic_unit"] = "bytes per launch (HBM, rocprofv3 FETCH_SIZE/WRITE_SIZE)"
+        roofline["traffic_source"] = os.path.relpath(TRAFFIC_PROFILE, REPO)
     if timed_live:
         roofline["share_of_timed_region"] = round(
             dominant["total_ms"] / (results["mixed"]["ms_per_solve"] * args.steps), 4)
@@ -244,7 +301,7 @@ def kernel_rooflines(kt):
     rows = []
     for name, v in kt.items():
         sec = v["total_ms"] * 1e-3
-        if name.startswith("spai_columns_kernel"):
+        if name.startswith(("spai_columns_kernel", "spai_wave_kernel", "spai_block_kernel")):
             prec = "float" if "<float>" in name else "double"
             ach = v["flops"] / sec / 1e12 if sec > 0 else 0.0
             bound, unit, peak = "mfma", "TFLOP/s", PEAK_TFLOPS[prec]
