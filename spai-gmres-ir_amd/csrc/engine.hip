@@ -662,6 +662,7 @@ GmresEngine<T>::GmresEngine(Context &c, const Layout &L, int cap) : c_(c), L_(L)
     args_.h = h_.data();
     args_.gram = gram_.data();
     args_.gstride = gram_size(cap);
+    args_.defer = 0;
 }
 
 template <typename T> void GmresEngine<T>::ensure_basis(int count) {
@@ -748,26 +749,15 @@ void GmresEngine<T>::solve(const DevCsr &m1, const DevCsr &m2, GmresMode mode, c
     auto spmv_bytes = [&](const DevCsr &m) {
         return (double)m.nnz * (sz + 4) + 4.0 * (double)(m.n + 1) + 2.0 * (double)m.n * sz;
     };
-    for (int k = 0; k < cap_; ++k) {
-        ensure_basis(std::min(cap_ + 1, k + 2));
-        timed(c_, l1.c_str(), spmv_bytes(m1), 2.0 * (double)m1.nnz, [&] {
-            launch_spmv<T, T, T, T>(s, m1.lanes, tl, active, stride, m1.ptr.data(),
-                                    m1.col.data(), m1v, vptrs_[k], t_.data());
-        });
-        timed(c_, l2.c_str(), spmv_bytes(m2), 2.0 * (double)m2.nnz, [&] {
-            launch_spmv<T, T, T, T>(s, m2.lanes, tl, active, stride, m2.ptr.data(),
-                                    m2.col.data(), m2v, t_.data(), w_.data());
-        });
-        timed(c_, lproj.c_str(), (k + 3) * nn * sz, 2.0 * (2 * k + 1) * nn,
-              [&] { gmres_project<T>(s, tl, args_, k); });
-        timed(c_, lred.c_str(), 2.0 * (2 * k + 1) * tl.ntiles * 8.0, 0.0,
-              [&] { gmres_gram_reduce<T>(s, tl, args_, k); });
-        timed(c_, (k <= kGramSolveLdsMaxK ? lsol_lds : lsol).c_str(), 0.0, 0.0,
-              [&] { gmres_gram_solve<T>(s, tl, args_, k); });
-        timed(c_, lupd.c_str(), (k + 3) * nn * sz, 2.0 * (k + 1) * nn,
-              [&] { gmres_gram_update<T>(s, tl, args_, k); });
-        timed(c_, lnrm.c_str(), 2.0 * nn * sz, 0.0,
-              [&] { gmres_normalize_givens<T>(s, tl, args_, k); });
+    // Step k's kernels are gated by run[k & 1] (active and not held).
+    auto run_flag = [&](int k) {
+        return reinterpret_cast<const int *>(reinterpret_cast<const char *>(st) +
+                                             offsetof(SysState, run) + (k & 1) * sizeof(int));
+    };
+    // Verification of a triggered step (src/gmres.cpp:215-253): candidate,
+    // recomputed residual(s), accept / fail / go on.  Every kernel is gated on
+    // active && triggered.
+    auto verify = [&]() {
         gmres_candidate<T>(s, tl, args_);
         if (left) {
             gmres_spmv_resid<T, T>(s, tl, st, part, m1.ptr.data(), m1.col.data(), m1v, x, rhs,
@@ -782,9 +772,50 @@ void GmresEngine<T>::solve(const DevCsr &m1, const DevCsr &m2, GmresMode mode, c
                                    ax_.data(), kSlotChkOrig, 1);
         }
         gmres_decide<T>(s, tl, args_, kPhaseCheck, !left);
+    };
+    // A single system defers verification: until its first trigger, the
+    // verification kernels are not launched after every step (they would
+    // exit at once).  A triggered step holds the system -- later launched
+    // steps do nothing -- and the host, at its next check point, runs the
+    // verification of the held step and resumes after it, so every accept
+    // decision is the one the step would have made at once.  From then on
+    // every step is verified.  Batched systems step in lockstep and verify
+    // every step.
+    bool defer = L_.nsys == 1;
+    for (int k = 0; k < cap_; ++k) {
+        ensure_basis(std::min(cap_ + 1, k + 2));
+        args_.defer = defer ? 1 : 0;
+        const int *run = run_flag(k);
+        timed(c_, l1.c_str(), spmv_bytes(m1), 2.0 * (double)m1.nnz, [&] {
+            launch_spmv<T, T, T, T>(s, m1.lanes, tl, run, stride, m1.ptr.data(), m1.col.data(),
+                                    m1v, vptrs_[k], t_.data());
+        });
+        timed(c_, l2.c_str(), spmv_bytes(m2), 2.0 * (double)m2.nnz, [&] {
+            launch_spmv<T, T, T, T>(s, m2.lanes, tl, run, stride, m2.ptr.data(), m2.col.data(),
+                                    m2v, t_.data(), w_.data());
+        });
+        timed(c_, lproj.c_str(), (k + 3) * nn * sz, 2.0 * (2 * k + 1) * nn,
+              [&] { gmres_project<T>(s, tl, args_, k); });
+        timed(c_, lred.c_str(), 2.0 * (2 * k + 1) * tl.ntiles * 8.0, 0.0,
+              [&] { gmres_gram_reduce<T>(s, tl, args_, k); });
+        timed(c_, (k <= kGramSolveLdsMaxK ? lsol_lds : lsol).c_str(), 0.0, 0.0,
+              [&] { gmres_gram_solve<T>(s, tl, args_, k); });
+        timed(c_, lupd.c_str(), (k + 3) * nn * sz, 2.0 * (k + 1) * nn,
+              [&] { gmres_gram_update<T>(s, tl, args_, k); });
+        timed(c_, lnrm.c_str(), 2.0 * nn * sz, 0.0,
+              [&] { gmres_normalize_givens<T>(s, tl, args_, k); });
+        if (!defer)
+            verify();
         if ((k + 1) % sync_every == 0 || k + 1 == cap_) {
             download(c_, host.data(), st_.data(), L_.nsys);
             sync(c_);
+            if (defer && host[0].active && host[0].triggered) {
+                verify();
+                download(c_, host.data(), st_.data(), L_.nsys);
+                sync(c_);
+                defer = false;
+                k = host[0].iterations - 1; // the loop resumes at the held step + 1
+            }
             bool any = false;
             for (const SysState &h : host)
                 any = any || h.active;

@@ -123,7 +123,7 @@ __device__ inline double reduce_max(const double *p, int t0, int t1, double *lds
 // the candidate must be formed and verified; y is then solved by
 // back_substitute.
 __device__ void givens_step(SysState &st, double *H, double *cs, double *sn, double *g, int k,
-                            int cap, double hnorm) {
+                            int cap, double hnorm, int defer) {
     double *col = H + hcol_offset(k);
     col[k + 1] = hnorm;
     st.hnorm = hnorm;
@@ -159,9 +159,11 @@ __device__ void givens_step(SysState &st, double *H, double *cs, double *sn, dou
         st.status = kGmNumericalFailure;
         st.active = 0;
         st.triggered = 0;
+        st.run[(k + 1) & 1] = 0;
         return;
     }
     st.ccount = count;
+    st.run[(k + 1) & 1] = !(st.triggered && defer);
 }
 
 // y = R^{-1} g for the leading count x count block (src/gmres.cpp:47-59), by
@@ -227,7 +229,7 @@ project_kernel(Tiles tiles, KrylovArgs<T> a, int k) {
     T *ws = reinterpret_cast<T *>(ws4), *vs = reinterpret_cast<T *>(vs4);
     const int t = blockIdx.x, s = tiles.sys[t];
     const int i0 = blockIdx.y * kProjVecs;
-    if (!a.st[s].active || i0 > k)
+    if (!a.st[s].active || !a.st[s].run[k & 1] || i0 > k)
         return;
     const int r0 = tiles.row0[t], nr = tiles.row1[t] - r0;
     const T *__restrict__ w = a.w + r0;
@@ -312,7 +314,7 @@ template <typename T>
 __global__ void __launch_bounds__(kReduceWaves *kWave)
 gram_reduce_kernel(Tiles tiles, KrylovArgs<T> a, int k) {
     const int s = blockIdx.x;
-    if (!a.st[s].active)
+    if (!a.st[s].active || !a.st[s].run[k & 1])
         return;
     const int v = blockIdx.y * kReduceWaves + (int)(threadIdx.x / kWave);
     const int lane = threadIdx.x % kWave;
@@ -346,7 +348,7 @@ __global__ void __launch_bounds__(kWave) gram_solve_kernel(Tiles tiles, KrylovAr
     __shared__ double lcs[kRotChunk], lsn[kRotChunk], lrow[kRotChunk];
     __shared__ double tile[kWave][kWave + 1]; // the block's own triangle, [i][lane]
     const int s = blockIdx.x;
-    if (!a.st[s].active)
+    if (!a.st[s].active || !a.st[s].run[k & 1])
         return;
     const int lane = threadIdx.x, cap = a.cap;
     const double *q = a.q + (std::int64_t)s * (cap + 1);
@@ -457,7 +459,7 @@ gram_solve_lds_kernel(Tiles tiles, KrylovArgs<T> a, int k) {
     __shared__ double hs[kSolveSlots * kWave], lcs[kSolveSlots * kWave],
         lsn[kSolveSlots * kWave];
     const int s = blockIdx.x;
-    if (!a.st[s].active)
+    if (!a.st[s].active || !a.st[s].run[k & 1])
         return;
     const int tid = threadIdx.x, wave = tid / kWave, lane = tid % kWave, cap = a.cap;
     const double *q = a.q + (std::int64_t)s * (cap + 1);
@@ -484,44 +486,56 @@ gram_solve_lds_kernel(Tiles tiles, KrylovArgs<T> a, int k) {
             used += k - c1;
             ++c1;
         }
-        // flat entry e of the chunk -> column i (largest with off(i) <= e),
-        // row j = i + 1 + e - off(i); all of a thread's loads issued together
+        // flat entry e of the chunk -> column i, row j = i + 1 + e - off(i).
+        // Absolute offsets from column 0, F(i) = i (2k + 1 - i) / 2, are
+        // inverted in closed form (then corrected by one step either way);
+        // all of a thread's loads are issued together.
         {
-            int src[kSolveChunk / kSolveThreads];
+            constexpr int R = kSolveChunk / kSolveThreads;
+            const long long base = (long long)c0 * (2 * k + 1 - c0) / 2;
+            const double b2 = 2.0 * k + 1.0;
+            int src[R];
 #pragma unroll
-            for (int u = 0; u < kSolveChunk / kSolveThreads; ++u) {
-                const int e = min(tid + u * kSolveThreads, used - 1);
-                int lo = c0, hi = c1 - 1;
-                while (lo < hi) {
-                    const int mid = (lo + hi + 1) >> 1;
-                    if ((mid - c0) * k - ((mid - 1) * mid / 2 - (c0 - 1) * c0 / 2) <= e)
-                        lo = mid;
-                    else
-                        hi = mid - 1;
-                }
-                const int off = (lo - c0) * k - ((lo - 1) * lo / 2 - (c0 - 1) * c0 / 2);
-                src[u] = gram_index(lo + 1 + (e - off), lo, cap);
+            for (int u = 0; u < R; ++u) {
+                const long long e = base + min(tid + u * kSolveThreads, used - 1);
+                int i = (int)floor((b2 - sqrt(b2 * b2 - 8.0 * (double)e)) * 0.5);
+                i = max(c0, min(i, c1 - 1));
+                if ((long long)i * (2 * k + 1 - i) / 2 > e)
+                    --i;
+                else if ((long long)(i + 1) * (2 * k - i) / 2 <= e)
+                    ++i;
+                const long long fi = (long long)i * (2 * k + 1 - i) / 2;
+                src[u] = gram_index(i + 1 + (int)(e - fi), i, cap);
             }
-            double v[kSolveChunk / kSolveThreads];
+            double v[R];
 #pragma unroll
-            for (int u = 0; u < kSolveChunk / kSolveThreads; ++u)
+            for (int u = 0; u < R; ++u)
                 v[u] = L[src[u]];
 #pragma unroll
-            for (int u = 0; u < kSolveChunk / kSolveThreads; ++u)
+            for (int u = 0; u < R; ++u)
                 if (tid + u * kSolveThreads < used)
                     Lc[tid + u * kSolveThreads] = v[u];
         }
         __syncthreads();
         if (wave == 0) {
-            for (int i = c0; i < c1; ++i) {
+            // L entries of step i+1 are read during step i (LDS latency off
+            // the chain)
+            auto load_step = [&](int i, double (&lv)[kSolveSlots]) {
                 const int lo = (i - c0) * k - ((i - 1) * i / 2 - (c0 - 1) * c0 / 2);
-                // this step's L entries first (they do not wait for the chain)
-                double lv[kSolveSlots];
 #pragma unroll
                 for (int uu = 0; uu < kSolveSlots; ++uu) {
                     const int j = lane + uu * kWave;
                     lv[uu] = Lc[lo + min(max(j - i - 1, 0), k - i - 1)];
                 }
+            };
+            double nx[kSolveSlots];
+            load_step(c0, nx);
+            for (int i = c0; i < c1; ++i) {
+                double lv[kSolveSlots];
+#pragma unroll
+                for (int uu = 0; uu < kSolveSlots; ++uu)
+                    lv[uu] = nx[uu];
+                load_step(min(i + 1, c1 - 1), nx);
                 const int u = i / kWave;
                 const double mine = u == 0 ? acc[0] : u == 1 ? acc[1] : u == 2 ? acc[2] : acc[3];
                 const double hi = readlane_f64(mine, i % kWave);
@@ -625,7 +639,7 @@ gram_update_kernel(Tiles tiles, KrylovArgs<T> a, int k) {
     constexpr int G = kTileRows / (kUpdateThreads * E);
     __shared__ double lds[kUpdateThreads / kWave];
     BSPAI_SYS_TILES
-    if (!a.st[s].active)
+    if (!a.st[s].active || !a.st[s].run[k & 1])
         return;
     const double *h = a.h + (std::int64_t)s * (a.cap + 1);
     const int r0 = tiles.row0[t], nr = tiles.row1[t] - r0;
@@ -709,6 +723,11 @@ normalize_givens_kernel(Tiles tiles, KrylovArgs<T> a, int k) {
     SysState &st = a.st[s];
     if (!st.active)
         return;
+    if (!st.run[k & 1]) { // held: the hold carries to the next step
+        if (writer)
+            st.run[(k + 1) & 1] = 0;
+        return;
+    }
     const double hn = reduce_norm(slot_ptr(a.part, tiles.ntiles, kSlotNorm), t0, t1, lds);
     if (hn != 0 && isfinite(hn) && k + 1 <= a.cap) {
         const T h = (T)hn;
@@ -721,7 +740,7 @@ normalize_givens_kernel(Tiles tiles, KrylovArgs<T> a, int k) {
     double *H = a.H + (std::int64_t)s * a.hstride, *g = a.g + (std::int64_t)s * (a.cap + 1);
     if (writer)
         givens_step(st, H, a.cs + (std::int64_t)s * a.cap, a.sn + (std::int64_t)s * a.cap, g, k,
-                    a.cap, hn);
+                    a.cap, hn, a.defer);
     __syncthreads();
     if (threadIdx.x < kWave && st.triggered) {
         const bool ok = back_substitute(H, g, a.y + (std::int64_t)s * a.cap, st.ccount);
@@ -889,6 +908,8 @@ decide_kernel(Tiles tiles, KrylovArgs<T> a, int phase, int right) {
             st.status = a.cap < st.nrows ? kGmIterationLimit : kGmNonconvergence;
             st.active = 0;
         }
+        if (st.active) // not accepted: the next step (a held one resumes) runs
+            st.run[st.iterations & 1] = 1;
     }
 }
 
@@ -929,6 +950,7 @@ __global__ void gmres_reset_kernel(int nsys, KrylovArgs<T> k, const IrSysState *
     st.nrows = nrows[s];
     st.participating = ir ? ir[s].active : 1;
     st.active = st.participating;
+    st.run[0] = st.run[1] = 1;
     st.init_orig = st.init_rel = st.final_orig = st.final_rel = __builtin_nan("");
     k.st[s] = st;
 }

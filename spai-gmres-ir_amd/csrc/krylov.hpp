@@ -72,6 +72,11 @@ struct SysState {
     int triggered, ccount, zero_pivot, breakdown;
     int rot_bad; // a previous rotation made column k non-finite
     int rhs_zero, x0_is_solution, has_solution;
+    // run[k & 1]: Arnoldi step k does work for this system (active and not
+    // held for a deferred verification).  Step k's kernels read run[k & 1];
+    // the step's normalize writer and the check of step k write run[(k+1) & 1],
+    // so no launch reads a flag that it also writes.
+    int run[2];
 };
 
 // Largest k solved by the LDS-staged Gram solve (beyond it, the one-wave kernel).
@@ -92,6 +97,9 @@ template <typename T> struct KrylovArgs {
     // h = MGS coefficients ([cap+1] each) and the packed Gram triangle.
     double *gp, *q, *h, *gram;
     std::int64_t gstride;
+    // 1: a triggered step holds the system (run cleared) until the host
+    // launches the verification; 0: the verification follows every step.
+    int defer;
 };
 
 struct IrSysState {
