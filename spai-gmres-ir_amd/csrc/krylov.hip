@@ -88,6 +88,7 @@ __device__ inline double reduce_dot(const double *p, int t0, int t1, double *lds
 
 __device__ inline double reduce_norm(const double *p, int t0, int t1, double *lds) {
     double emax = -1e300;
+#pragma unroll 4
     for (int t = t0 + (int)threadIdx.x; t < t1; t += blockDim.x)
         if (p[2 * t + 1] > 0)
             emax = fmax(emax, p[2 * t]);
@@ -95,6 +96,7 @@ __device__ inline double reduce_norm(const double *p, int t0, int t1, double *ld
     if (emax == -1e300)
         return 0.0;
     double s = 0;
+#pragma unroll 4
     for (int t = t0 + (int)threadIdx.x; t < t1; t += blockDim.x)
         if (p[2 * t + 1] > 0)
             s += ldexp(p[2 * t + 1], 2 * (int)(p[2 * t] - emax));
@@ -452,10 +454,58 @@ __device__ inline double readlane_f64(double v, int l) {
     return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
 
+// Steps i in [i, min(iend, 64 (B+1))) of the substitution, whose pivot h_i
+// sits in slot B (lane i - 64 B).  Column i of L starts at Lc + lo; row j's
+// entry is at Lc + lo + (j - i - 1).  Rows of slot B below or at i keep their
+// value; every row of a later slot lies below i, so no mask is needed there
+// (rows beyond k compute values that are never stored).  Lc is padded by
+// kSolvePad entries on both sides, so these reads stay in bounds.
+constexpr int kSolvePad = kSolveSlots * kWave;
+template <int B>
+__device__ inline void gram_steps(double (&acc)[kSolveSlots], const double *Lc, int &i, int &lo,
+                                  int iend, int k, int lane) {
+    const int stop = min(iend, (B + 1) * kWave);
+    if (i >= stop)
+        return;
+    auto load = [&](int ii, int l, double(&lv)[kSolveSlots]) {
+        const double *colp = Lc + kSolvePad + l - ii - 1 + lane;
+#pragma unroll
+        for (int u = B; u < kSolveSlots; ++u)
+            lv[u] = colp[u * kWave];
+    };
+    auto step = [&](int ii, const double(&lv)[kSolveSlots]) {
+        const double hi = readlane_f64(acc[B], ii - B * kWave);
+        const double y = acc[B] - lv[B] * hi;
+        acc[B] = lane + B * kWave > ii ? y : acc[B];
+#pragma unroll
+        for (int u = B + 1; u < kSolveSlots; ++u)
+            acc[u] = acc[u] - lv[u] * hi;
+    };
+    // two register sets: step i's entries were read during step i-1
+    double la[kSolveSlots], lb[kSolveSlots];
+    load(i, lo, la);
+    for (;;) {
+        int nlo = lo + (k - i);
+        if (i + 1 < stop)
+            load(i + 1, nlo, lb);
+        step(i, la);
+        lo = nlo;
+        if (++i >= stop)
+            break;
+        nlo = lo + (k - i);
+        if (i + 1 < stop)
+            load(i + 1, nlo, la);
+        step(i, lb);
+        lo = nlo;
+        if (++i >= stop)
+            break;
+    }
+}
+
 template <typename T>
 __global__ void __launch_bounds__(kSolveThreads)
 gram_solve_lds_kernel(Tiles tiles, KrylovArgs<T> a, int k) {
-    __shared__ double Lc[kSolveChunk];
+    __shared__ double Lc[kSolvePad + kSolveChunk + kSolvePad];
     __shared__ double hs[kSolveSlots * kWave], lcs[kSolveSlots * kWave],
         lsn[kSolveSlots * kWave];
     const int s = blockIdx.x;
@@ -467,18 +517,19 @@ gram_solve_lds_kernel(Tiles tiles, KrylovArgs<T> a, int k) {
     double *h = a.h + (std::int64_t)s * (cap + 1);
     double *col = a.H + (std::int64_t)s * a.hstride + hcol_offset(k);
     const double *cs = a.cs + (std::int64_t)s * cap, *sn = a.sn + (std::int64_t)s * cap;
-    for (int r = tid; r < k; r += kSolveThreads) {
-        lcs[r] = cs[r];
-        lsn[r] = sn[r];
-    }
+    // The rotations (k <= 255 < kSolveThreads: one each) and q are loaded
+    // with the first chunk of L, so the three are one round of latency.
+    const bool rot = tid < k;
+    const double ccs = rot ? cs[tid] : 0.0, csn = rot ? sn[tid] : 0.0;
     double acc[kSolveSlots];
 #pragma unroll
     for (int u = 0; u < kSolveSlots; ++u) {
         const int j = lane + u * kWave;
         acc[u] = wave == 0 && j <= k ? q[j] : 0.0;
     }
+    bool staged = false;
     // Steps i = 0..k-1 (column i of L holds rows j = i+1..k), in chunks of
-    // whole columns.  Offset of column i in a chunk from column c0:
+    // whole columns; column i of a chunk starting at c0 begins at entry
     // sum_{c0 <= i' < i} (k - i').
     for (int c0 = 0; c0 < k;) {
         int c1 = c0, used = 0;
@@ -514,38 +565,20 @@ gram_solve_lds_kernel(Tiles tiles, KrylovArgs<T> a, int k) {
 #pragma unroll
             for (int u = 0; u < R; ++u)
                 if (tid + u * kSolveThreads < used)
-                    Lc[tid + u * kSolveThreads] = v[u];
+                    Lc[kSolvePad + tid + u * kSolveThreads] = v[u];
+            if (!staged && rot) {
+                lcs[tid] = ccs;
+                lsn[tid] = csn;
+            }
+            staged = true;
         }
         __syncthreads();
         if (wave == 0) {
-            // L entries of step i+1 are read during step i (LDS latency off
-            // the chain)
-            auto load_step = [&](int i, double (&lv)[kSolveSlots]) {
-                const int lo = (i - c0) * k - ((i - 1) * i / 2 - (c0 - 1) * c0 / 2);
-#pragma unroll
-                for (int uu = 0; uu < kSolveSlots; ++uu) {
-                    const int j = lane + uu * kWave;
-                    lv[uu] = Lc[lo + min(max(j - i - 1, 0), k - i - 1)];
-                }
-            };
-            double nx[kSolveSlots];
-            load_step(c0, nx);
-            for (int i = c0; i < c1; ++i) {
-                double lv[kSolveSlots];
-#pragma unroll
-                for (int uu = 0; uu < kSolveSlots; ++uu)
-                    lv[uu] = nx[uu];
-                load_step(min(i + 1, c1 - 1), nx);
-                const int u = i / kWave;
-                const double mine = u == 0 ? acc[0] : u == 1 ? acc[1] : u == 2 ? acc[2] : acc[3];
-                const double hi = readlane_f64(mine, i % kWave);
-#pragma unroll
-                for (int uu = 0; uu < kSolveSlots; ++uu) {
-                    const int j = lane + uu * kWave;
-                    if (j > i && j <= k)
-                        acc[uu] = acc[uu] - lv[uu] * hi;
-                }
-            }
+            int i = c0, lo = 0;
+            gram_steps<0>(acc, Lc, i, lo, c1, k, lane);
+            gram_steps<1>(acc, Lc, i, lo, c1, k, lane);
+            gram_steps<2>(acc, Lc, i, lo, c1, k, lane);
+            gram_steps<3>(acc, Lc, i, lo, c1, k, lane);
         }
         __syncthreads();
         c0 = c1;
@@ -563,10 +596,11 @@ gram_solve_lds_kernel(Tiles tiles, KrylovArgs<T> a, int k) {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-    // Rotations 0..k-1 on rows 0..k (src/gmres.cpp:183-189), lane 0.
+    // Rotations 0..k-1 on rows 0..k (src/gmres.cpp:183-189), lane 0.  A
+    // non-finite intermediate carried row stays non-finite to the end, so
+    // finiteness is checked once over the rotated column afterwards.
     if (lane == 0) {
         double up = hs[0];
-        bool bad = false;
         constexpr int B = 8; // LDS reads batched ahead of the rotation chain
         for (int r0 = 0; r0 < k; r0 += B) {
             double lo[B], c[B], z[B];
@@ -583,18 +617,23 @@ gram_solve_lds_kernel(Tiles tiles, KrylovArgs<T> a, int k) {
                     const double top = c[u] * up + z[u] * lo[u];
                     up = -z[u] * up + c[u] * lo[u];
                     hs[r0 + u] = top;
-                    bad = bad || !isfinite(top) || !isfinite(up);
                 }
             }
         }
         hs[k] = up;
-        a.st[s].rot_bad = bad;
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-    for (int r = lane; r <= k; r += kWave)
-        col[r] = hs[r];
+    bool bad = false;
+    for (int r = lane; r <= k; r += kWave) {
+        const double v = hs[r];
+        col[r] = v;
+        bad = bad || !isfinite(v);
+    }
+    bad = __any(bad) && k > 0;
+    if (lane == 0)
+        a.st[s].rot_bad = bad;
 }
 
 // Pass 2: w <- w - sum_i h_i V_i (i ascending, in T as the reference's
@@ -730,10 +769,33 @@ normalize_givens_kernel(Tiles tiles, KrylovArgs<T> a, int k) {
     }
     const double hn = reduce_norm(slot_ptr(a.part, tiles.ntiles, kSlotNorm), t0, t1, lds);
     if (hn != 0 && isfinite(hn) && k + 1 <= a.cap) {
+        // V_{k+1} = w / h, a full aligned tile in 16-byte accesses (all of a
+        // thread's loads issued together), other tiles element by element
+        using V16 = typename Vec16<T>::type;
+        constexpr int E = 16 / sizeof(T);
+        constexpr int G = kTileRows / (kTileThreads * E);
         const T h = (T)hn;
-        T *v = a.V[k + 1];
-        for (int r = tiles.row0[t] + threadIdx.x; r < tiles.row1[t]; r += kTileThreads)
-            v[r] = a.w[r] / h;
+        const int r0 = tiles.row0[t], nr = tiles.row1[t] - r0;
+        T *v = a.V[k + 1] + r0;
+        const T *w = a.w + r0;
+        if (nr == kTileRows && r0 % E == 0) {
+            V16 x[G];
+#pragma unroll
+            for (int g = 0; g < G; ++g)
+                x[g] = reinterpret_cast<const V16 *>(w)[g * kTileThreads + threadIdx.x];
+#pragma unroll
+            for (int g = 0; g < G; ++g) {
+                T *xp = reinterpret_cast<T *>(&x[g]);
+#pragma unroll
+                for (int e = 0; e < E; ++e)
+                    xp[e] = xp[e] / h;
+                reinterpret_cast<V16 *>(v)[g * kTileThreads + threadIdx.x] = x[g];
+            }
+        } else {
+#pragma unroll 8
+            for (int r = threadIdx.x; r < nr; r += kTileThreads)
+                v[r] = w[r] / h;
+        }
     }
     if (t != t0)
         return;
