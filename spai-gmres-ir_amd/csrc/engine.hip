@@ -713,7 +713,7 @@ void GmresEngine<T>::solve(const DevCsr &m1, const DevCsr &m2, GmresMode mode, c
     const T *prhs = rhs;
     if (left) {
         launch_spmv<T, T, T, T>(s, m2.lanes, tl, active, stride, m2.ptr.data(), m2.col.data(),
-                                m2v, rhs, prhs_.data());
+                                m2v, rhs, prhs_.data(), stream_per(m2.n, m2.nnz));
         prhs = prhs_.data();
     }
     gmres_diff_partial<T, T, T>(s, tl, st, part, prhs, none, (T *)nullptr, kSlotDen);
@@ -788,11 +788,11 @@ void GmresEngine<T>::solve(const DevCsr &m1, const DevCsr &m2, GmresMode mode, c
         const int *run = run_flag(k);
         timed(c_, l1.c_str(), spmv_bytes(m1), 2.0 * (double)m1.nnz, [&] {
             launch_spmv<T, T, T, T>(s, m1.lanes, tl, run, stride, m1.ptr.data(), m1.col.data(),
-                                    m1v, vptrs_[k], t_.data());
+                                    m1v, vptrs_[k], t_.data(), stream_per(m1.n, m1.nnz));
         });
         timed(c_, l2.c_str(), spmv_bytes(m2), 2.0 * (double)m2.nnz, [&] {
             launch_spmv<T, T, T, T>(s, m2.lanes, tl, run, stride, m2.ptr.data(), m2.col.data(),
-                                    m2v, t_.data(), w_.data());
+                                    m2v, t_.data(), w_.data(), stream_per(m2.n, m2.nnz));
         });
         timed(c_, lproj.c_str(), (k + 3) * nn * sz, 2.0 * (2 * k + 1) * nn,
               [&] { gmres_project<T>(s, tl, args_, k); });

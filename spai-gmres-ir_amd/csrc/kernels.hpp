@@ -103,9 +103,16 @@ constexpr int kSpmvSub = kTileRows / kSpmvRows;
 // number of lanes cooperating on one row (1 = sequential CSR order, which is
 // bit-identical to the reference's CsrMatrix::multiply, src/cpu.cpp:80-94).
 // flag (optional): a tile runs only if flag[sys * flag_stride] != 0.
+// stream_per (lanes = 1): entries per thread per chunk, 8 or 16 (for matrices
+// averaging more than 8 entries per row); the result does not depend on it.
 template <typename TV, typename TX, typename TA, typename TY>
 void launch_spmv(hipStream_t s, int lanes, const Tiles &tiles, const int *flag, int flag_stride,
-                 const int *ptr, const int *col, const TV *val, const TX *x, TY *y);
+                 const int *ptr, const int *col, const TV *val, const TX *x, TY *y,
+                 int stream_per = 8);
+// Entries per thread for the streaming SpMV of an n x n matrix with nnz entries.
+inline int stream_per(long long n, long long nnz) {
+    return nnz > 8 * n ? 16 : 8;
+}
 
 template <typename TS, typename TD>
 void launch_convert(hipStream_t s, std::int64_t n, const TS *src, TD *dst);

@@ -52,12 +52,14 @@ spmv_group_kernel(Tiles tiles, const int *__restrict__ flag, int flag_stride,
 // range of (val, col); all threads load it with unit stride, form the products
 // in LDS, then each thread adds its row's products in CSR order -- the
 // reference's sequential row sum (src/cpu.cpp:88-89), bit for bit.
-constexpr int kStreamChunk = 2048;
-template <typename TV, typename TX, typename TA, typename TY>
+// PER entries per thread per chunk: 8 for rows of up to ~8 entries, 16 for
+// longer ones, so that a block's rows usually fit one chunk.
+template <typename TV, typename TX, typename TA, typename TY, int PER>
 __global__ void __launch_bounds__(kSpmvRows)
 spmv_stream_kernel(Tiles tiles, const int *__restrict__ flag, int flag_stride,
                    const int *__restrict__ ptr, const int *__restrict__ col,
                    const TV *__restrict__ val, const TX *__restrict__ x, TY *__restrict__ y) {
+    constexpr int kStreamChunk = kSpmvRows * PER;
     __shared__ int rp[kSpmvRows + 1];
     __shared__ TA prod[kStreamChunk];
     const int b = xcd_block(blockIdx.x, gridDim.x);
@@ -76,7 +78,7 @@ spmv_stream_kernel(Tiles tiles, const int *__restrict__ flag, int flag_stride,
     const int tid = threadIdx.x;
     const int mb = tid < nr ? rp[tid] : 0, me = tid < nr ? rp[tid + 1] : 0;
     TA acc = 0;
-    constexpr int per = kStreamChunk / kSpmvRows;
+    constexpr int per = PER;
     for (int c0 = e0; c0 < e1; c0 += kStreamChunk) {
         const int c1 = min(e1, c0 + kStreamChunk);
         // Unconditional loads (indices clamped into the chunk) so that all of
@@ -111,7 +113,8 @@ spmv_stream_kernel(Tiles tiles, const int *__restrict__ flag, int flag_stride,
 
 template <typename TV, typename TX, typename TA, typename TY>
 void launch_spmv(hipStream_t s, int lanes, const Tiles &tiles, const int *flag, int flag_stride,
-                 const int *ptr, const int *col, const TV *val, const TX *x, TY *y) {
+                 const int *ptr, const int *col, const TV *val, const TX *x, TY *y,
+                 int stream_per) {
     if (tiles.ntiles == 0)
         return;
     const dim3 grid((unsigned)tiles.ntiles * kSpmvSub), block(kSpmvRows);
@@ -122,8 +125,12 @@ void launch_spmv(hipStream_t s, int lanes, const Tiles &tiles, const int *flag, 
         break;
     switch (lanes) {
     case 1:
-        hipLaunchKernelGGL((spmv_stream_kernel<TV, TX, TA, TY>), grid, block, 0, s, tiles, flag,
-                           flag_stride, ptr, col, val, x, y);
+        if (stream_per > 8)
+            hipLaunchKernelGGL((spmv_stream_kernel<TV, TX, TA, TY, 16>), grid, block, 0, s, tiles,
+                               flag, flag_stride, ptr, col, val, x, y);
+        else
+            hipLaunchKernelGGL((spmv_stream_kernel<TV, TX, TA, TY, 8>), grid, block, 0, s, tiles,
+                               flag, flag_stride, ptr, col, val, x, y);
         break;
     case -1:
         hipLaunchKernelGGL((spmv_group_kernel<1, TV, TX, TA, TY>), grid, block, 0, s, tiles, flag,
@@ -145,7 +152,7 @@ void launch_spmv(hipStream_t s, int lanes, const Tiles &tiles, const int *flag, 
 #define BSPAI_SPMV_INST(TV, TX, TA, TY)                                                           \
     template void launch_spmv<TV, TX, TA, TY>(hipStream_t, int, const Tiles &, const int *, int,  \
                                               const int *, const int *, const TV *, const TX *,   \
-                                              TY *);
+                                              TY *, int);
 BSPAI_SPMV_INST(double, double, double, double)
 BSPAI_SPMV_INST(float, float, float, float)
 BSPAI_SPMV_INST(float, double, double, double)
