@@ -84,7 +84,7 @@ const std::vector<KernelTimer::Total> &KernelTimer::collect() {
 
 std::string spmv_label(int lanes, const char *tname, const char *role) {
     std::string k = lanes == 1 ? std::string("spmv_stream_kernel")
-                               : "spmv_group_kernel<" + std::to_string(lanes) + ">";
+                               : "spmv_group_kernel<" + std::to_string(lanes < 0 ? 1 : lanes) + ">";
     return k + "<" + tname + "> " + role;
 }
 
@@ -137,6 +137,59 @@ void DevCsr::ensure32(const Context &c) {
     val32.resize(nnz);
     launch_convert<double, float>(c.stream, nnz, val.data(), val32.data());
     has32 = true;
+}
+
+// SpMV autotuning.  For a matrix whose rows are short enough for the
+// sequential row order (lanes = 1), the candidates are the streaming kernel
+// with 8 or 16 entries per thread and one thread per row (lanes = -1): all
+// three add each row's products in CSR order, so the choice changes timing
+// only, never a result bit.  Each candidate runs once untimed and then three
+// times between HIP events on the context's stream; the fastest is kept per
+// matrix shape (n, nnz, value type), so a rebuilt preconditioner of the same
+// shape reuses it.  Long-row matrices keep their lane-group choice.
+// BSPAI_SPMV_AUTOTUNE=0 turns tuning off (the row-length rule decides).
+template <typename T> SpmvChoice spmv_choice(Context &c, const DevCsr &m) {
+    SpmvChoice rule{m.lanes, stream_per(m.n, m.nnz)};
+    if (m.lanes != 1 || m.n == 0)
+        return rule;
+    if (const char *env = std::getenv("BSPAI_SPMV_AUTOTUNE"))
+        if (std::string(env) == "0")
+            return rule;
+    const auto key = std::make_tuple(m.n, m.nnz, (int)sizeof(T));
+    const auto it = c.spmv_tuned.find(key);
+    if (it != c.spmv_tuned.end())
+        return it->second;
+    const SpmvChoice cand[3] = {{1, 8}, {1, 16}, {-1, 8}};
+    DeviceArray<T> x((std::size_t)m.n), y((std::size_t)m.n);
+    HIP_CHECK(hipMemsetAsync(x.data(), 0, (std::size_t)m.n * sizeof(T), c.stream));
+    hipEvent_t e0, e1;
+    HIP_CHECK(hipEventCreate(&e0));
+    HIP_CHECK(hipEventCreate(&e1));
+    const Tiles tl = m.layout->tiles();
+    SpmvChoice best = rule;
+    float best_ms = 0;
+    for (const SpmvChoice &k : cand) {
+        auto run = [&] {
+            launch_spmv<T, T, T, T>(c.stream, k.lanes, tl, nullptr, 0, m.ptr.data(),
+                                    m.col.data(), m.values<T>(), x.data(), y.data(), k.per);
+        };
+        run();
+        HIP_CHECK(hipEventRecord(e0, c.stream));
+        for (int r = 0; r < 3; ++r)
+            run();
+        HIP_CHECK(hipEventRecord(e1, c.stream));
+        HIP_CHECK(hipEventSynchronize(e1));
+        float ms = 0;
+        HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+        if (&k == cand || ms < best_ms) {
+            best_ms = ms;
+            best = k;
+        }
+    }
+    HIP_CHECK(hipEventDestroy(e0));
+    HIP_CHECK(hipEventDestroy(e1));
+    c.spmv_tuned[key] = best;
+    return best;
 }
 
 void spmv_any(Context &c, const DevCsr &a, const double *x, double *y, int lanes) {
@@ -701,6 +754,7 @@ void GmresEngine<T>::solve(const DevCsr &m1, const DevCsr &m2, GmresMode mode, c
                                                     offsetof(SysState, triggered));
     const int stride = (int)(sizeof(SysState) / sizeof(int));
     const T *m1v = m1.values<T>(), *m2v = m2.values<T>();
+    const SpmvChoice k1 = spmv_choice<T>(c_, m1), k2 = spmv_choice<T>(c_, m2);
     const T *none = nullptr;
     // ---- start (src/gmres.cpp:134-167)
     gmres_diff_partial<T, T, T>(s, tl, st, part, rhs, none, (T *)nullptr, kSlotRhs);
@@ -712,8 +766,8 @@ void GmresEngine<T>::solve(const DevCsr &m1, const DevCsr &m2, GmresMode mode, c
     gmres_decide<T>(s, tl, args_, kPhaseInit, !left);
     const T *prhs = rhs;
     if (left) {
-        launch_spmv<T, T, T, T>(s, m2.lanes, tl, active, stride, m2.ptr.data(), m2.col.data(),
-                                m2v, rhs, prhs_.data(), stream_per(m2.n, m2.nnz));
+        launch_spmv<T, T, T, T>(s, k2.lanes, tl, active, stride, m2.ptr.data(), m2.col.data(),
+                                m2v, rhs, prhs_.data(), k2.per);
         prhs = prhs_.data();
     }
     gmres_diff_partial<T, T, T>(s, tl, st, part, prhs, none, (T *)nullptr, kSlotDen);
@@ -738,7 +792,7 @@ void GmresEngine<T>::solve(const DevCsr &m1, const DevCsr &m2, GmresMode mode, c
     // and out (update).
     const char *tname = sizeof(T) == 8 ? "double" : "float";
     const char *role1 = left ? "A" : "P^T", *role2 = left ? "P" : "A^T";
-    const std::string l1 = spmv_label(m1.lanes, tname, role1), l2 = spmv_label(m2.lanes, tname, role2);
+    const std::string l1 = spmv_label(k1.lanes, tname, role1), l2 = spmv_label(k2.lanes, tname, role2);
     const std::string lproj = std::string("project_kernel<") + tname + ">";
     const std::string lred = std::string("gram_reduce_kernel<") + tname + ">";
     const std::string lsol = std::string("gram_solve_kernel<") + tname + ">";
@@ -787,12 +841,12 @@ void GmresEngine<T>::solve(const DevCsr &m1, const DevCsr &m2, GmresMode mode, c
         args_.defer = defer ? 1 : 0;
         const int *run = run_flag(k);
         timed(c_, l1.c_str(), spmv_bytes(m1), 2.0 * (double)m1.nnz, [&] {
-            launch_spmv<T, T, T, T>(s, m1.lanes, tl, run, stride, m1.ptr.data(), m1.col.data(),
-                                    m1v, vptrs_[k], t_.data(), stream_per(m1.n, m1.nnz));
+            launch_spmv<T, T, T, T>(s, k1.lanes, tl, run, stride, m1.ptr.data(), m1.col.data(),
+                                    m1v, vptrs_[k], t_.data(), k1.per);
         });
         timed(c_, l2.c_str(), spmv_bytes(m2), 2.0 * (double)m2.nnz, [&] {
-            launch_spmv<T, T, T, T>(s, m2.lanes, tl, run, stride, m2.ptr.data(), m2.col.data(),
-                                    m2v, t_.data(), w_.data(), stream_per(m2.n, m2.nnz));
+            launch_spmv<T, T, T, T>(s, k2.lanes, tl, run, stride, m2.ptr.data(), m2.col.data(),
+                                    m2v, t_.data(), w_.data(), k2.per);
         });
         timed(c_, lproj.c_str(), (k + 3) * nn * sz, 2.0 * (2 * k + 1) * nn,
               [&] { gmres_project<T>(s, tl, args_, k); });

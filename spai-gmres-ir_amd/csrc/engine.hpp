@@ -5,7 +5,9 @@
 #include "kernels.hpp"
 #include "krylov.hpp"
 
+#include <map>
 #include <memory>
+#include <tuple>
 #include <string>
 #include <vector>
 
@@ -113,6 +115,12 @@ class KernelTimer {
     int used_ = 0;
 };
 
+// SpMV launch configuration (lanes per row; entries per thread of the
+// streaming kernel).
+struct SpmvChoice {
+    int lanes = 1, per = 8;
+};
+
 struct Context {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -121,6 +129,8 @@ struct Context {
     // timing events
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     KernelTimer timer;
+    // SpMV autotuning results by matrix shape (n, nnz, value bytes)
+    std::map<std::tuple<std::int64_t, std::int64_t, int>, SpmvChoice> spmv_tuned;
 };
 
 // Runs `launch` (one kernel launch on c.stream), timed when the context's

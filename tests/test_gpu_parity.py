@@ -418,6 +418,32 @@ def test_many_systems_batched_equals_looped(ctx):
         assert np.array_equal(bits(x[o0:o1]), bits(np.asarray(single.solution)))
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_spmv_autotune_keeps_bits(monkeypatch, precision):
+    """The SpMV autotuner picks among order-preserving kernels only: an IR
+    solve with tuning on returns the same bits and iteration counts as with
+    tuning off (fresh contexts, 32^3 convection-diffusion)."""
+    import bspai_hip
+
+    P = _problems()
+    a = P.convection_diffusion_3d(32)
+    b = P.equal_rhs(a.n)
+    kw = dict(precision=precision, tolerance=1e-8 if precision == "fp64" else 2e-5,
+              max_iterations=200, relres_target=1e-12)
+    out = {}
+    for mode in ("0", "1"):
+        monkeypatch.setenv("BSPAI_SPMV_AUTOTUNE", mode)
+        c = bspai_hip.Context(0)
+        r = bspai_hip.spai_gmres_ir(bspai_hip.Matrix(a, c), b, **kw)
+        r2 = bspai_hip.spai_gmres_ir(bspai_hip.Matrix(a, c), b, **kw)  # cached choice
+        assert r.status == r2.status == "success"
+        assert r.inner_iterations == r2.inner_iterations
+        assert np.array_equal(bits(np.asarray(r.solution)), bits(np.asarray(r2.solution)))
+        out[mode] = r
+    assert out["0"].inner_iterations == out["1"].inner_iterations
+    assert np.array_equal(bits(np.asarray(out["0"].solution)), bits(np.asarray(out["1"].solution)))
+
+
 def test_gmres_second_inner_solve_same_input(ctx):
     """The second inner solve of the 64x64 IR, given the oracle's own residual
     r1 = b - A x1: GPU and oracle iteration counts agree within 1."""
