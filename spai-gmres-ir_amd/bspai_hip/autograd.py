@@ -18,19 +18,36 @@ import torch
 from . import api
 
 
+_SPAI_KEYS = ("epsilon", "alpha", "beta", "precision")
+
+
+def _spai_options(opts: dict, n: int) -> dict:
+    """SPAI parameters of a solve (alpha 0 = ceil(n/8), as the reference CLI)."""
+    return dict(epsilon=opts.get("epsilon", 0.3), alpha=opts.get("alpha", 0) or -(-n // 8),
+                beta=opts.get("beta", 8), precision=opts.get("precision", "fp64"))
+
+
 class SparseSolve(torch.autograd.Function):
     @staticmethod
     def forward(ctx, values, b, handle, opts):
         m = handle.matrix
         m.set_values(values.detach().contiguous())
         fwd = {k: v for k, v in opts.items() if k != "adjoint"}
-        res = api.spai_gmres_ir(m, b.detach().contiguous(), **fwd)
+        # construct_spai + refine_with_preconditioner is the reference's
+        # spai_gmres_ir (src/ir.cpp:206-227); P is kept for the backward pass.
+        sk = _spai_options(fwd, m.n)
+        spai = api.construct_spai(m, **sk)
+        if spai.status != "success":
+            raise RuntimeError(f"forward SPAI failed: {spai.status}: {spai.message}")
+        ir = {k: v for k, v in fwd.items() if k not in ("epsilon", "alpha", "beta")}
+        res = api.refine_with_preconditioner(m, spai, b.detach().contiguous(), **ir)
         if res.status != "success":
             raise RuntimeError(f"forward solve failed: {res.status}: {res.message}")
         handle.last_forward = res
         x = res.solution
         ctx.handle = handle
         ctx.opts = opts
+        ctx.spai = (spai, sk)
         ctx.save_for_backward(x)
         return x
 
@@ -41,12 +58,11 @@ class SparseSolve(torch.autograd.Function):
         m = h.matrix
         opts = {k: v for k, v in ctx.opts.items() if k != "adjoint"}
         opts.update(ctx.opts.get("adjoint") or {})
-        # P of the current A (the forward's construction is rebuilt here so the
-        # transposed preconditioner matches the values used in forward).
-        spai = api.construct_spai(m, epsilon=opts.get("epsilon", 0.3),
-                                  alpha=opts.get("alpha", 0) or -(-m.n // 8),
-                                  beta=opts.get("beta", 8),
-                                  precision=opts.get("precision", "fp64"))
+        # P of the values used in forward: the forward's construction when the
+        # adjoint asks for the same SPAI parameters, else a new one.
+        spai, sk = ctx.spai
+        if _spai_options(opts, m.n) != sk:
+            spai = api.construct_spai(m, **_spai_options(opts, m.n))
         if spai.status != "success":
             raise RuntimeError(f"backward SPAI failed: {spai.status}")
         at = m.transpose()

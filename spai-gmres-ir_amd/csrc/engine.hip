@@ -724,7 +724,7 @@ template <typename T> void GmresEngine<T>::ensure_basis(int count) {
     if (count <= vcount_)
         return;
     // Vector stride padded to 64 elements: every basis vector starts 256-byte
-    // aligned, which the 2-wide loads of the Gram-Schmidt passes rely on.
+    // aligned, which the 16-byte accesses of the Gram-Schmidt passes rely on.
     const std::size_t ld = (std::size_t)((L_.n + 63) / 64 * 64);
     while ((int)vptrs_.size() < count) {
         const int take = std::min(chunk, cap_ + 1 - (int)vptrs_.size());
@@ -835,7 +835,10 @@ void GmresEngine<T>::solve(const DevCsr &m1, const DevCsr &m2, GmresMode mode, c
     // decision is the one the step would have made at once.  From then on
     // every step is verified.  Batched systems step in lockstep and verify
     // every step.
-    bool defer = L_.nsys == 1;
+    // (BSPAI_GMRES_VERIFY_EVERY_STEP=1 keeps per-step verification, so the
+    // two schedules can be compared.)
+    const char *every = std::getenv("BSPAI_GMRES_VERIFY_EVERY_STEP");
+    bool defer = L_.nsys == 1 && !(every && std::string(every) == "1");
     for (int k = 0; k < cap_; ++k) {
         ensure_basis(std::min(cap_ + 1, k + 2));
         args_.defer = defer ? 1 : 0;

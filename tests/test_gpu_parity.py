@@ -444,6 +444,37 @@ def test_spmv_autotune_keeps_bits(monkeypatch, precision):
     assert np.array_equal(bits(np.asarray(out["0"].solution)), bits(np.asarray(out["1"].solution)))
 
 
+@pytest.mark.parametrize("case", ["c1_fp64", "c1_mixed_floor", "cube32_mixed", "c1_mixed_cap"])
+def test_deferred_verification_same_as_every_step(monkeypatch, case):
+    """A single system verifies triggered steps at the host's check points
+    (the held step is verified, then the loop resumes after it).  Against
+    per-step verification: same statuses, inner iteration counts, outer steps
+    and solution bits -- including mixed solves whose tolerance sits at the
+    FP32 floor, where triggered steps fail verification and the loop resumes,
+    and a cap that ends the inner solves."""
+    import bspai_hip
+
+    P = _problems()
+    a = P.convection_diffusion_3d(32) if case == "cube32_mixed" else P.convection_diffusion_2d()
+    b = P.equal_rhs(a.n)
+    kw = {"c1_fp64": dict(precision="fp64", tolerance=1e-8),
+          "c1_mixed_floor": dict(precision="mixed", tolerance=2e-7, max_iterations=300),
+          "cube32_mixed": dict(precision="mixed", tolerance=2e-5),
+          "c1_mixed_cap": dict(precision="mixed", tolerance=1e-6, max_iterations=37,
+                               max_outer_steps=4)}[case]
+    out = {}
+    for mode in ("0", "1"):
+        monkeypatch.setenv("BSPAI_GMRES_VERIFY_EVERY_STEP", mode)
+        c = bspai_hip.Context(0)
+        out[mode] = bspai_hip.spai_gmres_ir(bspai_hip.Matrix(a, c), b, **kw)
+    d, e = out["0"], out["1"]
+    print(case, d.status, d.outer_steps, d.inner_iterations, d.inner_statuses)
+    assert d.status == e.status and d.outer_steps == e.outer_steps
+    assert d.inner_iterations == e.inner_iterations
+    assert d.inner_statuses == e.inner_statuses
+    assert np.array_equal(bits(np.asarray(d.solution)), bits(np.asarray(e.solution)))
+
+
 def test_gmres_second_inner_solve_same_input(ctx):
     """The second inner solve of the 64x64 IR, given the oracle's own residual
     r1 = b - A x1: GPU and oracle iteration counts agree within 1."""
