@@ -685,37 +685,61 @@ GmresEngine<T>::GmresEngine(Context &c, const Layout &L, int cap) : c_(c), L_(L)
     xz_.resize(n);
     ax_.resize(n);
     pax_.resize(n);
-    const std::int64_t hs = hessenberg_size(cap);
-    H_.resize((std::size_t)hs * L.nsys);
     cs_.resize((std::size_t)cap * L.nsys);
     sn_.resize((std::size_t)cap * L.nsys);
     g_.resize((std::size_t)(cap + 1) * L.nsys);
     y_.resize((std::size_t)cap * L.nsys);
     part_.resize((std::size_t)kNumSlots * 2 * std::max(1, L.ntiles));
-    gp_.resize((std::size_t)2 * (cap + 1) * std::max(1, L.ntiles));
     q_.resize((std::size_t)(cap + 1) * L.nsys);
     h_.resize((std::size_t)(cap + 1) * L.nsys);
-    gram_.resize((std::size_t)gram_size(cap) * L.nsys);
     st_.resize(L.nsys);
     vdev_.resize(cap + 1);
     vptrs_.reserve(cap + 1);
     args_.st = st_.data();
     args_.V = vdev_.data();
     args_.w = w_.data();
-    args_.H = H_.data();
     args_.cs = cs_.data();
     args_.sn = sn_.data();
     args_.g = g_.data();
     args_.y = y_.data();
-    args_.hstride = hs;
     args_.cap = cap;
     args_.part = part_.data();
-    args_.gp = gp_.data();
     args_.q = q_.data();
     args_.h = h_.data();
-    args_.gram = gram_.data();
-    args_.gstride = gram_size(cap);
     args_.defer = 0;
+    ensure_steps(0);
+}
+
+template <typename T> void GmresEngine<T>::ensure_steps(int k) {
+    // H and the Gram triangle are O(steps^2) per system: a cap of n (the
+    // reference CLI's innercap 0) must not allocate n^2 up front, so they
+    // start at 64 steps and double as the Krylov space grows.
+    if (k < ksteps_)
+        return;
+    const int K = std::min(cap_, std::max({k + 1, 2 * ksteps_, 64}));
+    const std::size_t nsys = (std::size_t)L_.nsys;
+    const std::int64_t hs = hessenberg_size(K), gs = gram_size(K);
+    DeviceArray<double> H2((std::size_t)hs * nsys), G2((std::size_t)gs * nsys);
+    if (ksteps_ > 0) {
+        const std::int64_t hs0 = hessenberg_size(ksteps_), gs0 = gram_size(ksteps_);
+        HIP_CHECK(hipMemcpy2DAsync(H2.data(), hs * sizeof(double), H_.data(),
+                                   hs0 * sizeof(double), hs0 * sizeof(double), nsys,
+                                   hipMemcpyDeviceToDevice, c_.stream));
+        HIP_CHECK(hipMemcpy2DAsync(G2.data(), gs * sizeof(double), gram_.data(),
+                                   gs0 * sizeof(double), gs0 * sizeof(double), nsys,
+                                   hipMemcpyDeviceToDevice, c_.stream));
+        sync(c_); // the old buffers are released below
+    }
+    H_ = std::move(H2);
+    gram_ = std::move(G2);
+    // per-tile partials of one step (2k+2 values): no contents to keep
+    gp_.resize((std::size_t)2 * K * std::max(1, L_.ntiles));
+    args_.H = H_.data();
+    args_.hstride = hs;
+    args_.gram = gram_.data();
+    args_.gstride = gs;
+    args_.gp = gp_.data();
+    ksteps_ = K;
 }
 
 template <typename T> void GmresEngine<T>::ensure_basis(int count) {
@@ -841,6 +865,7 @@ void GmresEngine<T>::solve(const DevCsr &m1, const DevCsr &m2, GmresMode mode, c
     bool defer = L_.nsys == 1 && !(every && std::string(every) == "1");
     for (int k = 0; k < cap_; ++k) {
         ensure_basis(std::min(cap_ + 1, k + 2));
+        ensure_steps(k);
         args_.defer = defer ? 1 : 0;
         const int *run = run_flag(k);
         timed(c_, l1.c_str(), spmv_bytes(m1), 2.0 * (double)m1.nnz, [&] {

@@ -225,6 +225,9 @@ template <typename T> class GmresEngine {
 
   private:
     void ensure_basis(int count);
+    // Capacity of H, the Gram triangle and the projection partials for
+    // Krylov steps 0..k (grown by doubling, contents kept).
+    void ensure_steps(int k);
     Context &c_;
     const Layout &L_;
     int cap_;
@@ -232,6 +235,7 @@ template <typename T> class GmresEngine {
     std::vector<T *> vptrs_;
     DeviceArray<T *> vdev_;
     int vcount_ = 0;
+    int ksteps_ = 0; // Krylov steps H, gram_ and gp_ hold
     DeviceArray<T> w_, t_, prhs_, r0_;
     DeviceArray<double> xz_, ax_, pax_, ax0_, tmpd_;
     DeviceArray<double> H_, cs_, sn_, g_, y_, part_, gp_, q_, h_, gram_;

@@ -296,14 +296,14 @@ project_kernel(Tiles tiles, KrylovArgs<T> a, int k) {
         dv[u] = wave_sum(dv[u]);
     }
     if (lane == 0) {
-        const std::int64_t nt = tiles.ntiles, off = a.cap + 1;
+        const std::int64_t nt = tiles.ntiles;
 #pragma unroll
         for (int u = 0; u < P; ++u) {
             const int i = ib + u;
             if (i <= k)
-                a.gp[i * nt + t] = dw[u];
+                a.gp[(2 * (std::int64_t)i) * nt + t] = dw[u];
             if (i < k)
-                a.gp[(off + i) * nt + t] = dv[u];
+                a.gp[(2 * (std::int64_t)i + 1) * nt + t] = dv[u];
         }
     }
 }
@@ -323,7 +323,7 @@ gram_reduce_kernel(Tiles tiles, KrylovArgs<T> a, int k) {
     if (v > 2 * k)
         return;
     const std::int64_t nt = tiles.ntiles;
-    const double *row = a.gp + (v <= k ? v : a.cap + 1 + (v - k - 1)) * nt;
+    const double *row = a.gp + (v <= k ? 2 * (std::int64_t)v : 2 * (std::int64_t)(v - k - 1) + 1) * nt;
     double sum = 0;
 #pragma unroll 8
     for (int t = tiles.sys_tile0[s] + lane; t < tiles.sys_tile1[s]; t += kWave)
@@ -333,7 +333,7 @@ gram_reduce_kernel(Tiles tiles, KrylovArgs<T> a, int k) {
         if (v <= k)
             a.q[(std::int64_t)s * (a.cap + 1) + v] = sum;
         else
-            a.gram[(std::int64_t)s * a.gstride + gram_index(k, v - k - 1, a.cap)] = sum;
+            a.gram[(std::int64_t)s * a.gstride + gram_index(k, v - k - 1)] = sum;
     }
 }
 
@@ -368,7 +368,7 @@ __global__ void __launch_bounds__(kWave) gram_solve_kernel(Tiles tiles, KrylovAr
 #pragma unroll
             for (int u = 0; u < kLoadBatch; ++u) {
                 const int i = i0 + u;
-                v[u] = i <= last && mine && j > i ? L[gram_index(j, i, cap)] : 0.0;
+                v[u] = i <= last && mine && j > i ? L[gram_index(j, i)] : 0.0;
             }
 #pragma unroll
             for (int u = 0; u < kLoadBatch; ++u)
@@ -380,7 +380,7 @@ __global__ void __launch_bounds__(kWave) gram_solve_kernel(Tiles tiles, KrylovAr
             double lv[kLoadBatch], hv[kLoadBatch];
 #pragma unroll
             for (int u = 0; u < kLoadBatch; ++u) {
-                lv[u] = mine ? L[gram_index(j, i0 + u, cap)] : 0.0;
+                lv[u] = mine ? L[gram_index(j, i0 + u)] : 0.0;
                 hv[u] = h[i0 + u];
             }
 #pragma unroll
@@ -437,9 +437,9 @@ __global__ void __launch_bounds__(kWave) gram_solve_kernel(Tiles tiles, KrylovAr
 }
 
 // The same solve for k < 256 with a 1024-thread block: all waves stage
-// column chunks of L in LDS (coalesced, every load of a chunk in flight at
-// once), then wave 0 runs the substitution from LDS, unknown j in lane
-// j % 64, slot j / 64, each pivot broadcast by readlane.  Every h_j again
+// column chunks of L in LDS (every load of a chunk in flight at once), then
+// wave 0 runs the substitution from LDS, unknown j in lane j % 64, slot
+// j / 64, each pivot broadcast by readlane.  Every h_j again
 // sees its terms in the order i = 0, 1, ..., j-1 (bit-identical to the
 // kernel above); the rotations follow from LDS.
 constexpr int kSolveThreads = 1024;
@@ -556,7 +556,7 @@ gram_solve_lds_kernel(Tiles tiles, KrylovArgs<T> a, int k) {
                 else if ((long long)(i + 1) * (2 * k - i) / 2 <= e)
                     ++i;
                 const long long fi = (long long)i * (2 * k + 1 - i) / 2;
-                src[u] = gram_index(i + 1 + (int)(e - fi), i, cap);
+                src[u] = gram_index(i + 1 + (int)(e - fi), i);
             }
             double v[R];
 #pragma unroll

@@ -50,17 +50,15 @@ __host__ __device__ inline std::int64_t hcol_offset(int k) {
 inline std::int64_t hessenberg_size(int cap) {
     return hcol_offset(cap) + 2;
 }
-// Strictly lower Gram triangle L(j, i) = <V_j, V_i>, i < j <= cap, packed by
-// columns: column i holds rows i+1..cap, so the lanes of a wave that own
-// consecutive rows j read consecutive words.
-__host__ __device__ inline std::int64_t gcol_offset(int i, int cap) {
-    return (std::int64_t)i * cap - (std::int64_t)i * (i - 1) / 2;
+// Strictly lower Gram triangle L(j, i) = <V_j, V_i>, i < j, packed by rows:
+// row j (written at Krylov step j) holds columns 0..j-1.  The offsets do not
+// depend on the capacity, so the triangle grows with the Krylov space.
+__host__ __device__ inline std::int64_t gram_index(int j, int i) {
+    return (std::int64_t)j * (j - 1) / 2 + i;
 }
-__host__ __device__ inline std::int64_t gram_index(int j, int i, int cap) {
-    return gcol_offset(i, cap) + (j - i - 1);
-}
-inline std::int64_t gram_size(int cap) {
-    return (std::int64_t)cap * (cap + 1) / 2;
+// Entries of rows 0..steps-1.
+inline std::int64_t gram_size(int steps) {
+    return (std::int64_t)steps * (steps + 1) / 2;
 }
 
 struct SysState {
@@ -93,8 +91,10 @@ template <typename T> struct KrylovArgs {
     double *x;    // candidate / solution (FP64)
     const double *x0;
     // Compact Gram-Schmidt workspace: gp[v][tile] per-tile partials of
-    // <V_i, w> (v = i) and <V_i, V_k> (v = cap+1+i); per system q = V^T w,
-    // h = MGS coefficients ([cap+1] each) and the packed Gram triangle.
+    // <V_i, w> (v = 2i) and <V_i, V_k> (v = 2i+1); per system q = V^T w,
+    // h = MGS coefficients ([cap+1] each) and the packed Gram triangle.  H,
+    // the Gram triangle and gp are sized for the Krylov steps reached so far
+    // (hstride, gstride per system) and grow with them.
     double *gp, *q, *h, *gram;
     std::int64_t gstride;
     // 1: a triggered step holds the system (run cleared) until the host
