@@ -105,6 +105,10 @@ void Layout::build(const Context &c, std::int64_t n_, const std::vector<std::int
         st1[s] = (int)ts.size();
     }
     ntiles = (int)ts.size();
+    uniform = 1;
+    for (int t = 0; t < ntiles; ++t)
+        if (r1[t] - r0[t] != kTileRows || r0[t] % 4 != 0)
+            uniform = 0;
     t_sys.resize(ntiles);
     t_row0.resize(ntiles);
     t_row1.resize(ntiles);
@@ -725,9 +729,7 @@ template <typename T> void GmresEngine<T>::ensure_steps(int k) {
         HIP_CHECK(hipMemcpy2DAsync(H2.data(), hs * sizeof(double), H_.data(),
                                    hs0 * sizeof(double), hs0 * sizeof(double), nsys,
                                    hipMemcpyDeviceToDevice, c_.stream));
-        HIP_CHECK(hipMemcpy2DAsync(G2.data(), gs * sizeof(double), gram_.data(),
-                                   gs0 * sizeof(double), gs0 * sizeof(double), nsys,
-                                   hipMemcpyDeviceToDevice, c_.stream));
+        gmres_gram_relayout(c_.stream, (int)nsys, gram_.data(), gs0, ksteps_, G2.data(), gs, K);
         sync(c_); // the old buffers are released below
     }
     H_ = std::move(H2);
@@ -738,6 +740,7 @@ template <typename T> void GmresEngine<T>::ensure_steps(int k) {
     args_.hstride = hs;
     args_.gram = gram_.data();
     args_.gstride = gs;
+    args_.gsteps = K;
     args_.gp = gp_.data();
     ksteps_ = K;
 }

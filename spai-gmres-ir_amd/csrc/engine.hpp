@@ -154,9 +154,10 @@ struct Layout {
     std::vector<std::int64_t> offsets; // nsys+1
     DeviceArray<int> t_sys, t_row0, t_row1, s_tile0, s_tile1, s_rows;
     int ntiles = 0;
+    int uniform = 0; // every tile full and 16-byte aligned for FP32 and FP64
     Tiles tiles() const {
-        return Tiles{ntiles, nsys, t_sys.data(), t_row0.data(), t_row1.data(), s_tile0.data(),
-                     s_tile1.data()};
+        return Tiles{ntiles,         nsys,           t_sys.data(), t_row0.data(),
+                     t_row1.data(),  s_tile0.data(), s_tile1.data(), uniform};
     }
     void build(const Context &c, std::int64_t n_, const std::vector<std::int64_t> &offs);
 };

@@ -88,6 +88,7 @@ struct Tiles {
     const int *row1;
     const int *sys_tile0;
     const int *sys_tile1;
+    int uniform; // every tile full (kTileRows rows) and starting at a multiple of 4 rows
 };
 // A tile is one block of the vector kernels: 256 threads x 8 rows, one
 // partial per tile and reduction.

@@ -32,6 +32,7 @@
 //
 // Precision: T is the Krylov vector/matrix type (double for the plain path,
 // float for the mixed path); Hessenberg, rotations, y and candidates are FP64.
+#include <type_traits>
 #include "common.hpp"
 #include "krylov.hpp"
 
@@ -211,6 +212,37 @@ template <> __device__ inline float lane_of<float>(const float4 &v, int e) {
 template <> __device__ inline double lane_of<double>(const double2 &v, int e) {
     return e == 0 ? v.x : v.y;
 }
+// 16 bytes at p: one access when p is 16-byte aligned (A), else element by
+// element -- the same values either way, so a tile's result does not depend
+// on its alignment.  Checked (C): only the first lim elements exist, the
+// others read as zero and are not stored (the last tile of a system).
+template <typename T, bool A, bool C = false>
+__device__ inline typename Vec16<T>::type ld16(const T *p, int lim = 16) {
+    using V16 = typename Vec16<T>::type;
+    if constexpr (A && !C) {
+        return *reinterpret_cast<const V16 *>(p);
+    } else {
+        V16 v;
+        T *q = reinterpret_cast<T *>(&v);
+#pragma unroll
+        for (int e = 0; e < (int)(16 / sizeof(T)); ++e)
+            q[e] = !C || e < lim ? p[e] : T(0);
+        return v;
+    }
+}
+template <typename T, bool A, bool C = false>
+__device__ inline void st16(T *p, const typename Vec16<T>::type &v, int lim = 16) {
+    using V16 = typename Vec16<T>::type;
+    if constexpr (A && !C) {
+        *reinterpret_cast<V16 *>(p) = v;
+    } else {
+        const T *q = reinterpret_cast<const T *>(&v);
+#pragma unroll
+        for (int e = 0; e < (int)(16 / sizeof(T)); ++e)
+            if (!C || e < lim)
+                p[e] = q[e];
+    }
+}
 
 // Pass 1, grid (tile, group of kProjVecs vectors): w and V_k of the tile are
 // staged in LDS; each wave streams kProjPerWave basis vectors of the group
@@ -220,7 +252,7 @@ template <> __device__ inline double lane_of<double>(const double2 &v, int e) {
 // gp[cap+1+i][tile].  Full tiles whose start is 16-byte aligned (every tile
 // of a single system) use 16-byte accesses; the others read the same rows
 // per lane in the same order, one element at a time.
-template <typename T>
+template <typename T, bool kUniform>
 __global__ void __launch_bounds__(kProjWaves *kWave)
 project_kernel(Tiles tiles, KrylovArgs<T> a, int k) {
     using V16 = typename Vec16<T>::type;
@@ -253,14 +285,18 @@ project_kernel(Tiles tiles, KrylovArgs<T> a, int k) {
 #pragma unroll
     for (int u = 0; u < P; ++u)
         dw[u] = dv[u] = 0;
-    if (nr == kTileRows && r0 % E == 0) {
-#pragma unroll 2
-        for (int g = 0; g < G; ++g) {
+    // 16-byte groups: one access each (aligned full tiles), element by
+    // element (full tiles that start mid-group, as in batched systems), or
+    // element by element with the rows past the tile's end reading as zero
+    // (a system's last tile).  Same rows per lane and order in all three.
+    auto groups = [&](auto aligned, auto checked) {
+        constexpr bool A = decltype(aligned)::value, C = decltype(checked)::value;
+        auto step = [&](int g) {
             const int q = g * kWave + lane;
             V16 x[P];
 #pragma unroll
             for (int u = 0; u < P; ++u)
-                x[u] = reinterpret_cast<const V16 *>(vp[u])[q];
+                x[u] = ld16<T, A, C>(vp[u] + (std::int64_t)q * E, nr - q * E);
             const V16 wq = ws4[q], vq = vs4[q];
 #pragma unroll
             for (int e = 0; e < E; ++e) {
@@ -272,23 +308,26 @@ project_kernel(Tiles tiles, KrylovArgs<T> a, int k) {
                     dv[u] += xe * ve;
                 }
             }
+        };
+        if constexpr (C) { // a system's last tile: keep its registers small
+#pragma unroll 1
+            for (int g = 0; g < G; ++g)
+                step(g);
+        } else {
+#pragma unroll 2
+            for (int g = 0; g < G; ++g)
+                step(g);
         }
-    } else {
-        for (int g = 0; g < G; ++g) {
-            const int q = g * kWave + lane;
-            for (int e = 0; e < E; ++e) {
-                const int r = q * E + e;
-                if (r < nr) {
-                    const double we = (double)ws[r], ve = (double)vs[r];
-#pragma unroll
-                    for (int u = 0; u < P; ++u) {
-                        const double xe = (double)vp[u][r];
-                        dw[u] += xe * we;
-                        dv[u] += xe * ve;
-                    }
-                }
-            }
-        }
+    };
+    // (kUniform: the host found every tile full and aligned, and this
+    // instantiation holds that path only, with its smaller register budget)
+    if (kUniform || (nr == kTileRows && r0 % E == 0))
+        groups(std::true_type{}, std::false_type{});
+    else if constexpr (!kUniform) {
+        if (nr == kTileRows)
+            groups(std::false_type{}, std::false_type{});
+        else
+            groups(std::false_type{}, std::true_type{});
     }
 #pragma unroll
     for (int u = 0; u < P; ++u) {
@@ -333,7 +372,7 @@ gram_reduce_kernel(Tiles tiles, KrylovArgs<T> a, int k) {
         if (v <= k)
             a.q[(std::int64_t)s * (a.cap + 1) + v] = sum;
         else
-            a.gram[(std::int64_t)s * a.gstride + gram_index(k, v - k - 1)] = sum;
+            a.gram[(std::int64_t)s * a.gstride + gram_index(k, v - k - 1, a.gsteps)] = sum;
     }
 }
 
@@ -368,7 +407,7 @@ __global__ void __launch_bounds__(kWave) gram_solve_kernel(Tiles tiles, KrylovAr
 #pragma unroll
             for (int u = 0; u < kLoadBatch; ++u) {
                 const int i = i0 + u;
-                v[u] = i <= last && mine && j > i ? L[gram_index(j, i)] : 0.0;
+                v[u] = i <= last && mine && j > i ? L[gram_index(j, i, a.gsteps)] : 0.0;
             }
 #pragma unroll
             for (int u = 0; u < kLoadBatch; ++u)
@@ -380,7 +419,7 @@ __global__ void __launch_bounds__(kWave) gram_solve_kernel(Tiles tiles, KrylovAr
             double lv[kLoadBatch], hv[kLoadBatch];
 #pragma unroll
             for (int u = 0; u < kLoadBatch; ++u) {
-                lv[u] = mine ? L[gram_index(j, i0 + u)] : 0.0;
+                lv[u] = mine ? L[gram_index(j, i0 + u, a.gsteps)] : 0.0;
                 hv[u] = h[i0 + u];
             }
 #pragma unroll
@@ -556,7 +595,7 @@ gram_solve_lds_kernel(Tiles tiles, KrylovArgs<T> a, int k) {
                 else if ((long long)(i + 1) * (2 * k - i) / 2 <= e)
                     ++i;
                 const long long fi = (long long)i * (2 * k + 1 - i) / 2;
-                src[u] = gram_index(i + 1 + (int)(e - fi), i);
+                src[u] = gram_index(i + 1 + (int)(e - fi), i, a.gsteps);
             }
             double v[R];
 #pragma unroll
@@ -642,16 +681,19 @@ gram_solve_lds_kernel(Tiles tiles, KrylovArgs<T> a, int k) {
 // of kUpdBatch<T> vectors, two batches in registers, so that one batch's loads
 // are in flight while the other is applied.  Tiles that are not full and
 // 16-byte aligned take the same rows per thread, element by element.
-template <typename T, int G>
+template <typename T, int G, bool A, bool C>
 __device__ inline void upd_load(typename Vec16<T>::type (&buf)[kUpdBatch<T>][G],
-                                const KrylovArgs<T> &a, int base, int k, int r0, int tid) {
-    using V16 = typename Vec16<T>::type;
+                                const KrylovArgs<T> &a, int base, int k, int r0, int tid,
+                                int nr) {
+    constexpr int E = 16 / sizeof(T);
 #pragma unroll
     for (int u = 0; u < kUpdBatch<T>; ++u) {
-        const V16 *__restrict__ v4 = reinterpret_cast<const V16 *>(a.V[min(base + u, k)] + r0);
+        const T *__restrict__ v = a.V[min(base + u, k)] + r0;
 #pragma unroll
-        for (int g = 0; g < G; ++g)
-            buf[u][g] = v4[g * kUpdateThreads + tid];
+        for (int g = 0; g < G; ++g) {
+            const int r = (g * kUpdateThreads + tid) * E;
+            buf[u][g] = ld16<T, A, C>(v + r, nr - r);
+        }
     }
 }
 template <typename T, int G, int E>
@@ -685,24 +727,30 @@ gram_update_kernel(Tiles tiles, KrylovArgs<T> a, int k) {
     const int tid = threadIdx.x;
     T *__restrict__ w = a.w + r0;
     T x[G][E];
-    if (nr == kTileRows && r0 % E == 0) {
-        V16 *__restrict__ w4 = reinterpret_cast<V16 *>(w);
+    // 16-byte groups: one access each (aligned full tiles), element by
+    // element (full tiles that start mid-group, as in batched systems), or
+    // element by element with the rows past the tile's end reading as zero
+    // and left unwritten (a system's last tile).  Same rows per thread and
+    // order in all three.
+    auto groups = [&](auto aligned, auto checked) {
+        constexpr bool A = decltype(aligned)::value, C = decltype(checked)::value;
 #pragma unroll
         for (int g = 0; g < G; ++g) {
-            const V16 y = w4[g * kUpdateThreads + tid];
+            const int r = (g * kUpdateThreads + tid) * E;
+            const V16 y = ld16<T, A, C>(w + r, nr - r);
 #pragma unroll
             for (int e = 0; e < E; ++e)
                 x[g][e] = lane_of<T>(y, e);
         }
         constexpr int B = kUpdBatch<T>;
         V16 b0[B][G], b1[B][G];
-        upd_load<T, G>(b0, a, 0, k, r0, tid);
+        upd_load<T, G, A, C>(b0, a, 0, k, r0, tid, nr);
         for (int base = 0; base <= k; base += 2 * B) {
-            upd_load<T, G>(b1, a, base + B, k, r0, tid);
+            upd_load<T, G, A, C>(b1, a, base + B, k, r0, tid, nr);
             upd_apply<T, G, E>(x, b0, h, base, k);
             if (base + B > k)
                 break;
-            upd_load<T, G>(b0, a, base + 2 * B, k, r0, tid);
+            upd_load<T, G, A, C>(b0, a, base + 2 * B, k, r0, tid, nr);
             upd_apply<T, G, E>(x, b1, h, base + B, k);
         }
 #pragma unroll
@@ -712,37 +760,16 @@ gram_update_kernel(Tiles tiles, KrylovArgs<T> a, int k) {
 #pragma unroll
             for (int e = 0; e < E; ++e)
                 yp[e] = x[g][e];
-            w4[g * kUpdateThreads + tid] = y;
+            const int r = (g * kUpdateThreads + tid) * E;
+            st16<T, A, C>(w + r, y, nr - r);
         }
-    } else {
-#pragma unroll
-        for (int g = 0; g < G; ++g)
-#pragma unroll
-            for (int e = 0; e < E; ++e) {
-                const int r = (g * kUpdateThreads + tid) * E + e;
-                x[g][e] = r < nr ? w[r] : T(0);
-            }
-        for (int i = 0; i <= k; ++i) {
-            const T *__restrict__ v = a.V[i] + r0;
-            const T hi = (T)h[i];
-#pragma unroll
-            for (int g = 0; g < G; ++g)
-#pragma unroll
-                for (int e = 0; e < E; ++e) {
-                    const int r = (g * kUpdateThreads + tid) * E + e;
-                    if (r < nr)
-                        x[g][e] = x[g][e] - hi * v[r];
-                }
-        }
-#pragma unroll
-        for (int g = 0; g < G; ++g)
-#pragma unroll
-            for (int e = 0; e < E; ++e) {
-                const int r = (g * kUpdateThreads + tid) * E + e;
-                if (r < nr)
-                    w[r] = x[g][e];
-            }
-    }
+    };
+    if (nr == kTileRows && r0 % E == 0)
+        groups(std::true_type{}, std::false_type{});
+    else if (nr == kTileRows)
+        groups(std::false_type{}, std::false_type{});
+    else
+        groups(std::false_type{}, std::true_type{});
     double vals[G * E];
 #pragma unroll
     for (int g = 0; g < G; ++g)
@@ -778,19 +805,25 @@ normalize_givens_kernel(Tiles tiles, KrylovArgs<T> a, int k) {
         const int r0 = tiles.row0[t], nr = tiles.row1[t] - r0;
         T *v = a.V[k + 1] + r0;
         const T *w = a.w + r0;
-        if (nr == kTileRows && r0 % E == 0) {
+        auto full = [&](auto aligned) {
+            constexpr bool A = decltype(aligned)::value;
             V16 x[G];
 #pragma unroll
             for (int g = 0; g < G; ++g)
-                x[g] = reinterpret_cast<const V16 *>(w)[g * kTileThreads + threadIdx.x];
+                x[g] = ld16<T, A>(w + (std::int64_t)(g * kTileThreads + threadIdx.x) * E);
 #pragma unroll
             for (int g = 0; g < G; ++g) {
                 T *xp = reinterpret_cast<T *>(&x[g]);
 #pragma unroll
                 for (int e = 0; e < E; ++e)
                     xp[e] = xp[e] / h;
-                reinterpret_cast<V16 *>(v)[g * kTileThreads + threadIdx.x] = x[g];
+                st16<T, A>(v + (std::int64_t)(g * kTileThreads + threadIdx.x) * E, x[g]);
             }
+        };
+        if (nr == kTileRows && r0 % E == 0) {
+            full(std::true_type{});
+        } else if (nr == kTileRows) {
+            full(std::false_type{});
         } else {
 #pragma unroll 8
             for (int r = threadIdx.x; r < nr; r += kTileThreads)
@@ -1017,12 +1050,37 @@ __global__ void gmres_reset_kernel(int nsys, KrylovArgs<T> k, const IrSysState *
     k.st[s] = st;
 }
 
+// Gram triangle re-laid from K0 to K steps (column i moves from
+// gcol_offset(i, K0) to gcol_offset(i, K); its rows i+1..K0-1 are kept).
+// Grid (system, column group): block y copies columns y, y + gridDim.y, ...
+__global__ void gram_relayout_kernel(const double *__restrict__ src, std::int64_t sstride, int K0,
+                                     double *__restrict__ dst, std::int64_t dstride, int K) {
+    const int s = blockIdx.x;
+    for (int i = blockIdx.y; i < K0 - 1; i += gridDim.y) {
+        const double *a = src + s * sstride + gcol_offset(i, K0);
+        double *b = dst + s * dstride + gcol_offset(i, K);
+        for (int r = threadIdx.x; r < K0 - 1 - i; r += blockDim.x)
+            b[r] = a[r];
+    }
+}
+void gmres_gram_relayout(hipStream_t s, int nsys, const double *src, std::int64_t sstride, int K0,
+                         double *dst, std::int64_t dstride, int K) {
+    if (K0 < 2 || nsys == 0)
+        return;
+    hipLaunchKernelGGL(gram_relayout_kernel, dim3(nsys, std::min(K0 - 1, 1024)), dim3(256), 0, s,
+                       src, sstride, K0, dst, dstride, K);
+    HIP_LAUNCH_CHECK();
+}
+
 // ------------------------------------------------------------------ launchers
 #define BSPAI_GRID dim3(tl.ntiles), dim3(kTileThreads), 0, s
 
 template <typename T> void gmres_project(hipStream_t s, const Tiles &tl, const KrylovArgs<T> &a, int k) {
-    hipLaunchKernelGGL(project_kernel<T>, dim3(tl.ntiles, ceil_div(k + 1, kProjVecs)),
-                       dim3(kProjWaves * kWave), 0, s, tl, a, k);
+    const dim3 grid(tl.ntiles, ceil_div(k + 1, kProjVecs)), block(kProjWaves * kWave);
+    if (tl.uniform)
+        hipLaunchKernelGGL((project_kernel<T, true>), grid, block, 0, s, tl, a, k);
+    else
+        hipLaunchKernelGGL((project_kernel<T, false>), grid, block, 0, s, tl, a, k);
     HIP_LAUNCH_CHECK();
 }
 template <typename T> void gmres_gram_reduce(hipStream_t s, const Tiles &tl, const KrylovArgs<T> &a, int k) {

@@ -50,15 +50,18 @@ __host__ __device__ inline std::int64_t hcol_offset(int k) {
 inline std::int64_t hessenberg_size(int cap) {
     return hcol_offset(cap) + 2;
 }
-// Strictly lower Gram triangle L(j, i) = <V_j, V_i>, i < j, packed by rows:
-// row j (written at Krylov step j) holds columns 0..j-1.  The offsets do not
-// depend on the capacity, so the triangle grows with the Krylov space.
-__host__ __device__ inline std::int64_t gram_index(int j, int i) {
-    return (std::int64_t)j * (j - 1) / 2 + i;
+// Strictly lower Gram triangle L(j, i) = <V_j, V_i>, i < j < K, packed by
+// columns: column i holds rows i+1..K-1, so the lanes of a wave that own
+// consecutive rows j read consecutive words.  K is the number of Krylov steps
+// the workspace currently holds (it grows; a larger K re-lays the columns).
+__host__ __device__ inline std::int64_t gcol_offset(int i, int K) {
+    return (std::int64_t)i * K - (std::int64_t)i * (i + 1) / 2;
 }
-// Entries of rows 0..steps-1.
-inline std::int64_t gram_size(int steps) {
-    return (std::int64_t)steps * (steps + 1) / 2;
+__host__ __device__ inline std::int64_t gram_index(int j, int i, int K) {
+    return gcol_offset(i, K) + (j - i - 1);
+}
+inline std::int64_t gram_size(int K) {
+    return (std::int64_t)K * (K - 1) / 2 + 1;
 }
 
 struct SysState {
@@ -97,6 +100,7 @@ template <typename T> struct KrylovArgs {
     // (hstride, gstride per system) and grow with them.
     double *gp, *q, *h, *gram;
     std::int64_t gstride;
+    int gsteps; // K of the Gram layout
     // 1: a triggered step holds the system (run cleared) until the host
     // launches the verification; 0: the verification follows every step.
     int defer;
@@ -122,6 +126,9 @@ struct IrArgs {
 
 // Compact MGS of iteration k: project (V^T w and the Gram row), reduce over
 // tiles, triangular solve for h (+ older rotations), update w -= V h.
+// Copies the Gram triangles of nsys systems from a K0-step to a K-step layout.
+void gmres_gram_relayout(hipStream_t s, int nsys, const double *src, std::int64_t sstride, int K0,
+                         double *dst, std::int64_t dstride, int K);
 template <typename T> void gmres_project(hipStream_t s, const Tiles &tl, const KrylovArgs<T> &a, int k);
 template <typename T> void gmres_gram_reduce(hipStream_t s, const Tiles &tl, const KrylovArgs<T> &a, int k);
 template <typename T> void gmres_gram_solve(hipStream_t s, const Tiles &tl, const KrylovArgs<T> &a, int k);
