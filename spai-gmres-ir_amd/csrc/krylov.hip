@@ -475,16 +475,17 @@ __global__ void __launch_bounds__(kWave) gram_solve_kernel(Tiles tiles, KrylovAr
     }
 }
 
-// The same solve for k < 256 with a 1024-thread block: all waves stage
+// The same solve for k < 512 with a 1024-thread block: all waves stage
 // column chunks of L in LDS (every load of a chunk in flight at once), then
 // wave 0 runs the substitution from LDS, unknown j in lane j % 64, slot
 // j / 64, each pivot broadcast by readlane.  Every h_j again
 // sees its terms in the order i = 0, 1, ..., j-1 (bit-identical to the
 // kernel above); the rotations follow from LDS.
 constexpr int kSolveThreads = 1024;
-constexpr int kSolveSlots = 4;        // unknowns per lane: k < 256
-constexpr int kSolveChunk = 6144;     // L entries per LDS chunk (48 KiB)
-static_assert(kGramSolveLdsMaxK == kSolveSlots * kWave - 1, "LDS Gram solve capacity");
+// S unknowns per lane (k < 64 S): 4 for k < 256 with 48 KiB chunks of L, 8
+// up to k < 512 with 40 KiB chunks (the larger slot arrays share the LDS).
+template <int S> constexpr int kSolveChunk = S == 4 ? 6144 : 5120;
+static_assert(kGramSolveLdsMaxK == 8 * kWave - 1, "LDS Gram solve capacity");
 
 __device__ inline double readlane_f64(double v, int l) {
     const long long b = __double_as_longlong(v);
@@ -499,29 +500,29 @@ __device__ inline double readlane_f64(double v, int l) {
 // value; every row of a later slot lies below i, so no mask is needed there
 // (rows beyond k compute values that are never stored).  Lc is padded by
 // kSolvePad entries on both sides, so these reads stay in bounds.
-constexpr int kSolvePad = kSolveSlots * kWave;
-template <int B>
-__device__ inline void gram_steps(double (&acc)[kSolveSlots], const double *Lc, int &i, int &lo,
-                                  int iend, int k, int lane) {
+template <int S, int B>
+__device__ inline void gram_steps(double (&acc)[S], const double *Lc, int &i, int &lo, int iend,
+                                  int k, int lane) {
+    constexpr int kSolvePad = S * kWave;
     const int stop = min(iend, (B + 1) * kWave);
     if (i >= stop)
         return;
-    auto load = [&](int ii, int l, double(&lv)[kSolveSlots]) {
+    auto load = [&](int ii, int l, double(&lv)[S]) {
         const double *colp = Lc + kSolvePad + l - ii - 1 + lane;
 #pragma unroll
-        for (int u = B; u < kSolveSlots; ++u)
+        for (int u = B; u < S; ++u)
             lv[u] = colp[u * kWave];
     };
-    auto step = [&](int ii, const double(&lv)[kSolveSlots]) {
+    auto step = [&](int ii, const double(&lv)[S]) {
         const double hi = readlane_f64(acc[B], ii - B * kWave);
         const double y = acc[B] - lv[B] * hi;
         acc[B] = lane + B * kWave > ii ? y : acc[B];
 #pragma unroll
-        for (int u = B + 1; u < kSolveSlots; ++u)
+        for (int u = B + 1; u < S; ++u)
             acc[u] = acc[u] - lv[u] * hi;
     };
     // two register sets: step i's entries were read during step i-1
-    double la[kSolveSlots], lb[kSolveSlots];
+    double la[S], lb[S];
     load(i, lo, la);
     for (;;) {
         int nlo = lo + (k - i);
@@ -541,12 +542,12 @@ __device__ inline void gram_steps(double (&acc)[kSolveSlots], const double *Lc, 
     }
 }
 
-template <typename T>
+template <typename T, int S>
 __global__ void __launch_bounds__(kSolveThreads)
 gram_solve_lds_kernel(Tiles tiles, KrylovArgs<T> a, int k) {
-    __shared__ double Lc[kSolvePad + kSolveChunk + kSolvePad];
-    __shared__ double hs[kSolveSlots * kWave], lcs[kSolveSlots * kWave],
-        lsn[kSolveSlots * kWave];
+    constexpr int kSolvePad = S * kWave, kChunk = kSolveChunk<S>;
+    __shared__ double Lc[kSolvePad + kChunk + kSolvePad];
+    __shared__ double hs[S * kWave], lcs[S * kWave], lsn[S * kWave];
     const int s = blockIdx.x;
     if (!a.st[s].active || !a.st[s].run[k & 1])
         return;
@@ -556,13 +557,13 @@ gram_solve_lds_kernel(Tiles tiles, KrylovArgs<T> a, int k) {
     double *h = a.h + (std::int64_t)s * (cap + 1);
     double *col = a.H + (std::int64_t)s * a.hstride + hcol_offset(k);
     const double *cs = a.cs + (std::int64_t)s * cap, *sn = a.sn + (std::int64_t)s * cap;
-    // The rotations (k <= 255 < kSolveThreads: one each) and q are loaded
+    // The rotations (k <= 511 < kSolveThreads: one each) and q are loaded
     // with the first chunk of L, so the three are one round of latency.
     const bool rot = tid < k;
     const double ccs = rot ? cs[tid] : 0.0, csn = rot ? sn[tid] : 0.0;
-    double acc[kSolveSlots];
+    double acc[S];
 #pragma unroll
-    for (int u = 0; u < kSolveSlots; ++u) {
+    for (int u = 0; u < S; ++u) {
         const int j = lane + u * kWave;
         acc[u] = wave == 0 && j <= k ? q[j] : 0.0;
     }
@@ -572,7 +573,7 @@ gram_solve_lds_kernel(Tiles tiles, KrylovArgs<T> a, int k) {
     // sum_{c0 <= i' < i} (k - i').
     for (int c0 = 0; c0 < k;) {
         int c1 = c0, used = 0;
-        while (c1 < k && used + (k - c1) <= kSolveChunk) {
+        while (c1 < k && used + (k - c1) <= kChunk) {
             used += k - c1;
             ++c1;
         }
@@ -581,7 +582,7 @@ gram_solve_lds_kernel(Tiles tiles, KrylovArgs<T> a, int k) {
         // inverted in closed form (then corrected by one step either way);
         // all of a thread's loads are issued together.
         {
-            constexpr int R = kSolveChunk / kSolveThreads;
+            constexpr int R = kChunk / kSolveThreads;
             const long long base = (long long)c0 * (2 * k + 1 - c0) / 2;
             const double b2 = 2.0 * k + 1.0;
             int src[R];
@@ -614,10 +615,16 @@ gram_solve_lds_kernel(Tiles tiles, KrylovArgs<T> a, int k) {
         __syncthreads();
         if (wave == 0) {
             int i = c0, lo = 0;
-            gram_steps<0>(acc, Lc, i, lo, c1, k, lane);
-            gram_steps<1>(acc, Lc, i, lo, c1, k, lane);
-            gram_steps<2>(acc, Lc, i, lo, c1, k, lane);
-            gram_steps<3>(acc, Lc, i, lo, c1, k, lane);
+            gram_steps<S, 0>(acc, Lc, i, lo, c1, k, lane);
+            gram_steps<S, 1>(acc, Lc, i, lo, c1, k, lane);
+            gram_steps<S, 2>(acc, Lc, i, lo, c1, k, lane);
+            gram_steps<S, 3>(acc, Lc, i, lo, c1, k, lane);
+            if constexpr (S > 4) {
+                gram_steps<S, 4>(acc, Lc, i, lo, c1, k, lane);
+                gram_steps<S, 5>(acc, Lc, i, lo, c1, k, lane);
+                gram_steps<S, 6>(acc, Lc, i, lo, c1, k, lane);
+                gram_steps<S, 7>(acc, Lc, i, lo, c1, k, lane);
+            }
         }
         __syncthreads();
         c0 = c1;
@@ -625,7 +632,7 @@ gram_solve_lds_kernel(Tiles tiles, KrylovArgs<T> a, int k) {
     if (wave != 0)
         return;
 #pragma unroll
-    for (int u = 0; u < kSolveSlots; ++u) {
+    for (int u = 0; u < S; ++u) {
         const int j = lane + u * kWave;
         if (j <= k) {
             h[j] = acc[u];
@@ -1089,9 +1096,12 @@ template <typename T> void gmres_gram_reduce(hipStream_t s, const Tiles &tl, con
     HIP_LAUNCH_CHECK();
 }
 template <typename T> void gmres_gram_solve(hipStream_t s, const Tiles &tl, const KrylovArgs<T> &a, int k) {
-    if (k <= kGramSolveLdsMaxK)
-        hipLaunchKernelGGL(gram_solve_lds_kernel<T>, dim3(tl.nsys), dim3(kSolveThreads), 0, s, tl,
-                           a, k);
+    if (k < 4 * kWave)
+        hipLaunchKernelGGL((gram_solve_lds_kernel<T, 4>), dim3(tl.nsys), dim3(kSolveThreads), 0,
+                           s, tl, a, k);
+    else if (k <= kGramSolveLdsMaxK)
+        hipLaunchKernelGGL((gram_solve_lds_kernel<T, 8>), dim3(tl.nsys), dim3(kSolveThreads), 0,
+                           s, tl, a, k);
     else
         hipLaunchKernelGGL(gram_solve_kernel<T>, dim3(tl.nsys), dim3(kWave), 0, s, tl, a, k);
     HIP_LAUNCH_CHECK();

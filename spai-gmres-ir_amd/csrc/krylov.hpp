@@ -81,7 +81,7 @@ struct SysState {
 };
 
 // Largest k solved by the LDS-staged Gram solve (beyond it, the one-wave kernel).
-constexpr int kGramSolveLdsMaxK = 255;
+constexpr int kGramSolveLdsMaxK = 511;
 
 template <typename T> struct KrylovArgs {
     SysState *st;
