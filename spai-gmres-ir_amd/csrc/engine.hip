@@ -355,15 +355,16 @@ void spai_build_typed(Context &c, const DevCsr &a, const T *aval, T *d, double e
         return;
     }
     // ---- B structure = transpose of A's pattern (row i lists rows j with a_ji != 0)
-    // FP32 first bucket: one wavefront per column while the columns are too
-    // few to fill the device with one lane per column (about 160 per CU: the
-    // measured crossover lies between 40k and 110k columns on 256 CUs), the
-    // lane kernel above that.  BSPAI_SPAI_FP32_KERNEL=lane|wave forces one.
+    // FP32 first bucket: one wavefront per column (measured faster than one
+    // lane per column from 4k to 2.1M columns: 58 against 103 ms at 128^3),
+    // up to the column count its 32-bit sort keys hold; the lane kernel
+    // beyond.  BSPAI_SPAI_FP32_KERNEL=lane|wave forces one.
     hipDeviceProp_t prop;
     HIP_CHECK(hipGetDeviceProperties(&prop, c.device));
-    bool wave_fp32 = std::is_same<T, float>::value && n <= 160LL * prop.multiProcessorCount;
+    bool wave_fp32 = std::is_same<T, float>::value;
     if (const char *env = std::getenv("BSPAI_SPAI_FP32_KERNEL"))
         wave_fp32 = std::is_same<T, float>::value && std::string(env) == "wave";
+    wave_fp32 = wave_fp32 && n <= spai_wave_max_n();
     DeviceArray<int> bptr(n + 1), bcol(a.nnz > 0 ? a.nnz : 1), cursor(n + 1);
     DeviceArray<T> bval(wave_fp32 && a.nnz > 0 ? a.nnz : 1);
     DeviceArray<int> scan_tmp(scan_tmp_ints(n + 1));

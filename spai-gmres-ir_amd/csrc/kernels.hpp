@@ -44,13 +44,15 @@ void launch_spai_columns(hipStream_t s, int blocks, int threads, int n, const in
                          int nwork, int *cursor, int mcap, int qcap, int ccap, T *dwork,
                          int *iwork, SpaiColumnOut out, int *out_j, T *out_c,
                          std::int64_t out_stride);
-// FP32 wavefront-per-column kernel (first bucket of the mixed path).
-// bval: the values of B = A^T (unscaled), aligned with bcol.
+// FP32 wavefront-per-column kernel (first bucket of the mixed path), n up to
+// spai_wave_max_n() (32-bit sort keys).  bval: the values of B = A^T
+// (unscaled), aligned with bcol.
 void launch_spai_wave(hipStream_t s, int blocks, const int *aptr, const int *acol,
                       const float *aval, const float *dscale, const int *bptr, const int *bcol,
                       const float *bval, float eps, long long alpha, int beta,
                       const int *worklist, int nwork, int *cursor, SpaiColumnOut out,
                       int *out_j, float *out_c, std::int64_t out_stride);
+int spai_wave_max_n();
 int spai_wave_qcap();
 // Workgroup-per-column kernel for the columns that outgrow the lane kernel.
 std::int64_t spai_block_doubles(int mcap, int qcap, int ccap);

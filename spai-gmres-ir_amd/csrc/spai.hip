@@ -662,7 +662,8 @@ template void launch_spai_columns<float>(hipStream_t, int, int, int, const int *
 //     res_i * b_ij d_j and (b_ij d_j)^2 are formed per entry and summed per
 //     candidate in sorted order (ascending i, as the lane kernel sums A's
 //     row j), with no further loads;
-//   * the beta smallest (score, index) pairs are found by ranking.
+//   * the beta smallest (score, index) pairs are found by argmin rounds
+//     over the wave.
 // The method and its decisions are the lane kernel's; the FP32 summation
 // order of the QR and norms differs, which the FP32 build allows (its
 // patterns are compared with the FP64 oracle's and differences counted).
@@ -671,15 +672,21 @@ constexpr int kWM = 64;      // rows (one per lane)
 constexpr int kWQ = 16;      // pattern columns
 constexpr int kWC = 128;     // distinct candidate rows including J
 constexpr int kWS = 512;     // staged entries (9-bit entry numbers in the sort keys)
+constexpr int kWKeyColBits = 23; // column bits of a 32-bit sort key (n < 2^23)
 constexpr int kWW = kWQ + 1; // LDS row stride of R
 
 struct WaveLds {
+    // R is read last by the residual, before the candidate stage; the
+    // candidate stage's arrays are read last by the scores, before the next
+    // fit's gather.  9,344 bytes in all: 16 waves fit one CU's LDS.
     union {
         float R[kWM][kWW]; // gathered B(I, J) (unchanged by the QR, which runs in registers)
-        float sden[kWS];   // candidate stage: (b_ij d_j)^2 in sorted order
+        struct {
+            float sden[kWS]; // candidate stage: (b_ij d_j)^2 in sorted order
+            int skey[kWS];   // candidate stage: keys in sorted order
+        };
     };
     float sval[kWS];     // staged values by entry; candidate stage: res_i b_ij d_j, sorted
-    int skey[kWS];       // candidate stage: keys in sorted order
     signed char sc[kWS]; // entry -> staging row
     int sbeg[kWM];       // staging row r holds entries e of source index sbeg[r] + e
     float sd[kWM];       // staging row scale
@@ -738,34 +745,42 @@ __device__ int wave_rows(WaveLds &W, const int *__restrict__ ptr,
     return total;
 }
 
-// Bitonic sort of 64 P keys, element e = lane * P + i held in key[i].
-template <int P> __device__ inline void wave_bitonic(unsigned long long (&key)[P], int lane) {
-#pragma unroll
+// Bitonic sort of 64 P keys, element e = lane * P + i held in key[i].  The
+// stage loops stay rolled (k and j are wave-uniform) so that the sorts of all
+// widths together stay small in the instruction cache; only the P keys of a
+// lane are unrolled.
+template <int P> __device__ inline void wave_bitonic(unsigned (&key)[P], int lane) {
+#pragma unroll 1
     for (int k = 2; k <= kWave * P; k <<= 1) {
-#pragma unroll
+#pragma unroll 1
         for (int j = k >> 1; j > 0; j >>= 1) {
-            if (j < P) {
-#pragma unroll
-                for (int i = 0; i < P; ++i) {
-                    const int p = i ^ j;
-                    if (p > i) {
-                        const bool up = ((lane * P + i) & k) == 0;
-                        const unsigned long long x = key[i], y = key[p];
-                        const bool sw = up ? x > y : x < y;
-                        key[i] = sw ? y : x;
-                        key[p] = sw ? x : y;
-                    }
-                }
-            } else {
+            if (j >= P) { // partner in another lane: lane ^ (j / P), same slot
                 const int lj = j / P;
                 const bool lower = (lane & lj) == 0;
 #pragma unroll
                 for (int i = 0; i < P; ++i) {
-                    const unsigned long long o = __shfl_xor(key[i], lj, kWave);
+                    const unsigned o = __shfl_xor(key[i], lj, kWave);
                     const bool up = ((lane * P + i) & k) == 0;
                     const bool take_min = lower == up;
                     const bool less = o < key[i];
                     key[i] = take_min == less ? o : key[i];
+                }
+            } else { // partner in the same lane: slot i ^ j
+#pragma unroll
+                for (int jj = P / 2; jj > 0; jj >>= 1) {
+                    if (jj == j) {
+#pragma unroll
+                        for (int i = 0; i < P; ++i) {
+                            const int p = i ^ jj;
+                            if (p > i) {
+                                const bool up = ((lane * P + i) & k) == 0;
+                                const unsigned x = key[i], y = key[p];
+                                const bool sw = up ? x > y : x < y;
+                                key[i] = sw ? y : x;
+                                key[p] = sw ? x : y;
+                            }
+                        }
+                    }
                 }
             }
         }
@@ -782,7 +797,7 @@ template <int P, bool kCand>
 __device__ int wave_sort_set(WaveLds &W, const int *__restrict__ col,
                              const float *__restrict__ val, const float *__restrict__ dscale,
                              int total, int q, int lane) {
-    unsigned long long key[P];
+    unsigned key[P]; // (column << 9) | entry; the empty key ~0u sorts last
     {
         int src[P];
 #pragma unroll
@@ -803,7 +818,7 @@ __device__ int wave_sort_set(WaveLds &W, const int *__restrict__ col,
             const int e = lane * P + i;
             if (e < total)
                 W.sval[e] = v[i] * W.sd[W.sc[e]];
-            key[i] = e < total ? ((unsigned long long)(unsigned)c[i] << 9) | (unsigned)e : ~0ull;
+            key[i] = e < total ? ((unsigned)c[i] << 9) | (unsigned)e : ~0u;
         }
     }
     wave_bitonic<P>(key, lane);
@@ -812,7 +827,7 @@ __device__ int wave_sort_set(WaveLds &W, const int *__restrict__ col,
     bool keep[P], first[P];
 #pragma unroll
     for (int i = 0; i < P; ++i) {
-        kv[i] = key[i] == ~0ull ? -1 : (int)(key[i] >> 9);
+        kv[i] = key[i] == ~0u ? -1 : (int)(key[i] >> 9);
         ev[i] = (int)(key[i] & 511u);
     }
     int prev = __shfl_up(kv[P - 1], 1, kWave);
@@ -917,8 +932,13 @@ __device__ inline float wave_reduce_scatter16(float (&a)[kWQ], int lane) {
         const bool hi = (lane & mask) != 0;
 #pragma unroll
         for (int i = 0; i < half; ++i) {
-            const float keep = hi ? a[i + half] : a[i];
-            const float send = hi ? a[i] : a[i + half];
+            float lo = a[i], up = a[i + half];
+            // (an empty asm keeps both values in registers: otherwise the
+            // two selects are folded into one load at a lane-dependent
+            // index, lowered as a 16-way compare-and-select chain)
+            asm volatile("" : "+v"(lo), "+v"(up));
+            const float keep = hi ? up : lo;
+            const float send = hi ? lo : up;
             a[i] = keep + __shfl_xor(send, mask, kWave);
         }
     }
@@ -928,7 +948,8 @@ __device__ inline float wave_reduce_scatter16(float (&a)[kWQ], int lane) {
     return x;
 }
 
-__global__ void __launch_bounds__(kWave)
+// Four waves per SIMD: 128 VGPRs, and 16 waves' LDS fit one CU.
+__global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4)))
 spai_wave_kernel(const int *__restrict__ aptr, const int *__restrict__ acol,
                  const float *__restrict__ aval, const float *__restrict__ dscale,
                  const int *__restrict__ bptr, const int *__restrict__ bcol,
@@ -959,12 +980,14 @@ spai_wave_kernel(const int *__restrict__ aptr, const int *__restrict__ acol,
 #pragma unroll
             for (int c = 0; c < kWW; ++c)
                 W.R[lane][c] = 0.f;
-            const int na = wave_rows(W, aptr, dscale, lane < q ? W.J[lane] : 0, q, lane);
+            const int na = __builtin_amdgcn_readfirstlane(
+                wave_rows(W, aptr, dscale, lane < q ? W.J[lane] : 0, q, lane));
             if (na < 0) {
                 status = kSpaiOverflow;
                 break;
             }
-            const int m = wave_set<false>(W, acol, aval, nullptr, na, q, lane);
+            const int m =
+                __builtin_amdgcn_readfirstlane(wave_set<false>(W, acol, aval, nullptr, na, q, lane));
             if (m > kWM) {
                 status = kSpaiOverflow;
                 break;
@@ -993,88 +1016,99 @@ spai_wave_kernel(const int *__restrict__ aptr, const int *__restrict__ acol,
             const float inf_norm = wave_max(rowsum);
             float min_pivot = INFINITY, dg = 0.f;
             int fit_status = kSpaiSuccess;
-            // (every step is unrolled, so that r[] stays in registers; steps
-            // at or past q, or after a failure, are skipped as a whole)
+            // One rolled loop over the steps: column j of the row is picked
+            // from r[] by compares with j, so r[] stays in registers and the
+            // code holds one step (an unrolled loop over 16 steps made the
+            // kernel larger than the instruction cache).  Every value tested
+            // below is wave-uniform (butterfly sums give every lane the same
+            // bits), so the loop exits are uniform.
+#pragma unroll 1
+            for (int j = 0; j < q; ++j) {
+                const bool act = lane >= j && lane < m;
+                float x = 0.f;
 #pragma unroll
-            for (int j = 0; j < kWQ; ++j) {
-                if (j < q && fit_status == kSpaiSuccess) {
-                    const bool act = lane >= j && lane < m;
-                    const float x = act ? r[j] : 0.f;
-                    const float length = (float)sqrt(wave_sum((double)x * x));
-                    if (!isfinite(length)) {
-                        fit_status = kSpaiNumericalFailure;
-                    } else if (length == 0.f) {
-                        fit_status = kSpaiRankDeficient;
-                    } else {
-                        const float xj = lane_value(x, j);
-                        const float diagonal = -copysignf(length, xj);
-                        const float vnorm = sqrtf(2.f + 2.f * fabsf(xj) / length);
-                        float v = act ? x / length : 0.f;
-                        if (lane == j)
-                            v += copysignf(1.f, v);
-                        v = v / vnorm;
-                        float p[kWQ];
+                for (int c = 0; c < kWQ; ++c)
+                    x = c == j ? r[c] : x;
+                x = act ? x : 0.f;
+                const float length = __int_as_float(__builtin_amdgcn_readfirstlane(
+                    __float_as_int((float)sqrt(wave_sum((double)x * x)))));
+                if (!isfinite(length)) {
+                    fit_status = kSpaiNumericalFailure;
+                    break;
+                }
+                if (length == 0.f) {
+                    fit_status = kSpaiRankDeficient;
+                    break;
+                }
+                const float xj = lane_value(x, j);
+                const float diagonal = -copysignf(length, xj);
+                const float vnorm = sqrtf(2.f + 2.f * fabsf(xj) / length);
+                float v = act ? x / length : 0.f;
+                if (lane == j)
+                    v += copysignf(1.f, v);
+                v = v / vnorm;
+                float p[kWQ];
 #pragma unroll
-                        for (int c = 0; c < kWQ; ++c)
-                            p[c] = act && c > j ? v * r[c] : 0.f;
-                        const float d = wave_reduce_scatter16(p, lane);
-                        const float dr = wave_sum(v * rh);
-                        const float v2 = 2.f * v;
-                        bool bad = false;
+                for (int c = 0; c < kWQ; ++c)
+                    p[c] = act && c > j ? v * r[c] : 0.f;
+                const float d = wave_reduce_scatter16(p, lane);
+                const float dr = wave_sum(v * rh);
+                const float v2 = 2.f * v;
+                bool bad = false;
 #pragma unroll
-                        for (int c = j + 1; c < kWQ; ++c) {
-                            if (c < q) {
-                                const float dc = lane_value(d, 4 * c);
-                                if (act) {
-                                    r[c] -= v2 * dc;
-                                    bad = bad || !isfinite(r[c]);
-                                }
-                            }
-                        }
+                for (int c = 1; c < kWQ; ++c) {
+                    if (c > j && c < q) {
+                        const float dc = lane_value(d, 4 * c);
                         if (act) {
-                            rh -= v2 * dr;
-                            bad = bad || !isfinite(rh);
-                        }
-                        if (__any(bad)) {
-                            fit_status = kSpaiNumericalFailure;
-                        } else {
-                            if (lane == j)
-                                dg = diagonal;
-                            min_pivot = fminf(min_pivot, fabsf(diagonal));
+                            r[c] -= v2 * dc;
+                            bad = bad || !isfinite(r[c]);
                         }
                     }
                 }
+                if (act) {
+                    rh -= v2 * dr;
+                    bad = bad || !isfinite(rh);
+                }
+                if (__any(bad)) {
+                    fit_status = kSpaiNumericalFailure;
+                    break;
+                }
+                if (lane == j)
+                    dg = diagonal;
+                min_pivot = fminf(min_pivot, fabsf(diagonal));
             }
             // back substitution, one coefficient per lane (lane j: coef_j)
             float coef = 0.f;
             if (fit_status == kSpaiSuccess) {
                 float acc = rh;
-#pragma unroll
-                for (int j = kWQ - 1; j >= 0; --j) {
-                    if (j < q && fit_status == kSpaiSuccess) {
-                        int st = kSpaiSuccess;
-                        float cj = 0.f;
-                        if (lane == j) {
-                            if (!isfinite(dg))
+#pragma unroll 1
+                for (int j = q - 1; j >= 0; --j) {
+                    int st = kSpaiSuccess;
+                    float cj = 0.f;
+                    if (lane == j) {
+                        if (!isfinite(dg))
+                            st = kSpaiNumericalFailure;
+                        else if (dg == 0.f)
+                            st = kSpaiRankDeficient;
+                        else {
+                            cj = acc / dg;
+                            if (!isfinite(cj))
                                 st = kSpaiNumericalFailure;
-                            else if (dg == 0.f)
-                                st = kSpaiRankDeficient;
-                            else {
-                                cj = acc / dg;
-                                if (!isfinite(cj))
-                                    st = kSpaiNumericalFailure;
-                            }
-                            coef = cj;
                         }
-                        st = __builtin_amdgcn_readlane(st, j);
-                        if (st != kSpaiSuccess) {
-                            fit_status = st;
-                        } else {
-                            cj = lane_value(cj, j);
-                            if (lane < j)
-                                acc -= r[j] * cj;
-                        }
+                        coef = cj;
                     }
+                    st = __builtin_amdgcn_readlane(st, j);
+                    if (st != kSpaiSuccess) {
+                        fit_status = st;
+                        break;
+                    }
+                    cj = lane_value(cj, j);
+                    float rj = 0.f;
+#pragma unroll
+                    for (int c = 0; c < kWQ; ++c)
+                        rj = c == j ? r[c] : rj;
+                    if (lane < j)
+                        acc -= rj * cj;
                 }
             }
             if (fit_status != kSpaiSuccess) {
@@ -1123,12 +1157,14 @@ spai_wave_kernel(const int *__restrict__ aptr, const int *__restrict__ acol,
             // ---- candidates: union of B rows i in I, minus J (the union
             // holds J: row j of A lies in the shadow, so j is in B(I))
             wave_sync(); // residual visible to the candidate stage
-            const int nb = wave_rows(W, bptr, nullptr, irow < 0 ? 0 : irow, m, lane);
+            const int nb = __builtin_amdgcn_readfirstlane(
+                wave_rows(W, bptr, nullptr, irow < 0 ? 0 : irow, m, lane));
             if (nb < 0) {
                 status = kSpaiOverflow;
                 break;
             }
-            const int nc = wave_set<true>(W, bcol, bval, dscale, nb, q, lane);
+            const int nc =
+                __builtin_amdgcn_readfirstlane(wave_set<true>(W, bcol, bval, dscale, nb, q, lane));
             if (nc + q > kWC) {
                 status = kSpaiOverflow;
                 break;
@@ -1190,27 +1226,25 @@ spai_wave_kernel(const int *__restrict__ aptr, const int *__restrict__ acol,
                 status = kSpaiNumericalFailure;
                 break;
             }
-            // ---- the beta smallest (score, index) with score <= mean: each
-            // eligible candidate counts the eligible ones before it
-            int nsel = 0;
+            // ---- the beta smallest (score, index) with score <= mean, by
+            // argmin rounds over 64-bit keys (score bits, candidate): scores
+            // are finite and >= 0, so their bits order as their values
+            unsigned long long skey[kWC / kWave];
+            int elig = 0;
 #pragma unroll
             for (int u = 0; u < kWC / kWave; ++u) {
                 const int t = lane + u * kWave;
-                int rank = INT_MAX;
-                if (t < nc && W.score[t] <= mean) {
+                skey[u] = ~0ull;
+                if (t < nc) {
                     const float st_ = W.score[t];
-                    const int jt = W.cand[t];
-                    rank = 0;
-                    for (int o = 0; o < nc; ++o) {
-                        const float so = W.score[o];
-                        if (so <= mean && (so < st_ || (so == st_ && W.cand[o] < jt)))
-                            ++rank;
+                    if (st_ <= mean) {
+                        const unsigned bits = st_ == 0.f ? 0u : __float_as_uint(st_);
+                        skey[u] = ((unsigned long long)bits << 32) | (unsigned)W.cand[t];
+                        ++elig;
                     }
                 }
-                nsel += wave_sum_int(rank != INT_MAX ? 1 : 0);
-                if (rank < beta && rank < kWQ) // (more would overflow the pattern)
-                    W.sel[rank] = W.cand[t];
             }
+            int nsel = __builtin_amdgcn_readfirstlane(wave_sum_int(elig));
             nsel = nsel < beta ? nsel : beta;
             if (nsel == 0) {
                 status = kSpaiNoGrowth;
@@ -1219,6 +1253,23 @@ spai_wave_kernel(const int *__restrict__ aptr, const int *__restrict__ acol,
             if (q + nsel > kWQ) { // the grown pattern would not fit
                 status = kSpaiOverflow;
                 break;
+            }
+            for (int r = 0; r < nsel; ++r) {
+                unsigned long long lo = skey[0];
+#pragma unroll
+                for (int u = 1; u < kWC / kWave; ++u)
+                    lo = skey[u] < lo ? skey[u] : lo;
+#pragma unroll
+                for (int off = kWave / 2; off > 0; off >>= 1) {
+                    const unsigned long long o = __shfl_xor(lo, off, kWave);
+                    lo = o < lo ? o : lo;
+                }
+#pragma unroll
+                for (int u = 0; u < kWC / kWave; ++u)
+                    if (skey[u] == lo)
+                        skey[u] = ~0ull;
+                if (lane == 0)
+                    W.sel[r] = (int)(unsigned)(lo & 0xffffffffull);
             }
             wave_sync();
             // J <- sorted(J u selection)
@@ -1267,6 +1318,9 @@ void launch_spai_wave(hipStream_t s, int blocks, const int *aptr, const int *aco
 }
 int spai_wave_qcap() {
     return kWQ;
+}
+int spai_wave_max_n() {
+    return (1 << kWKeyColBits) - 1;
 }
 
 // ================================================================ block kernel
