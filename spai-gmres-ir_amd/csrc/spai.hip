@@ -713,11 +713,129 @@ __device__ inline int wave_scan_excl(int v, int lane) {
     }
     return x - v;
 }
-__device__ inline int wave_sum_int(int v) {
+// Cross-lane moves without the LDS crossbar (gfx950; every lane active):
+//   DPP quad_perm gives lane i the value of lane i^1, i^2 or i^3; DPP
+//   row_half_mirror lane i^7 (within 8 lanes); DPP row_ror:8 lane i^8 (within
+//   16 lanes);
+//   permlane16_swap(x, y) trades rows 1 and 3 of x (16 lanes each) for rows 0
+//   and 2 of y; permlane32_swap(x, y) trades lanes 32..63 of x for lanes 0..31
+//   of y.
+constexpr int kDppXor1 = 0xB1; // quad_perm [1, 0, 3, 2]
+constexpr int kDppXor2 = 0x4E; // quad_perm [2, 3, 0, 1]
+constexpr int kDppXor3 = 0x1B; // quad_perm [3, 2, 1, 0]
+constexpr int kDppXor7 = 0x141; // row_half_mirror
+constexpr int kDppXor8 = 0x128; // row_ror:8
+template <int kCtrl> __device__ inline unsigned dpp_u(unsigned v) {
+    return (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, kCtrl, 0xf, 0xf, false);
+}
+__device__ inline void swap16(unsigned &x, unsigned &y) {
+    const auto r = __builtin_amdgcn_permlane16_swap(x, y, false, false);
+    x = r[0];
+    y = r[1];
+}
+__device__ inline void swap32(unsigned &x, unsigned &y) {
+    const auto r = __builtin_amdgcn_permlane32_swap(x, y, false, false);
+    x = r[0];
+    y = r[1];
+}
+// The value of lane (lane ^ m), m a power of two below 64 (wave-uniform).
+__device__ inline unsigned xor_lane(unsigned v, int m, int lane) {
+    switch (m) {
+    case 1:
+        return dpp_u<kDppXor1>(v);
+    case 2:
+        return dpp_u<kDppXor2>(v);
+    case 4:
+        return dpp_u<kDppXor3>(dpp_u<kDppXor7>(v)); // i ^ 7 ^ 3
+    case 8:
+        return dpp_u<kDppXor8>(v);
+    case 16: {
+        unsigned x = v, y = v;
+        swap16(x, y); // x: rows 0 0 2 2, y: rows 1 1 3 3
+        return (lane & 16) ? x : y;
+    }
+    default: {
+        unsigned x = v, y = v;
+        swap32(x, y); // x: lanes 0..31 twice, y: lanes 32..63 twice
+        return (lane & 32) ? x : y;
+    }
+    }
+}
+template <int kCtrl> __device__ inline float dpp_f(float v) {
+    return __uint_as_float(dpp_u<kCtrl>(__float_as_uint(v)));
+}
+// Sums over the wave, every lane ending with the same bits: each step adds a
+// lane's value and its partner's, which both lanes of a pair form alike.
+__device__ inline float wave_sum_f(float v) {
+    v += dpp_f<kDppXor1>(v);
+    v += dpp_f<kDppXor2>(v);
+    v += dpp_f<kDppXor7>(v);
+    v += dpp_f<kDppXor8>(v);
+    unsigned x = __float_as_uint(v), y = x;
+    swap16(x, y);
+    v = __uint_as_float(x) + __uint_as_float(y);
+    x = __float_as_uint(v);
+    y = x;
+    swap32(x, y);
+    return __uint_as_float(x) + __uint_as_float(y);
+}
+template <int kCtrl> __device__ inline double dpp_d(double v) {
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    const unsigned lo = dpp_u<kCtrl>((unsigned)b), hi = dpp_u<kCtrl>((unsigned)(b >> 32));
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+__device__ inline double wave_sum_d(double v) {
+    v += dpp_d<kDppXor1>(v);
+    v += dpp_d<kDppXor2>(v);
+    v += dpp_d<kDppXor7>(v);
+    v += dpp_d<kDppXor8>(v);
 #pragma unroll
-    for (int off = kWave / 2; off > 0; off >>= 1)
-        v += __shfl_xor(v, off, kWave);
+    for (int s = 0; s < 2; ++s) {
+        const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+        unsigned xl_ = (unsigned)b, yl = xl_, xh = (unsigned)(b >> 32), yh = xh;
+        if (s == 0) {
+            swap16(xl_, yl);
+            swap16(xh, yh);
+        } else {
+            swap32(xl_, yl);
+            swap32(xh, yh);
+        }
+        v = __longlong_as_double((long long)(((unsigned long long)xh << 32) | xl_)) +
+            __longlong_as_double((long long)(((unsigned long long)yh << 32) | yl));
+    }
     return v;
+}
+__device__ inline int wave_sum_int(int v) {
+    v += (int)dpp_u<kDppXor1>((unsigned)v);
+    v += (int)dpp_u<kDppXor2>((unsigned)v);
+    v += (int)dpp_u<kDppXor7>((unsigned)v);
+    v += (int)dpp_u<kDppXor8>((unsigned)v);
+    unsigned x = (unsigned)v, y = x;
+    swap16(x, y);
+    v = (int)(x + y);
+    x = (unsigned)v;
+    y = x;
+    swap32(x, y);
+    return (int)(x + y);
+}
+// Maximum over the wave (operands finite or +inf and >= +0: every lane ends
+// with the same bits).
+__device__ inline float wave_max_f(float v) {
+    float o = dpp_f<kDppXor1>(v);
+    v = o > v ? o : v;
+    o = dpp_f<kDppXor2>(v);
+    v = o > v ? o : v;
+    o = dpp_f<kDppXor7>(v);
+    v = o > v ? o : v;
+    o = dpp_f<kDppXor8>(v);
+    v = o > v ? o : v;
+    unsigned x = __float_as_uint(v), y = x;
+    swap16(x, y);
+    v = __uint_as_float(x) > __uint_as_float(y) ? __uint_as_float(x) : __uint_as_float(y);
+    x = __float_as_uint(v);
+    y = x;
+    swap32(x, y);
+    return __uint_as_float(x) > __uint_as_float(y) ? __uint_as_float(x) : __uint_as_float(y);
 }
 __device__ inline float lane_value(float v, int src) {
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), src));
@@ -908,9 +1026,11 @@ __device__ int wave_sort_set(WaveLds &W, const int *__restrict__ col,
     return nset;
 }
 
+// (kept out of line: the sorts' registers then do not add to the kernel's)
 template <bool kCand>
-__device__ int wave_set(WaveLds &W, const int *col, const float *val, const float *dscale,
-                        int total, int q, int lane) {
+__device__ __attribute__((noinline)) int wave_set(WaveLds &W, const int *col, const float *val,
+                                                  const float *dscale, int total, int q,
+                                                  int lane) {
     if (total <= 0) // (rows of A and B in these stages always hold an entry)
         return 0;
     if (total <= kWave)
@@ -923,28 +1043,37 @@ __device__ int wave_set(WaveLds &W, const int *col, const float *val, const floa
 }
 
 // Sums a[c] (c < 16) over the wave: lane l ends with the sum of column
-// (l >> 2) & 15 (halving exchanges at distances 32, 16, 8, 4, then 2 and 1).
+// (l >> 2) & 15.  Halving exchanges: lanes 32 apart, then 16 apart, trade
+// halves through the permlane swaps (after a swap both halves add alike);
+// then lanes 8 apart (DPP row_ror:8) and lanes i and i^7 (row_half_mirror)
+// each form both candidate sums and keep theirs; then the quad is summed.
 __device__ inline float wave_reduce_scatter16(float (&a)[kWQ], int lane) {
 #pragma unroll
-    for (int s = 0; s < 4; ++s) {
-        const int half = 8 >> s;
-        const int mask = 32 >> s;
-        const bool hi = (lane & mask) != 0;
+    for (int i = 0; i < 8; ++i) {
+        unsigned x = __float_as_uint(a[i]), y = __float_as_uint(a[i + 8]);
+        swap32(x, y);
+        a[i] = __uint_as_float(x) + __uint_as_float(y);
+    }
 #pragma unroll
-        for (int i = 0; i < half; ++i) {
-            float lo = a[i], up = a[i + half];
-            // (an empty asm keeps both values in registers: otherwise the
-            // two selects are folded into one load at a lane-dependent
-            // index, lowered as a 16-way compare-and-select chain)
-            asm volatile("" : "+v"(lo), "+v"(up));
-            const float keep = hi ? up : lo;
-            const float send = hi ? lo : up;
-            a[i] = keep + __shfl_xor(send, mask, kWave);
-        }
+    for (int i = 0; i < 4; ++i) {
+        unsigned x = __float_as_uint(a[i]), y = __float_as_uint(a[i + 4]);
+        swap16(x, y);
+        a[i] = __uint_as_float(x) + __uint_as_float(y);
+    }
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+        const float u = a[i] + dpp_f<kDppXor8>(a[i]);
+        const float w = a[i + 2] + dpp_f<kDppXor8>(a[i + 2]);
+        a[i] = (lane & 8) ? w : u;
+    }
+    {
+        const float u = a[0] + dpp_f<kDppXor7>(a[0]);
+        const float w = a[1] + dpp_f<kDppXor7>(a[1]);
+        a[0] = (lane & 4) ? w : u;
     }
     float x = a[0];
-    x += __shfl_xor(x, 2, kWave);
-    x += __shfl_xor(x, 1, kWave);
+    x += dpp_f<kDppXor1>(x);
+    x += dpp_f<kDppXor2>(x);
     return x;
 }
 
@@ -1013,7 +1142,7 @@ spai_wave_kernel(const int *__restrict__ aptr, const int *__restrict__ acol,
 #pragma unroll
             for (int c = 0; c < kWQ; ++c)
                 rowsum += fabsf(r[c]);
-            const float inf_norm = wave_max(rowsum);
+            const float inf_norm = wave_max_f(rowsum);
             float min_pivot = INFINITY, dg = 0.f;
             int fit_status = kSpaiSuccess;
             // One rolled loop over the steps: column j of the row is picked
@@ -1031,7 +1160,7 @@ spai_wave_kernel(const int *__restrict__ aptr, const int *__restrict__ acol,
                     x = c == j ? r[c] : x;
                 x = act ? x : 0.f;
                 const float length = __int_as_float(__builtin_amdgcn_readfirstlane(
-                    __float_as_int((float)sqrt(wave_sum((double)x * x)))));
+                    __float_as_int((float)sqrt(wave_sum_d((double)x * x)))));
                 if (!isfinite(length)) {
                     fit_status = kSpaiNumericalFailure;
                     break;
@@ -1052,7 +1181,7 @@ spai_wave_kernel(const int *__restrict__ aptr, const int *__restrict__ acol,
                 for (int c = 0; c < kWQ; ++c)
                     p[c] = act && c > j ? v * r[c] : 0.f;
                 const float d = wave_reduce_scatter16(p, lane);
-                const float dr = wave_sum(v * rh);
+                const float dr = wave_sum_f(v * rh);
                 const float v2 = 2.f * v;
                 bool bad = false;
 #pragma unroll
@@ -1127,7 +1256,7 @@ spai_wave_kernel(const int *__restrict__ aptr, const int *__restrict__ acol,
             if (lane >= m)
                 sres = 0.f;
             W.res[lane] = sres;
-            const float rnorm = (float)sqrt(wave_sum((double)sres * sres));
+            const float rnorm = (float)sqrt(wave_sum_d((double)sres * sres));
             if (!isfinite(rnorm)) {
                 status = kSpaiNumericalFailure;
                 break;
