@@ -14,6 +14,7 @@
  *
  * Reference interfaces replaced (reference tree, file:line):
  *   bspai_hip_matrix_create / bspai_hip_spmv    CudaCsrSpmv      include/bspai/cuda_spmv.hpp:7-19
+ *   bspai_hip_csr_from_entries                  CsrMatrix::from_entries include/bspai/cpu.hpp:19
  *   bspai_hip_spai_build (+ _info/_columns/...) cuda_construct_spai include/bspai/cuda_spai.hpp:47
  *                                               construct_spai    include/bspai/spai.hpp:69
  *   bspai_hip_score                             cuda_evaluate_spai_score include/bspai/cuda_spai.hpp:48-49
@@ -159,6 +160,23 @@ int bspai_hip_matrix_download(bspai_hip_ctx *ctx, const bspai_hip_matrix *m, int
                               int32_t *col, double *val);
 int bspai_hip_matrix_transpose(bspai_hip_ctx *ctx, const bspai_hip_matrix *m,
                                bspai_hip_matrix **out);
+
+/* Canonical rows x cols CSR from `count` coordinate entries (row, col, value),
+ * on the device.  Replaces CsrMatrix::from_entries (reference
+ * include/bspai/cpu.hpp:19, src/cpu.cpp:48-76) with the same result: entries
+ * sorted by (row, column), duplicates summed in input order, exact zeros
+ * dropped.  Errors (BSPAI_HIP_EINVAL, with the reference's messages): an index
+ * outside the matrix or a non-finite value ("Entry index outside matrix
+ * dimensions" / "Matrix entries must be finite", the first offending entry in
+ * input order deciding, its indices checked before its value), then a
+ * duplicate sum that becomes non-finite ("Duplicate sum is nonfinite").
+ * row, col, val: `count` entries in `where` memory.  Outputs in `out_where`
+ * memory: ptr (rows + 1 entries), col_out and val_out (room for `count`; the
+ * first *nnz are written).  rows, cols and count below 2^31 - 1. */
+int bspai_hip_csr_from_entries(bspai_hip_ctx *ctx, int64_t rows, int64_t cols, int64_t count,
+                               const int64_t *row, const int64_t *col, const double *val,
+                               int where, int32_t *ptr, int32_t *col_out, double *val_out,
+                               int out_where, int64_t *nnz);
 
 /* y = A x in FP64.  lanes: 0 = the matrix's default; 1 = streaming kernel
  * (the block's entries staged in LDS); -1 = one thread per row; 2..64 (a

@@ -7,7 +7,7 @@
 // It restates the reference's single-precision-free CPU algorithm operation by
 // operation, in the same floating-point order, so that its results are
 // bit-identical to the reference's `cpu_fp64_ir`:
-//   canonical CSR            src/cpu.cpp:49-103
+//   canonical CSR            src/cpu.cpp:48-103
 //   SPAI construction        src/spai.cpp:15-374
 //   left-preconditioned GMRES src/gmres.cpp:12-262
 //   iterative refinement     src/ir.cpp:13-227
@@ -31,6 +31,7 @@
 #include <cstdint>
 #include <cstring>
 #include <limits>
+#include <memory>
 #include <numeric>
 #include <stdexcept>
 #include <string>
@@ -1113,6 +1114,70 @@ void oracle_ir_p(const OracleIr *o, std::int64_t *ptr, std::int64_t *col, double
     std::copy(o->r.spai.p.ptr.begin(), o->r.spai.p.ptr.end(), ptr);
     std::copy(o->r.spai.p.col.begin(), o->r.spai.p.col.end(), col);
     std::copy(o->r.spai.p.val.begin(), o->r.spai.p.val.end(), val);
+}
+
+// Canonical CSR from coordinate entries (src/cpu.cpp:48-76): every entry's
+// indices, then its value, are checked in input order; a stable sort by
+// (row, column) keeps duplicates in input order; each run of duplicates is
+// summed left to right, a non-finite partial sum rejected; zero sums are
+// dropped.  Returns nullptr with the reference's message on error.
+struct OracleCsr {
+    std::vector<std::int64_t> ptr, col;
+    std::vector<double> val;
+};
+OracleCsr *oracle_from_entries(std::int64_t rows, std::int64_t cols, std::int64_t count,
+                               const std::int64_t *r, const std::int64_t *c, const double *v) {
+    try {
+        struct Entry {
+            std::int64_t row, col;
+            double value;
+        };
+        std::vector<Entry> e((std::size_t)count);
+        for (std::int64_t i = 0; i < count; ++i) {
+            if (r[i] < 0 || r[i] >= rows || c[i] < 0 || c[i] >= cols)
+                throw std::invalid_argument("Entry index outside matrix dimensions");
+            if (!std::isfinite(v[i]))
+                throw std::invalid_argument("Matrix entries must be finite");
+            e[(std::size_t)i] = Entry{r[i], c[i], v[i]};
+        }
+        std::stable_sort(e.begin(), e.end(), [](const Entry &x, const Entry &y) {
+            return x.row != y.row ? x.row < y.row : x.col < y.col;
+        });
+        std::unique_ptr<OracleCsr> o(new OracleCsr);
+        o->ptr.assign((std::size_t)rows + 1, 0);
+        std::size_t i = 0;
+        while (i < e.size()) {
+            const Entry head = e[i];
+            double sum = head.value;
+            for (++i; i < e.size() && e[i].row == head.row && e[i].col == head.col; ++i) {
+                sum += e[i].value;
+                if (!std::isfinite(sum))
+                    throw std::invalid_argument("Duplicate sum is nonfinite");
+            }
+            if (sum != 0.0) {
+                o->col.push_back(head.col);
+                o->val.push_back(sum);
+                ++o->ptr[(std::size_t)head.row + 1];
+            }
+        }
+        for (std::int64_t k = 0; k < rows; ++k)
+            o->ptr[(std::size_t)k + 1] += o->ptr[(std::size_t)k];
+        return o.release();
+    } catch (const std::exception &ex) {
+        g_error = ex.what();
+        return nullptr;
+    }
+}
+std::int64_t oracle_csr_nnz(const OracleCsr *o) {
+    return (std::int64_t)o->col.size();
+}
+void oracle_csr_get(const OracleCsr *o, std::int64_t *ptr, std::int64_t *col, double *val) {
+    std::copy(o->ptr.begin(), o->ptr.end(), ptr);
+    std::copy(o->col.begin(), o->col.end(), col);
+    std::copy(o->val.begin(), o->val.end(), val);
+}
+void oracle_csr_free(OracleCsr *o) {
+    delete o;
 }
 
 } // extern "C"

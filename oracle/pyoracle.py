@@ -85,6 +85,12 @@ def lib():
         L.oracle_spai_subset.argtypes = [C.c_int64, i64p, i64p, f64p, C.c_double, C.c_int64,
                                          C.c_int64, C.c_int64, i64p, i32p, i32p, i32p, i64p, i64p,
                                          f64p, C.c_int64]
+        L.oracle_from_entries.restype = P
+        L.oracle_from_entries.argtypes = [C.c_int64, C.c_int64, C.c_int64, i64p, i64p, f64p]
+        L.oracle_csr_nnz.argtypes = [P]
+        L.oracle_csr_nnz.restype = C.c_int64
+        L.oracle_csr_get.argtypes = [P, i64p, i64p, f64p]
+        L.oracle_csr_free.argtypes = [P]
         L.oracle_last_error.restype = C.c_char_p
         _lib = L
     return _lib
@@ -222,6 +228,29 @@ def score(rr, dot, den):
     cl = C.c_int()
     s = lib().oracle_score(rr, dot, den, C.byref(st), C.byref(cl))
     return SPAI_STATUS[st.value], s, bool(cl.value)
+
+
+def from_entries(rows, cols, row, col, val):
+    """reference CsrMatrix::from_entries (src/cpu.cpp:48-76): returns (ptr,
+    col, val) as int64/int64/float64 arrays; raises ValueError with the
+    reference's message."""
+    L = lib()
+    r = np.ascontiguousarray(row, dtype=np.int64)
+    c = np.ascontiguousarray(col, dtype=np.int64)
+    v = np.ascontiguousarray(val, dtype=np.float64)
+    o = L.oracle_from_entries(rows, cols, r.size, _p(r, C.c_int64), _p(c, C.c_int64),
+                              _p(v, C.c_double))
+    if not o:
+        raise ValueError(L.oracle_last_error().decode())
+    try:
+        nnz = L.oracle_csr_nnz(o)
+        ptr = np.zeros(rows + 1, dtype=np.int64)
+        oc = np.zeros(max(nnz, 1), dtype=np.int64)
+        ov = np.zeros(max(nnz, 1), dtype=np.float64)
+        L.oracle_csr_get(o, _p(ptr, C.c_int64), _p(oc, C.c_int64), _p(ov, C.c_double))
+        return ptr, oc[:nnz], ov[:nnz]
+    finally:
+        L.oracle_csr_free(o)
 
 
 def hypot_chain(v) -> float:

@@ -92,6 +92,9 @@ SIGNATURES = {
     "bspai_hip_matrix_set_values": (C.c_int, [_VP, _VP, _VP, C.c_int]),
     "bspai_hip_matrix_download": (C.c_int, [_VP, _VP, _VP, _VP, _VP]),
     "bspai_hip_matrix_transpose": (C.c_int, [_VP, _VP, _PP]),
+    "bspai_hip_csr_from_entries": (C.c_int, [_VP, C.c_int64, C.c_int64, C.c_int64, _VP, _VP,
+                                             _VP, C.c_int, _VP, _VP, _VP, C.c_int,
+                                             C.POINTER(C.c_int64)]),
     "bspai_hip_spmv": (C.c_int, [_VP, _VP, _VP, _VP, C.c_int, C.c_int]),
     "bspai_hip_spmv_f32": (C.c_int, [_VP, _VP, _VP, _VP, C.c_int, C.c_int]),
     "bspai_hip_spai_build": (C.c_int, [_VP, _VP, C.POINTER(SpaiOptions), _PP]),
@@ -132,6 +135,14 @@ def load():
         raise LibraryMissing(
             f"{LIB_PATH} is missing: build it with `make -C spai-gmres-ir_amd` "
             "(or __graft_entry__.build()); there is no fallback path")
+    # PyTorch for ROCm ships its own HIP and HSA runtimes (sonames
+    # libamdhip64.so.7, ...).  Loaded first, the library binds to those, and
+    # one runtime serves both; loaded before torch, it would pull in the
+    # system runtime, and torch's own second runtime then finds no GPU.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = C.CDLL(LIB_PATH)
     for name, (res, args) in SIGNATURES.items():
         fn = getattr(L, name)

@@ -113,8 +113,47 @@ def default_context() -> Context:
     return _default_ctx
 
 
+def csr_from_entries(rows: int, cols: int, row, col, val, ctx: Context | None = None):
+    """Canonical CSR from coordinate entries on the GPU (reference
+    CsrMatrix::from_entries, src/cpu.cpp:48-76): sorted by (row, column),
+    duplicates summed in input order, exact zeros dropped.  Returns (ptr, col,
+    val): int32/int32/float64 numpy arrays for host inputs, torch tensors on
+    the inputs' device for device inputs.  Raises ValueError with the
+    reference's message for an index outside the matrix, a non-finite value
+    or a non-finite duplicate sum."""
+    ctx = ctx or default_context()
+    r = _host(row, np.int64)
+    c = _host(col, np.int64)
+    v = _host(val, np.float64)
+    count = len(v) if isinstance(v, np.ndarray) else v.numel()
+    where = _where(v)
+    if where == DEVICE:
+        import torch
+
+        dev = v.device
+        ptr = torch.empty(rows + 1, dtype=torch.int32, device=dev)
+        oc = torch.empty(max(count, 1), dtype=torch.int32, device=dev)
+        ov = torch.empty(max(count, 1), dtype=torch.float64, device=dev)
+    else:
+        ptr = np.empty(rows + 1, np.int32)
+        oc = np.empty(max(count, 1), np.int32)
+        ov = np.empty(max(count, 1))
+    nnz = C.c_int64()
+    check(_lib.load().bspai_hip_csr_from_entries(
+        ctx.h, rows, cols, count, _addr(r), _addr(c), _addr(v), where, _addr(ptr), _addr(oc),
+        _addr(ov), where, C.byref(nnz)), ctx.h)
+    k = nnz.value
+    return ptr, oc[:k], ov[:k]
+
+
 class Matrix:
     """Device CSR (int32 structure, FP64 values); optional block-diagonal batch."""
+
+    @classmethod
+    def from_entries(cls, n: int, row, col, val, ctx: Context | None = None) -> "Matrix":
+        """n x n matrix from coordinate entries, canonicalized on the GPU."""
+        ptr, c, v = csr_from_entries(n, n, row, col, val, ctx)
+        return cls((ptr, c, v), ctx)
 
     def __init__(self, a, ctx: Context | None = None, systems=None, _handle=None):
         self.ctx = ctx or default_context()

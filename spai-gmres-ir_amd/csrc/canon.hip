@@ -1,0 +1,184 @@
+// Canonical CSR from coordinate entries on the device: the reference's
+// CsrMatrix::from_entries (src/cpu.cpp:48-76).
+//
+//   1. entries_check_kernel: every entry's index range and finiteness; the
+//      first offending entry in input order (atomicMin over its position)
+//      decides the error, as the reference's validation loop does; each valid
+//      entry gets the 64-bit sort key row * cols + col.
+//   2. a stable radix sort of (key, value) pairs (hipCUB): least-significant-
+//      digit radix sort keeps equal keys in input order, which is the
+//      reference's accumulation order for duplicates (its std::stable_sort).
+//   3. run_sums_kernel: the first entry of each run of equal keys adds the
+//      run's values one by one in that order, flags a non-finite partial sum
+//      and keeps the run when its sum is not zero (+0 and -0 both drop).
+//   4. an inclusive scan of the kept flags numbers the kept runs;
+//      scatter_kernel writes each one's column and value and counts its row;
+//      a scan of the row counts gives the row offsets.
+// The whole pass streams the entries a few times (HBM-bound); the sort is
+// the bulk of it.
+#include "engine.hpp"
+#include "kernels.hpp"
+
+#include <hipcub/hipcub.hpp>
+
+#include <climits>
+#include <stdexcept>
+
+namespace bspai_hip {
+
+namespace {
+
+constexpr int kCanonThreads = 256;
+
+__global__ void __launch_bounds__(kCanonThreads)
+entries_check_kernel(std::int64_t count, std::int64_t rows, std::int64_t cols,
+                     const std::int64_t *__restrict__ row, const std::int64_t *__restrict__ col,
+                     const double *__restrict__ val, unsigned long long *__restrict__ keys,
+                     unsigned long long *__restrict__ first_bad) {
+    const std::int64_t stride = (std::int64_t)gridDim.x * blockDim.x;
+    unsigned long long bad = ULLONG_MAX;
+    for (std::int64_t i = (std::int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count;
+         i += stride) {
+        const std::int64_t r = row[i], c = col[i];
+        const bool ok = r >= 0 && r < rows && c >= 0 && c < cols && isfinite(val[i]);
+        keys[i] = ok ? (unsigned long long)r * (unsigned long long)cols + (unsigned long long)c
+                     : 0ull;
+        if (!ok && (unsigned long long)i < bad)
+            bad = (unsigned long long)i;
+    }
+    if (bad != ULLONG_MAX)
+        atomicMin(first_bad, bad);
+}
+
+__global__ void __launch_bounds__(kCanonThreads)
+run_sums_kernel(std::int64_t count, const unsigned long long *__restrict__ keys,
+                const double *__restrict__ val, int *__restrict__ keep,
+                double *__restrict__ sums, int *__restrict__ nonfinite) {
+    const std::int64_t stride = (std::int64_t)gridDim.x * blockDim.x;
+    for (std::int64_t i = (std::int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count;
+         i += stride) {
+        const unsigned long long k = keys[i];
+        int kept = 0;
+        if (i == 0 || keys[i - 1] != k) {
+            double sum = val[i];
+            bool finite = true;
+            for (std::int64_t j = i + 1; j < count && keys[j] == k; ++j) {
+                sum += val[j];
+                finite = finite && isfinite(sum);
+            }
+            if (!finite)
+                *nonfinite = 1;
+            kept = sum != 0.0 ? 1 : 0;
+            sums[i] = sum;
+        }
+        keep[i] = kept;
+    }
+}
+
+__global__ void __launch_bounds__(kCanonThreads)
+scatter_kernel(std::int64_t count, std::int64_t cols, const unsigned long long *__restrict__ keys,
+               const int *__restrict__ keep, const int *__restrict__ slot,
+               const double *__restrict__ sums, int *__restrict__ out_col,
+               double *__restrict__ out_val, int *__restrict__ row_count) {
+    const std::int64_t stride = (std::int64_t)gridDim.x * blockDim.x;
+    for (std::int64_t i = (std::int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count;
+         i += stride) {
+        if (!keep[i])
+            continue;
+        const unsigned long long k = keys[i];
+        const unsigned long long r = k / (unsigned long long)cols;
+        const int s = slot[i] - 1;
+        out_col[s] = (int)(k - r * (unsigned long long)cols);
+        out_val[s] = sums[i];
+        atomicAdd(row_count + r + 1, 1);
+    }
+}
+
+int canon_blocks(std::int64_t count) {
+    const std::int64_t b = (count + kCanonThreads - 1) / kCanonThreads;
+    return (int)std::max<std::int64_t>(1, std::min<std::int64_t>(b, 1 << 16));
+}
+
+} // namespace
+
+std::int64_t csr_from_entries(Context &c, std::int64_t rows, std::int64_t cols,
+                              std::int64_t count, const std::int64_t *row,
+                              const std::int64_t *col, const double *val, DeviceArray<int> &ptr,
+                              DeviceArray<int> &out_col, DeviceArray<double> &out_val) {
+    const hipStream_t s = c.stream;
+    ptr.resize(rows + 1);
+    out_col.resize(std::max<std::int64_t>(count, 1));
+    out_val.resize(std::max<std::int64_t>(count, 1));
+    HIP_CHECK(hipMemsetAsync(ptr.data(), 0, (rows + 1) * sizeof(int), s));
+    if (count == 0) {
+        HIP_CHECK(hipStreamSynchronize(s));
+        return 0;
+    }
+    const int blocks = canon_blocks(count);
+    DeviceArray<unsigned long long> keys(count), keys_sorted(count), first_bad(1);
+    DeviceArray<double> vals_sorted(count);
+    HIP_CHECK(hipMemsetAsync(first_bad.data(), 0xff, sizeof(unsigned long long), s));
+    const double entry_bytes = 2.0 * 8 + 8; // row, col, value
+    timed(c, "entries_check_kernel", (entry_bytes + 8) * (double)count, 0.0, [&] {
+        hipLaunchKernelGGL(entries_check_kernel, dim3(blocks), dim3(kCanonThreads), 0, s, count,
+                           rows, cols, row, col, val, keys.data(), first_bad.data());
+        HIP_LAUNCH_CHECK();
+    });
+    unsigned long long bad = 0;
+    HIP_CHECK(hipMemcpyAsync(&bad, first_bad.data(), sizeof bad, hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    if (bad != ULLONG_MAX) {
+        // The reference checks the indices of an entry before its value.
+        std::int64_t r = 0, cc = 0;
+        HIP_CHECK(hipMemcpy(&r, row + bad, sizeof r, hipMemcpyDeviceToHost));
+        HIP_CHECK(hipMemcpy(&cc, col + bad, sizeof cc, hipMemcpyDeviceToHost));
+        if (r < 0 || r >= rows || cc < 0 || cc >= cols)
+            throw std::invalid_argument("Entry index outside matrix dimensions");
+        throw std::invalid_argument("Matrix entries must be finite");
+    }
+    // Sort on the key bits in use only.
+    const unsigned long long max_key =
+        (unsigned long long)rows * (unsigned long long)cols - 1ull;
+    const int end_bit = max_key == 0 ? 1 : 64 - __builtin_clzll(max_key);
+    std::size_t tmp_bytes = 0;
+    HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, keys.data(),
+                                                 keys_sorted.data(), val, vals_sorted.data(),
+                                                 (int)count, 0, end_bit, s));
+    DeviceArray<unsigned char> tmp(std::max<std::size_t>(tmp_bytes, 1));
+    timed(c, "entries_radix_sort", 2.0 * 2.0 * 16.0 * (double)count, 0.0, [&] {
+        HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(tmp.data(), tmp_bytes, keys.data(),
+                                                     keys_sorted.data(), val, vals_sorted.data(),
+                                                     (int)count, 0, end_bit, s));
+    });
+    DeviceArray<int> keep(count), slot(count), nonfinite(1), scan_tmp(scan_tmp_ints(count));
+    DeviceArray<double> sums(count);
+    HIP_CHECK(hipMemsetAsync(nonfinite.data(), 0, sizeof(int), s));
+    timed(c, "run_sums_kernel", 16.0 * (double)count + 12.0 * (double)count, 0.0, [&] {
+        hipLaunchKernelGGL(run_sums_kernel, dim3(blocks), dim3(kCanonThreads), 0, s, count,
+                           keys_sorted.data(), vals_sorted.data(), keep.data(), sums.data(),
+                           nonfinite.data());
+        HIP_LAUNCH_CHECK();
+    });
+    HIP_CHECK(hipMemcpyAsync(slot.data(), keep.data(), count * sizeof(int),
+                             hipMemcpyDeviceToDevice, s));
+    scan_inclusive_int(s, slot.data(), count, scan_tmp.data());
+    int flag = 0, nnz = 0;
+    HIP_CHECK(hipMemcpyAsync(&flag, nonfinite.data(), sizeof flag, hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipMemcpyAsync(&nnz, slot.data() + (count - 1), sizeof nnz, hipMemcpyDeviceToHost,
+                             s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    if (flag)
+        throw std::invalid_argument("Duplicate sum is nonfinite");
+    timed(c, "scatter_kernel", 8.0 * (double)count + 12.0 * (double)nnz, 0.0, [&] {
+        hipLaunchKernelGGL(scatter_kernel, dim3(blocks), dim3(kCanonThreads), 0, s, count, cols,
+                           keys_sorted.data(), keep.data(), slot.data(), sums.data(),
+                           out_col.data(), out_val.data(), ptr.data());
+        HIP_LAUNCH_CHECK();
+    });
+    DeviceArray<int> row_tmp(scan_tmp_ints(rows + 1));
+    scan_inclusive_int(s, ptr.data(), rows + 1, row_tmp.data());
+    HIP_CHECK(hipStreamSynchronize(s));
+    return nnz;
+}
+
+} // namespace bspai_hip

@@ -419,6 +419,34 @@ int bspai_hip_matrix_download(bspai_hip_ctx *ctx, const bspai_hip_matrix *m, int
     });
 }
 
+int bspai_hip_csr_from_entries(bspai_hip_ctx *ctx, int64_t rows, int64_t cols, int64_t count,
+                               const int64_t *row, const int64_t *col, const double *val,
+                               int where, int32_t *ptr, int32_t *col_out, double *val_out,
+                               int out_where, int64_t *nnz) {
+    return guard(ctx, [&] {
+        require(ctx && ptr && nnz, "null argument");
+        require(count == 0 || (row && col && val && col_out && val_out), "null argument");
+        require(rows >= 0 && rows < INT_MAX && cols >= 0 && cols < INT_MAX && count >= 0 &&
+                    count < INT_MAX,
+                "matrix size out of range");
+        Context &c = ctx->c;
+        In<int64_t> r(c, row, count, where), cl(c, col, count, where);
+        In<double> v(c, val, count, where);
+        DeviceArray<int> p, oc;
+        DeviceArray<double> ov;
+        const int64_t k = csr_from_entries(c, rows, cols, count, r.p, cl.p, v.p, p, oc, ov);
+        const hipMemcpyKind kind =
+            out_where == BSPAI_HIP_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+        HIP_CHECK(hipMemcpyAsync(ptr, p.data(), (rows + 1) * sizeof(int), kind, c.stream));
+        if (k > 0) {
+            HIP_CHECK(hipMemcpyAsync(col_out, oc.data(), k * sizeof(int), kind, c.stream));
+            HIP_CHECK(hipMemcpyAsync(val_out, ov.data(), k * sizeof(double), kind, c.stream));
+        }
+        HIP_CHECK(hipStreamSynchronize(c.stream));
+        *nnz = k;
+    });
+}
+
 int bspai_hip_matrix_transpose(bspai_hip_ctx *ctx, const bspai_hip_matrix *m,
                                bspai_hip_matrix **out) {
     return guard(ctx, [&] {

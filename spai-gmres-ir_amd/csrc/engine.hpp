@@ -270,4 +270,13 @@ void launch_sddmm(hipStream_t s, const Tiles &tiles, const int *ptr, const int *
 void spai_score_on_device(hipStream_t s, double rr, double dot, double den, int *status,
                           int *clamped, double *score);
 
+// Canonical rows x cols CSR from `count` coordinate entries held on the
+// device (reference CsrMatrix::from_entries, src/cpu.cpp:48-76).  Returns
+// nnz; ptr, out_col and out_val are sized here.  Throws std::invalid_argument
+// with the reference's messages for an index out of range, a non-finite value
+// or a non-finite duplicate sum.
+std::int64_t csr_from_entries(Context &c, std::int64_t rows, std::int64_t cols,
+                              std::int64_t count, const std::int64_t *row,
+                              const std::int64_t *col, const double *val, DeviceArray<int> &ptr,
+                              DeviceArray<int> &out_col, DeviceArray<double> &out_val);
 } // namespace bspai_hip
