@@ -8,7 +8,7 @@ stop (outercap 20, innercap 200).  `value` is the mixed FP32/FP64 path's
 solves/s; the plain FP64 path is timed in the same run and reported beside it.
 Inputs (matrix, right side) are resident in HBM before the timed region.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--extra many,powerlaw]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--extra many,powerlaw,canon]
        python bench.py --pmc-workload      (the short run profiled for HBM traffic)
 """
 from __future__ import annotations
@@ -38,7 +38,8 @@ def parse():
                     help="GMRES iterations in the CPU-baseline sample")
     ap.add_argument("--extra", default="",
                     help="comma list of extra workloads to time beside the headline: "
-                         "'many' (256 systems batched vs looped), 'powerlaw' (uneven columns)")
+                         "'many' (256 systems batched vs looped), 'powerlaw' (uneven columns), "
+                         "'canon' (canonical CSR from 29M coordinate entries)")
     ap.add_argument("--powerlaw-n", type=int, default=50000,
                     help="rows of the power-law matrix timed by --extra powerlaw")
     ap.add_argument("--pmc-workload", action="store_true",
@@ -232,6 +233,8 @@ def main():
         line["extra"]["many_systems"] = bench_many(ctx)
     if "powerlaw" in extras:
         line["extra"]["power_law"] = bench_power_law(ctx, args.powerlaw_n)
+    if "canon" in extras:
+        line["extra"]["canonical_csr"] = bench_canon(ctx, cpu=not args.no_cpu)
     print(json.dumps(line))
 
 
@@ -291,6 +294,66 @@ def bench_power_law(ctx, n):
     return {"n": n, "nnz": int(a.nnz), "status": r.status, "relres": relres,
             "spai_ms": round(s.build_ms, 1), "passes": s.passes, "fits": s.fits,
             "solve_ms": round(r.solve_ms, 1), "wall_s": round(el, 3)}
+
+
+def bench_canon(ctx, cpu=True):
+    """Canonical CSR from coordinate entries on the GPU (reference
+    CsrMatrix::from_entries): the 128^3 system's 14,581,760 entries, each
+    split into two exact halves, shuffled (29,163,520 entries), resident in
+    HBM.  Algorithmic bytes: the entries read once (24 B each) and the CSR
+    written once (12 B per nonzero + 4 (n+1))."""
+    import numpy as np
+    import torch
+
+    import bspai_hip
+    from bspai_hip import problems as P
+
+    a = P.convection_diffusion_3d()
+    n = a.n
+    rows = np.repeat(np.arange(n, dtype=np.int64), np.diff(a.ptr))
+    r = np.concatenate([rows, rows])
+    c = np.concatenate([a.col, a.col]).astype(np.int64)
+    v = np.concatenate([a.val / 2, a.val / 2])
+    perm = np.random.default_rng(5).permutation(r.size)
+    r, c, v = r[perm], c[perm], v[perm]
+    dev = torch.device("cuda:0")
+    rd, cd, vd = (torch.from_numpy(x).to(dev) for x in (r, c, v))
+    out = bspai_hip.csr_from_entries(n, n, rd, cd, vd, ctx)
+    exact = (np.array_equal(out[0].cpu().numpy(), np.asarray(a.ptr))
+             and np.array_equal(out[1].cpu().numpy(), np.asarray(a.col))
+             and np.array_equal(out[2].cpu().numpy().view(np.uint64), a.val.view(np.uint64)))
+    reps = 5
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        bspai_hip.csr_from_entries(n, n, rd, cd, vd, ctx)
+    torch.cuda.synchronize()
+    ms = 1e3 * (time.perf_counter() - t0) / reps
+    ctx.set_timing(True)
+    bspai_hip.csr_from_entries(n, n, rd, cd, vd, ctx)
+    kt = ctx.kernel_times()
+    ctx.set_timing(False)
+    count = r.size
+    alg = 24.0 * count + 12.0 * a.nnz + 4.0 * (n + 1)
+    line = {"entries": int(count), "n": n, "nnz": int(a.nnz), "equals_generator_csr": exact,
+            "ms_per_call": round(ms, 3), "entries_per_s": round(count / (ms * 1e-3), 1),
+            "algorithmic_bytes": alg,
+            "roofline": {"bound": "hbm", "achieved": round(alg / (ms * 1e-3) / 1e9, 1),
+                         "peak": 8000.0, "unit": "GB/s",
+                         "frac": round(alg / (ms * 1e-3) / 8e12, 4)},
+            "kernels": {k: round(x["avg_us"], 1) for k, x in kt.items()}}
+    if cpu:
+        sys.path.insert(0, REPO)
+        from oracle import pyoracle as O
+
+        t0 = time.perf_counter()
+        O.from_entries(n, n, r, c, v)
+        el = time.perf_counter() - t0
+        line["cpu_baseline"] = {"value": round(count / el, 1), "unit": "entries/s", "cores": 1,
+                                "kind": "port", "sample": "oracle from_entries on the same "
+                                "29,163,520 entries (std::stable_sort, one core)",
+                                "ms": round(1e3 * el, 1)}
+    return line
 
 
 PEAK_TFLOPS = {"double": 78.6, "float": 157.3}

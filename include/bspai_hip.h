@@ -172,7 +172,9 @@ int bspai_hip_matrix_transpose(bspai_hip_ctx *ctx, const bspai_hip_matrix *m,
  * duplicate sum that becomes non-finite ("Duplicate sum is nonfinite").
  * row, col, val: `count` entries in `where` memory.  Outputs in `out_where`
  * memory: ptr (rows + 1 entries), col_out and val_out (room for `count`; the
- * first *nnz are written).  rows, cols and count below 2^31 - 1. */
+ * first *nnz are written; after an error their contents are unspecified).
+ * rows, cols and count below 2^31 - 1.  Device scratch (about 60 bytes per
+ * entry) is kept by the context for later calls. */
 int bspai_hip_csr_from_entries(bspai_hip_ctx *ctx, int64_t rows, int64_t cols, int64_t count,
                                const int64_t *row, const int64_t *col, const double *val,
                                int where, int32_t *ptr, int32_t *col_out, double *val_out,

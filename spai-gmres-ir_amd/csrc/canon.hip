@@ -103,29 +103,73 @@ int canon_blocks(std::int64_t count) {
 
 std::int64_t csr_from_entries(Context &c, std::int64_t rows, std::int64_t cols,
                               std::int64_t count, const std::int64_t *row,
-                              const std::int64_t *col, const double *val, DeviceArray<int> &ptr,
-                              DeviceArray<int> &out_col, DeviceArray<double> &out_val) {
+                              const std::int64_t *col, const double *val, int *ptr, int *out_col,
+                              double *out_val) {
     const hipStream_t s = c.stream;
-    ptr.resize(rows + 1);
-    out_col.resize(std::max<std::int64_t>(count, 1));
-    out_val.resize(std::max<std::int64_t>(count, 1));
-    HIP_CHECK(hipMemsetAsync(ptr.data(), 0, (rows + 1) * sizeof(int), s));
+    CanonWork &w = c.canon;
+    HIP_CHECK(hipMemsetAsync(ptr, 0, (rows + 1) * sizeof(int), s));
     if (count == 0) {
         HIP_CHECK(hipStreamSynchronize(s));
         return 0;
     }
     const int blocks = canon_blocks(count);
-    DeviceArray<unsigned long long> keys(count), keys_sorted(count), first_bad(1);
-    DeviceArray<double> vals_sorted(count);
-    HIP_CHECK(hipMemsetAsync(first_bad.data(), 0xff, sizeof(unsigned long long), s));
-    const double entry_bytes = 2.0 * 8 + 8; // row, col, value
-    timed(c, "entries_check_kernel", (entry_bytes + 8) * (double)count, 0.0, [&] {
+    w.keys.resize(count);
+    w.keys_sorted.resize(count);
+    w.vals_sorted.resize(count);
+    w.sums.resize(count);
+    w.keep.resize(count);
+    w.slot.resize(count);
+    w.flags64.resize(1);
+    w.flags.resize(1);
+    w.scan_tmp.resize(scan_tmp_ints(count));
+    w.row_tmp.resize(scan_tmp_ints(rows + 1));
+    HIP_CHECK(hipMemsetAsync(w.flags64.data(), 0xff, sizeof(unsigned long long), s));
+    HIP_CHECK(hipMemsetAsync(w.flags.data(), 0, sizeof(int), s));
+    // One pass to the end, then one synchronization: an invalid entry gets
+    // key 0 (in range), so the later stages stay in bounds, and the errors
+    // are reported in the reference's order (validation first).
+    timed(c, "entries_check_kernel", 32.0 * (double)count, 0.0, [&] {
         hipLaunchKernelGGL(entries_check_kernel, dim3(blocks), dim3(kCanonThreads), 0, s, count,
-                           rows, cols, row, col, val, keys.data(), first_bad.data());
+                           rows, cols, row, col, val, w.keys.data(), w.flags64.data());
         HIP_LAUNCH_CHECK();
     });
+    // Sort on the key bits in use only.
+    const unsigned long long max_key =
+        (unsigned long long)rows * (unsigned long long)cols - 1ull;
+    const int end_bit = max_key == 0 ? 1 : 64 - __builtin_clzll(max_key);
+    std::size_t tmp_bytes = 0;
+    HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, w.keys.data(),
+                                                 w.keys_sorted.data(), val, w.vals_sorted.data(),
+                                                 (int)count, 0, end_bit, s));
+    w.sort_tmp.resize(std::max<std::size_t>(tmp_bytes, 1));
+    timed(c, "entries_radix_sort", 4.0 * 16.0 * (double)count, 0.0, [&] {
+        HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(w.sort_tmp.data(), tmp_bytes, w.keys.data(),
+                                                     w.keys_sorted.data(), val,
+                                                     w.vals_sorted.data(), (int)count, 0, end_bit,
+                                                     s));
+    });
+    timed(c, "run_sums_kernel", 28.0 * (double)count, 0.0, [&] {
+        hipLaunchKernelGGL(run_sums_kernel, dim3(blocks), dim3(kCanonThreads), 0, s, count,
+                           w.keys_sorted.data(), w.vals_sorted.data(), w.keep.data(),
+                           w.sums.data(), w.flags.data());
+        HIP_LAUNCH_CHECK();
+    });
+    HIP_CHECK(hipMemcpyAsync(w.slot.data(), w.keep.data(), count * sizeof(int),
+                             hipMemcpyDeviceToDevice, s));
+    scan_inclusive_int(s, w.slot.data(), count, w.scan_tmp.data());
+    timed(c, "scatter_kernel", 20.0 * (double)count, 0.0, [&] {
+        hipLaunchKernelGGL(scatter_kernel, dim3(blocks), dim3(kCanonThreads), 0, s, count, cols,
+                           w.keys_sorted.data(), w.keep.data(), w.slot.data(), w.sums.data(),
+                           out_col, out_val, ptr);
+        HIP_LAUNCH_CHECK();
+    });
+    scan_inclusive_int(s, ptr, rows + 1, w.row_tmp.data());
     unsigned long long bad = 0;
-    HIP_CHECK(hipMemcpyAsync(&bad, first_bad.data(), sizeof bad, hipMemcpyDeviceToHost, s));
+    int flag = 0, nnz = 0;
+    HIP_CHECK(hipMemcpyAsync(&bad, w.flags64.data(), sizeof bad, hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipMemcpyAsync(&flag, w.flags.data(), sizeof flag, hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipMemcpyAsync(&nnz, w.slot.data() + (count - 1), sizeof nnz,
+                             hipMemcpyDeviceToHost, s));
     HIP_CHECK(hipStreamSynchronize(s));
     if (bad != ULLONG_MAX) {
         // The reference checks the indices of an entry before its value.
@@ -136,48 +180,8 @@ std::int64_t csr_from_entries(Context &c, std::int64_t rows, std::int64_t cols,
             throw std::invalid_argument("Entry index outside matrix dimensions");
         throw std::invalid_argument("Matrix entries must be finite");
     }
-    // Sort on the key bits in use only.
-    const unsigned long long max_key =
-        (unsigned long long)rows * (unsigned long long)cols - 1ull;
-    const int end_bit = max_key == 0 ? 1 : 64 - __builtin_clzll(max_key);
-    std::size_t tmp_bytes = 0;
-    HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, keys.data(),
-                                                 keys_sorted.data(), val, vals_sorted.data(),
-                                                 (int)count, 0, end_bit, s));
-    DeviceArray<unsigned char> tmp(std::max<std::size_t>(tmp_bytes, 1));
-    timed(c, "entries_radix_sort", 2.0 * 2.0 * 16.0 * (double)count, 0.0, [&] {
-        HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(tmp.data(), tmp_bytes, keys.data(),
-                                                     keys_sorted.data(), val, vals_sorted.data(),
-                                                     (int)count, 0, end_bit, s));
-    });
-    DeviceArray<int> keep(count), slot(count), nonfinite(1), scan_tmp(scan_tmp_ints(count));
-    DeviceArray<double> sums(count);
-    HIP_CHECK(hipMemsetAsync(nonfinite.data(), 0, sizeof(int), s));
-    timed(c, "run_sums_kernel", 16.0 * (double)count + 12.0 * (double)count, 0.0, [&] {
-        hipLaunchKernelGGL(run_sums_kernel, dim3(blocks), dim3(kCanonThreads), 0, s, count,
-                           keys_sorted.data(), vals_sorted.data(), keep.data(), sums.data(),
-                           nonfinite.data());
-        HIP_LAUNCH_CHECK();
-    });
-    HIP_CHECK(hipMemcpyAsync(slot.data(), keep.data(), count * sizeof(int),
-                             hipMemcpyDeviceToDevice, s));
-    scan_inclusive_int(s, slot.data(), count, scan_tmp.data());
-    int flag = 0, nnz = 0;
-    HIP_CHECK(hipMemcpyAsync(&flag, nonfinite.data(), sizeof flag, hipMemcpyDeviceToHost, s));
-    HIP_CHECK(hipMemcpyAsync(&nnz, slot.data() + (count - 1), sizeof nnz, hipMemcpyDeviceToHost,
-                             s));
-    HIP_CHECK(hipStreamSynchronize(s));
     if (flag)
         throw std::invalid_argument("Duplicate sum is nonfinite");
-    timed(c, "scatter_kernel", 8.0 * (double)count + 12.0 * (double)nnz, 0.0, [&] {
-        hipLaunchKernelGGL(scatter_kernel, dim3(blocks), dim3(kCanonThreads), 0, s, count, cols,
-                           keys_sorted.data(), keep.data(), slot.data(), sums.data(),
-                           out_col.data(), out_val.data(), ptr.data());
-        HIP_LAUNCH_CHECK();
-    });
-    DeviceArray<int> row_tmp(scan_tmp_ints(rows + 1));
-    scan_inclusive_int(s, ptr.data(), rows + 1, row_tmp.data());
-    HIP_CHECK(hipStreamSynchronize(s));
     return nnz;
 }
 

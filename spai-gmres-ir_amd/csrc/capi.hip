@@ -432,15 +432,23 @@ int bspai_hip_csr_from_entries(bspai_hip_ctx *ctx, int64_t rows, int64_t cols, i
         Context &c = ctx->c;
         In<int64_t> r(c, row, count, where), cl(c, col, count, where);
         In<double> v(c, val, count, where);
-        DeviceArray<int> p, oc;
-        DeviceArray<double> ov;
-        const int64_t k = csr_from_entries(c, rows, cols, count, r.p, cl.p, v.p, p, oc, ov);
-        const hipMemcpyKind kind =
-            out_where == BSPAI_HIP_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
-        HIP_CHECK(hipMemcpyAsync(ptr, p.data(), (rows + 1) * sizeof(int), kind, c.stream));
+        if (out_where == BSPAI_HIP_DEVICE) {
+            *nnz = csr_from_entries(c, rows, cols, count, r.p, cl.p, v.p, ptr, col_out, val_out);
+            return;
+        }
+        CanonWork &w = c.canon;
+        w.out_ptr.resize(rows + 1);
+        w.out_col.resize(std::max<int64_t>(count, 1));
+        w.out_val.resize(std::max<int64_t>(count, 1));
+        const int64_t k = csr_from_entries(c, rows, cols, count, r.p, cl.p, v.p, w.out_ptr.data(),
+                                           w.out_col.data(), w.out_val.data());
+        HIP_CHECK(hipMemcpyAsync(ptr, w.out_ptr.data(), (rows + 1) * sizeof(int),
+                                 hipMemcpyDeviceToHost, c.stream));
         if (k > 0) {
-            HIP_CHECK(hipMemcpyAsync(col_out, oc.data(), k * sizeof(int), kind, c.stream));
-            HIP_CHECK(hipMemcpyAsync(val_out, ov.data(), k * sizeof(double), kind, c.stream));
+            HIP_CHECK(hipMemcpyAsync(col_out, w.out_col.data(), k * sizeof(int),
+                                     hipMemcpyDeviceToHost, c.stream));
+            HIP_CHECK(hipMemcpyAsync(val_out, w.out_val.data(), k * sizeof(double),
+                                     hipMemcpyDeviceToHost, c.stream));
         }
         HIP_CHECK(hipStreamSynchronize(c.stream));
         *nnz = k;

@@ -121,6 +121,17 @@ struct SpmvChoice {
     int lanes = 1, per = 8;
 };
 
+// Scratch of the coordinate-entry canonicalization, kept by the context and
+// grown on demand (entries of one call at a time).
+struct CanonWork {
+    DeviceArray<unsigned long long> keys, keys_sorted, flags64;
+    DeviceArray<double> vals_sorted, sums;
+    DeviceArray<int> keep, slot, flags, scan_tmp, row_tmp;
+    DeviceArray<unsigned char> sort_tmp;
+    DeviceArray<int> out_ptr, out_col;
+    DeviceArray<double> out_val;
+};
+
 struct Context {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -131,6 +142,7 @@ struct Context {
     KernelTimer timer;
     // SpMV autotuning results by matrix shape (n, nnz, value bytes)
     std::map<std::tuple<std::int64_t, std::int64_t, int>, SpmvChoice> spmv_tuned;
+    CanonWork canon;
 };
 
 // Runs `launch` (one kernel launch on c.stream), timed when the context's
@@ -271,12 +283,13 @@ void spai_score_on_device(hipStream_t s, double rr, double dot, double den, int 
                           int *clamped, double *score);
 
 // Canonical rows x cols CSR from `count` coordinate entries held on the
-// device (reference CsrMatrix::from_entries, src/cpu.cpp:48-76).  Returns
-// nnz; ptr, out_col and out_val are sized here.  Throws std::invalid_argument
-// with the reference's messages for an index out of range, a non-finite value
-// or a non-finite duplicate sum.
+// device (reference CsrMatrix::from_entries, src/cpu.cpp:48-76), written to
+// device arrays ptr (rows + 1), out_col and out_val (room for count).
+// Returns nnz.  Throws std::invalid_argument with the reference's messages
+// for an index out of range, a non-finite value or a non-finite duplicate
+// sum; the outputs are then unspecified.
 std::int64_t csr_from_entries(Context &c, std::int64_t rows, std::int64_t cols,
                               std::int64_t count, const std::int64_t *row,
-                              const std::int64_t *col, const double *val, DeviceArray<int> &ptr,
-                              DeviceArray<int> &out_col, DeviceArray<double> &out_val);
+                              const std::int64_t *col, const double *val, int *ptr, int *out_col,
+                              double *out_val);
 } // namespace bspai_hip
