@@ -22,6 +22,12 @@ struct Entry {
 class CsrMatrix {
   public:
     static CsrMatrix from_entries(std::size_t rows, std::size_t cols, std::vector<Entry> entries);
+    // Extension (no reference counterpart): wraps arrays that are already
+    // canonical, e.g. the output of hip_csr_from_entries; only the sizes are
+    // checked.
+    static CsrMatrix from_canonical(std::size_t rows, std::size_t cols,
+                                    std::vector<std::size_t> offsets,
+                                    std::vector<std::size_t> columns, std::vector<double> values);
     std::size_t rows() const { return rows_; }
     std::size_t cols() const { return cols_; }
     const std::vector<std::size_t> &row_offsets() const { return offsets_; }
@@ -39,6 +45,9 @@ class CsrMatrix {
 
 // Matrix Market coordinate real/integer, general/symmetric.
 CsrMatrix read_matrix_market(std::istream &input);
+// The reader's parse and checks alone: the dimensions and the entries it
+// hands to CsrMatrix::from_entries (symmetric files expanded).
+std::vector<Entry> parse_matrix_market(std::istream &input, std::size_t &rows, std::size_t &cols);
 
 } // namespace bspai
 #endif

@@ -32,6 +32,15 @@ struct HipSpaiResult {
     HipTelemetry telemetry;
 };
 
+// CsrMatrix::from_entries with the sort and the duplicate sums on the GPU
+// (bspai_hip_csr_from_entries): the same matrix, and the same
+// std::invalid_argument messages for bad entries.
+CsrMatrix hip_csr_from_entries(std::size_t rows, std::size_t cols,
+                               const std::vector<Entry> &entries, int device = 0);
+// read_matrix_market with the canonicalization on the GPU (the text is parsed
+// and checked on the host, as the reference's reader does).
+CsrMatrix hip_read_matrix_market(std::istream &input, int device = 0);
+
 // GPU SPAI construction (reference construct_spai / cuda_construct_spai).
 HipSpaiResult hip_construct_spai(const CsrMatrix &a, const SpaiOptions &options,
                                  HipPrecision precision = HipPrecision::fp64, int device = 0);

@@ -19,6 +19,19 @@ CsrMatrix::CsrMatrix(std::size_t rows, std::size_t cols) : rows_(rows), cols_(co
     offsets_.assign(rows + 1, 0);
 }
 
+CsrMatrix CsrMatrix::from_canonical(std::size_t rows, std::size_t cols,
+                                    std::vector<std::size_t> offsets,
+                                    std::vector<std::size_t> columns, std::vector<double> values) {
+    if (offsets.size() != rows + 1 || columns.size() != values.size() ||
+        offsets.back() != values.size())
+        throw std::invalid_argument("Canonical CSR arrays of inconsistent sizes");
+    CsrMatrix m(rows, cols);
+    m.offsets_ = std::move(offsets);
+    m.columns_ = std::move(columns);
+    m.values_ = std::move(values);
+    return m;
+}
+
 CsrMatrix CsrMatrix::from_entries(std::size_t rows, std::size_t cols, std::vector<Entry> entries) {
     for (const Entry &e : entries) {
         if (e.row >= rows || e.col >= cols)
@@ -114,6 +127,13 @@ void no_more(std::istringstream &s) {
 } // namespace
 
 CsrMatrix read_matrix_market(std::istream &in) {
+    std::size_t rows = 0, cols = 0;
+    std::vector<Entry> entries = parse_matrix_market(in, rows, cols);
+    return CsrMatrix::from_entries(rows, cols, std::move(entries));
+}
+
+std::vector<Entry> parse_matrix_market(std::istream &in, std::size_t &rows_out,
+                                       std::size_t &cols_out) {
     std::string line, magic, object, format, field, symmetry;
     if (!std::getline(in, line))
         throw std::runtime_error("Missing Matrix Market banner");
@@ -167,7 +187,9 @@ CsrMatrix read_matrix_market(std::istream &in) {
     }
     if (data_line(in, line))
         throw std::runtime_error("Trailing coordinate data");
-    return CsrMatrix::from_entries(rows, cols, std::move(entries));
+    rows_out = rows;
+    cols_out = cols;
+    return entries;
 }
 
 const char *spai_status_name(SpaiStatus s) {

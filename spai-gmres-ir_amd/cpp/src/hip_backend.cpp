@@ -7,7 +7,9 @@
 
 #include "bspai_hip.h"
 
+#include <algorithm>
 #include <climits>
+#include <cstdint>
 #include <cmath>
 #include <memory>
 #include <stdexcept>
@@ -161,6 +163,40 @@ void fill_ir(const bspai_hip_ir_info &i, std::vector<double> x, IrResult &r) {
 }
 
 } // namespace
+
+CsrMatrix hip_csr_from_entries(std::size_t rows, std::size_t cols,
+                               const std::vector<Entry> &entries, int device) {
+    const Ctx c(device);
+    const std::size_t count = entries.size();
+    std::vector<std::int64_t> r(count), cl(count);
+    std::vector<double> v(count);
+    for (std::size_t i = 0; i < count; ++i) {
+        r[i] = (std::int64_t)entries[i].row; // beyond INT64_MAX: negative, out of range
+        cl[i] = (std::int64_t)entries[i].col;
+        v[i] = entries[i].value;
+    }
+    std::vector<std::int32_t> ptr(rows + 1), col(std::max<std::size_t>(count, 1));
+    std::vector<double> val(std::max<std::size_t>(count, 1));
+    std::int64_t nnz = 0;
+    const int rc = bspai_hip_csr_from_entries(
+        c.h, (std::int64_t)rows, (std::int64_t)cols, (std::int64_t)count, r.data(), cl.data(),
+        v.data(), BSPAI_HIP_HOST, ptr.data(), col.data(), val.data(), BSPAI_HIP_HOST, &nnz);
+    if (rc == BSPAI_HIP_EINVAL)
+        throw std::invalid_argument(bspai_hip_last_error(c.h));
+    c.check(rc);
+    col.resize((std::size_t)nnz);
+    val.resize((std::size_t)nnz);
+    return CsrMatrix::from_canonical(rows, cols,
+                                     std::vector<std::size_t>(ptr.begin(), ptr.end()),
+                                     std::vector<std::size_t>(col.begin(), col.end()),
+                                     std::move(val));
+}
+
+CsrMatrix hip_read_matrix_market(std::istream &input, int device) {
+    std::size_t rows = 0, cols = 0;
+    const std::vector<Entry> entries = parse_matrix_market(input, rows, cols);
+    return hip_csr_from_entries(rows, cols, entries, device);
+}
 
 HipSpaiResult hip_construct_spai(const CsrMatrix &a, const SpaiOptions &o, HipPrecision p,
                                  int device) {

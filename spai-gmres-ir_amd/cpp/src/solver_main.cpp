@@ -45,7 +45,7 @@ int main(int argc, char **argv) {
         std::ifstream in(argv[1]);
         if (!in)
             throw std::invalid_argument("Cannot open Matrix Market input");
-        const bspai::CsrMatrix a = bspai::read_matrix_market(in);
+        const bspai::CsrMatrix a = bspai::hip_read_matrix_market(in);
         if (!a.rows() || a.rows() != a.cols())
             throw std::invalid_argument("Solver requires a nonempty square matrix");
         bspai::IrOptions o;

@@ -16,6 +16,7 @@ oracle/_ref via `make -C oracle ref`).  Fixtures written:
   cli_tiny.txt                `cpu_fp64_ir tiny_nonsymmetric.mtx 0.5` stdout
                               (tests/fixtures/tiny_nonsymmetric.mtx of the reference)
   cli_c1_head.txt             first 6 lines of `cpu_fp64_ir c1.mtx` stdout + x hash
+  cli_dup_overflow.txt        `cpu_fp64_ir dup.mtx` exit code and stderr, duplicate sum > DBL_MAX
 """
 from __future__ import annotations
 
@@ -82,6 +83,21 @@ def pack(g, prefix=""):
     return d
 
 
+DUP_OVERFLOW_MTX = ("%%MatrixMarket matrix coordinate real general\n2 2 3\n"
+                    "1 1 1e308\n1 1 1e308\n2 2 1\n")
+
+
+def cli_dup_overflow(tmp):
+    """The reference CLI on a file whose duplicate entries sum past DBL_MAX:
+    CsrMatrix::from_entries rejects it (invalid_argument, exit code 2)."""
+    mtx = os.path.join(tmp, "dup.mtx")
+    with open(mtx, "w") as f:
+        f.write(DUP_OVERFLOW_MTX)
+    r = subprocess.run([os.path.join(BIN, "cpu_fp64_ir"), mtx], capture_output=True, text=True)
+    with open(os.path.join(HERE, "cli_dup_overflow.txt"), "w") as f:
+        f.write(f"exit={r.returncode}\n" + r.stdout + r.stderr)
+
+
 def main():
     subprocess.run(["make", "-s", "-C", os.path.join(REPO, "oracle"), "all", "ref",
                     f"REF={REF}"], check=True)
@@ -126,6 +142,7 @@ def main():
                            capture_output=True, text=True)
         with open(os.path.join(HERE, "cli_bad_outercap.txt"), "w") as f:
             f.write(f"exit={r.returncode}\n" + r.stdout + r.stderr)
+        cli_dup_overflow(tmp)
     print("goldens written to", HERE)
 
 

@@ -83,6 +83,21 @@ def test_cli_invalid_arguments_exit_2():
         assert run("hip_fp64_ir", os.path.join(d, "missing.mtx")).returncode == 2
 
 
+def test_cli_duplicate_overflow_exit_2():
+    """Duplicate entries summing past DBL_MAX: the canonicalization (on the
+    GPU here) rejects the file with the reference CLI's message and code."""
+    bad = open(os.path.join(GOLDEN, "cli_dup_overflow.txt")).read().splitlines()
+    with tempfile.TemporaryDirectory() as d:
+        mtx = os.path.join(d, "dup.mtx")
+        with open(mtx, "w") as f:
+            f.write("%%MatrixMarket matrix coordinate real general\n2 2 3\n"
+                    "1 1 1e308\n1 1 1e308\n2 2 1\n")
+        for exe in ("hip_fp64_ir", "hip_mixed_ir"):
+            r = run(exe, mtx)
+            assert f"exit={r.returncode}" == bad[0] == "exit=2"
+            assert r.stderr.strip() == bad[1]
+
+
 def test_cli_failure_exit_1():
     """A construction failure (zero diagonal) is a solver failure: exit 1."""
     with tempfile.TemporaryDirectory() as d:
