@@ -504,41 +504,48 @@ template <int S, int B>
 __device__ inline void gram_steps(double (&acc)[S], const double *Lc, int &i, int &lo, int iend,
                                   int k, int lane) {
     constexpr int kSolvePad = S * kWave;
+    // Steps are taken in batches of kNB: the batch's L entries are read from
+    // LDS together (one round of LDS latency per batch, not per step), then
+    // the batch's steps run from registers.  Reads past `stop` repeat the
+    // last step's column, so every read stays inside the staged chunk.
+    constexpr int kNB = 32 / S; // 32 doubles of L in registers per batch
     const int stop = min(iend, (B + 1) * kWave);
-    if (i >= stop)
-        return;
-    auto load = [&](int ii, int l, double(&lv)[S]) {
-        const double *colp = Lc + kSolvePad + l - ii - 1 + lane;
+    const double *base = Lc + kSolvePad + lane;
+    while (i < stop) {
+        int col0[kNB], piv[kNB];
+        {
+            int l = lo;
 #pragma unroll
-        for (int u = B; u < S; ++u)
-            lv[u] = colp[u * kWave];
-    };
-    auto step = [&](int ii, const double(&lv)[S]) {
-        const double hi = readlane_f64(acc[B], ii - B * kWave);
-        const double y = acc[B] - lv[B] * hi;
-        acc[B] = lane + B * kWave > ii ? y : acc[B];
+            for (int t = 0; t < kNB; ++t) {
+                const bool ok = i + t < stop;
+                piv[t] = ok ? i + t : piv[t > 0 ? t - 1 : 0];
+                col0[t] = ok ? l : col0[t > 0 ? t - 1 : 0];
+                if (ok)
+                    l += k - (i + t);
+            }
+            lo = l;
+        }
+        double lv[kNB][S];
 #pragma unroll
-        for (int u = B + 1; u < S; ++u)
-            acc[u] = acc[u] - lv[u] * hi;
-    };
-    // two register sets: step i's entries were read during step i-1
-    double la[S], lb[S];
-    load(i, lo, la);
-    for (;;) {
-        int nlo = lo + (k - i);
-        if (i + 1 < stop)
-            load(i + 1, nlo, lb);
-        step(i, la);
-        lo = nlo;
-        if (++i >= stop)
-            break;
-        nlo = lo + (k - i);
-        if (i + 1 < stop)
-            load(i + 1, nlo, la);
-        step(i, lb);
-        lo = nlo;
-        if (++i >= stop)
-            break;
+        for (int t = 0; t < kNB; ++t) {
+            const double *colp = base + col0[t] - piv[t] - 1;
+#pragma unroll
+            for (int u = B; u < S; ++u)
+                lv[t][u] = colp[u * kWave];
+        }
+#pragma unroll
+        for (int t = 0; t < kNB; ++t) {
+            if (i + t < stop) {
+                const int ii = i + t;
+                const double hi = readlane_f64(acc[B], ii - B * kWave);
+                const double y = acc[B] - lv[t][B] * hi;
+                acc[B] = lane + B * kWave > ii ? y : acc[B];
+#pragma unroll
+                for (int u = B + 1; u < S; ++u)
+                    acc[u] = acc[u] - lv[t][u] * hi;
+            }
+        }
+        i = min(i + kNB, stop);
     }
 }
 
@@ -580,16 +587,18 @@ gram_solve_lds_kernel(Tiles tiles, KrylovArgs<T> a, int k) {
         // flat entry e of the chunk -> column i, row j = i + 1 + e - off(i).
         // Absolute offsets from column 0, F(i) = i (2k + 1 - i) / 2, are
         // inverted in closed form (then corrected by one step either way);
-        // all of a thread's loads are issued together.
+        // all of a thread's loads are issued together.  The root's argument
+        // (2k+1)^2 - 8e < 2^24 is exact in FP32, whose rounded root is within
+        // the one-step correction.
         {
             constexpr int R = kChunk / kSolveThreads;
             const long long base = (long long)c0 * (2 * k + 1 - c0) / 2;
-            const double b2 = 2.0 * k + 1.0;
+            const float b2 = 2.0f * (float)k + 1.0f;
             int src[R];
 #pragma unroll
             for (int u = 0; u < R; ++u) {
                 const long long e = base + min(tid + u * kSolveThreads, used - 1);
-                int i = (int)floor((b2 - sqrt(b2 * b2 - 8.0 * (double)e)) * 0.5);
+                int i = (int)floorf((b2 - sqrtf(b2 * b2 - 8.0f * (float)e)) * 0.5f);
                 i = max(c0, min(i, c1 - 1));
                 if ((long long)i * (2 * k + 1 - i) / 2 > e)
                     --i;
