@@ -1176,35 +1176,40 @@ spai_wave_kernel(const int *__restrict__ aptr, const int *__restrict__ acol,
                 if (lane == j)
                     v += copysignf(1.f, v);
                 v = v / vnorm;
+                // v is 0 outside rows j..m-1, so the products need no mask;
+                // the sums of columns c <= j are formed but never read.
                 float p[kWQ];
 #pragma unroll
                 for (int c = 0; c < kWQ; ++c)
-                    p[c] = act && c > j ? v * r[c] : 0.f;
+                    p[c] = v * r[c];
                 const float d = wave_reduce_scatter16(p, lane);
                 const float dr = wave_sum_f(v * rh);
                 const float v2 = 2.f * v;
-                bool bad = false;
 #pragma unroll
                 for (int c = 1; c < kWQ; ++c) {
                     if (c > j && c < q) {
                         const float dc = lane_value(d, 4 * c);
-                        if (act) {
+                        if (act)
                             r[c] -= v2 * dc;
-                            bad = bad || !isfinite(r[c]);
-                        }
                     }
                 }
-                if (act) {
+                if (act)
                     rh -= v2 * dr;
-                    bad = bad || !isfinite(rh);
-                }
-                if (__any(bad)) {
-                    fit_status = kSpaiNumericalFailure;
-                    break;
-                }
                 if (lane == j)
                     dg = diagonal;
                 min_pivot = fminf(min_pivot, fabsf(diagonal));
+            }
+            // A non-finite value left by a reflection stays non-finite through
+            // every later step (and makes a later column's norm non-finite),
+            // so one check after the loop finds what a check per step would,
+            // with the same status.
+            if (fit_status == kSpaiSuccess) {
+                bool bad = !isfinite(rh);
+#pragma unroll
+                for (int c = 1; c < kWQ; ++c)
+                    bad = bad || (c < q && !isfinite(r[c]));
+                if (__any(bad))
+                    fit_status = kSpaiNumericalFailure;
             }
             // back substitution, one coefficient per lane (lane j: coef_j)
             float coef = 0.f;
