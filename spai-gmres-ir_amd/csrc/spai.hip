@@ -64,7 +64,7 @@ struct Caps {
     int mcap, qcap, ccap;
 };
 
-constexpr int kFuse = 2; // trailing columns reflected together (register budget)
+constexpr int kFuse = 4; // trailing columns reflected together (register budget)
 
 template <typename T> struct Lane {
     T *d;          // float work area
