@@ -333,7 +333,9 @@ void spai_build_typed(Context &c, const DevCsr &a, const T *aval, T *d, double e
     const int n = (int)a.n;
     hipStream_t s = c.stream;
     // ---- scaling and validity (src/spai.cpp:322-348)
-    DeviceArray<int> bad(2);
+    SpaiWork &w = c.spai;
+    DeviceArray<int> &bad = w.bad;
+    bad.resize(2);
     {
         const int init[2] = {INT_MAX, INT_MAX};
         upload(c, bad.data(), init, 2);
@@ -359,15 +361,20 @@ void spai_build_typed(Context &c, const DevCsr &a, const T *aval, T *d, double e
     // lane per column from 4k to 2.1M columns: 58 against 103 ms at 128^3),
     // up to the column count its 32-bit sort keys hold; the lane kernel
     // beyond.  BSPAI_SPAI_FP32_KERNEL=lane|wave forces one.
-    hipDeviceProp_t prop;
-    HIP_CHECK(hipGetDeviceProperties(&prop, c.device));
+    if (w.cus == 0)
+        HIP_CHECK(hipDeviceGetAttribute(&w.cus, hipDeviceAttributeMultiprocessorCount, c.device));
+    const int cus = w.cus;
     bool wave_fp32 = std::is_same<T, float>::value;
     if (const char *env = std::getenv("BSPAI_SPAI_FP32_KERNEL"))
         wave_fp32 = std::is_same<T, float>::value && std::string(env) == "wave";
     wave_fp32 = wave_fp32 && n <= spai_wave_max_n();
-    DeviceArray<int> bptr(n + 1), bcol(a.nnz > 0 ? a.nnz : 1), cursor(n + 1);
-    DeviceArray<T> bval(wave_fp32 && a.nnz > 0 ? a.nnz : 1);
-    DeviceArray<int> scan_tmp(scan_tmp_ints(n + 1));
+    DeviceArray<int> &bptr = w.bptr, &bcol = w.bcol, &cursor = w.cursor, &scan_tmp = w.scan_tmp;
+    bptr.resize(n + 1);
+    bcol.resize(a.nnz > 0 ? a.nnz : 1);
+    cursor.resize(n + 1);
+    scan_tmp.resize(scan_tmp_ints(n + 1));
+    w.bval.resize((wave_fp32 && a.nnz > 0 ? a.nnz : 1) * sizeof(T));
+    T *const bval = w.bval.as<T>();
     HIP_CHECK(hipMemsetAsync(bptr.data(), 0, (n + 1) * sizeof(int), s));
     launch_count_cols(s, a.nnz, a.col.data(), bptr.data());
     scan_inclusive_int(s, bptr.data(), n + 1, scan_tmp.data());
@@ -375,7 +382,7 @@ void spai_build_typed(Context &c, const DevCsr &a, const T *aval, T *d, double e
                              hipMemcpyDeviceToDevice, s));
     launch_transpose_fill<T>(s, n, a.ptr.data(), a.col.data(), wave_fp32 ? aval : nullptr,
                              bptr.data(), cursor.data(), bcol.data(),
-                             wave_fp32 ? bval.data() : nullptr);
+                             wave_fp32 ? bval : nullptr);
     // ---- columns, in capacity passes
     out.c_status.resize(n);
     out.c_fits.resize(n);
@@ -387,7 +394,10 @@ void spai_build_typed(Context &c, const DevCsr &a, const T *aval, T *d, double e
     out.c_flops.resize(n);
     out.c_off.resize(n);
     out.c_pass.resize(n);
-    DeviceArray<int> work(n), next(n), counter(2);
+    DeviceArray<int> &work = w.work, &next = w.next, &counter = w.counter;
+    work.resize(n);
+    next.resize(n);
+    counter.resize(2);
     hipLaunchKernelGGL(iota_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, s, n, work.data());
     int nwork = n;
     int mcap = 64, qcap = 16, ccap = 128;
@@ -396,12 +406,12 @@ void spai_build_typed(Context &c, const DevCsr &a, const T *aval, T *d, double e
     int waves_per_cu = 32;
     if (const char *env = std::getenv("BSPAI_SPAI_WAVES_PER_CU"))
         waves_per_cu = std::max(1, std::atoi(env));
-    const int max_waves = prop.multiProcessorCount * waves_per_cu;
+    const int max_waves = cus * waves_per_cu;
     std::size_t free_bytes = 0, total_bytes = 0;
     HIP_CHECK(hipMemGetInfo(&free_bytes, &total_bytes));
     const std::int64_t area_cap =
         std::min<std::int64_t>(8LL << 30, std::max<std::int64_t>(1LL << 28, free_bytes / 4));
-    DeviceBuffer dwork, iwork;
+    DeviceBuffer &dwork = w.dwork, &iwork = w.iwork;
     std::vector<std::string> pass_labels; // timer label of each capacity pass
     while (nwork > 0) {
         if (out.passes >= 8)
@@ -426,7 +436,7 @@ void spai_build_typed(Context &c, const DevCsr &a, const T *aval, T *d, double e
             const std::int64_t bd = spai_block_doubles(mcap, qcap, ccap);
             const std::int64_t bi = spai_block_ints(mcap, qcap, ccap, words, n);
             const std::int64_t per_block = bd * (std::int64_t)sizeof(T) + bi * 4;
-            std::int64_t blocks = std::min<std::int64_t>(nwork, prop.multiProcessorCount * 4LL);
+            std::int64_t blocks = std::min<std::int64_t>(nwork, cus * 4LL);
             // Large buckets hold tens of MB per workgroup: allow up to a quarter
             // of free memory (48 GiB at most) so that a full wave of
             // workgroups stays resident.
@@ -470,7 +480,7 @@ void spai_build_typed(Context &c, const DevCsr &a, const T *aval, T *d, double e
             // FP32: one wavefront per column (a one-wave workgroup).
             const int wq = spai_wave_qcap();
             const int blocks = (int)std::max<std::int64_t>(
-                1, std::min<std::int64_t>(nwork, prop.multiProcessorCount * 16LL));
+                1, std::min<std::int64_t>(nwork, cus * 16LL));
             out.pool_j.emplace_back((std::size_t)nwork * wq);
             out.pool_c.emplace_back((std::size_t)nwork * wq * sizeof(T));
             HIP_CHECK(hipMemsetAsync(counter.data(), 0, 2 * sizeof(int), s));
@@ -482,7 +492,7 @@ void spai_build_typed(Context &c, const DevCsr &a, const T *aval, T *d, double e
             timed(c, "spai_wave_kernel<float>", 0.0, 0.0, [&] {
                 launch_spai_wave(s, blocks, a.ptr.data(), a.col.data(), (const float *)aval,
                                  (const float *)d, bptr.data(), bcol.data(),
-                                 (const float *)bval.data(), (float)eps, alpha, beta,
+                                 (const float *)bval, (float)eps, alpha, beta,
                                  work.data(), nwork, counter.data(), o,
                                  out.pool_j.back().data(), out.pool_c.back().as<float>(), wq);
             }, true);
@@ -537,9 +547,12 @@ void spai_build_typed(Context &c, const DevCsr &a, const T *aval, T *d, double e
         ccap *= 4;
     }
     // ---- summary and P assembly (src/spai.cpp:352-372)
-    DeviceArray<int> ff(2);
-    DeviceArray<unsigned long long> tot(4);
-    DeviceArray<double> ftot(8);
+    DeviceArray<int> &ff = w.ff;
+    DeviceArray<unsigned long long> &tot = w.tot;
+    DeviceArray<double> &ftot = w.ftot;
+    ff.resize(2);
+    tot.resize(4);
+    ftot.resize(8);
     {
         const int init[2] = {INT_MAX, INT_MAX};
         upload(c, ff.data(), init, 2);
@@ -934,6 +947,20 @@ template class GmresEngine<double>;
 template class GmresEngine<float>;
 
 // ---------------------------------------------------------------- IR
+// The context's engine when it was made for this layout and capacity, else a
+// new one in its place (the old workspace is freed first).
+template <typename T>
+GmresEngine<T> &cached_engine(Context &c, EngineSlot<T> &slot,
+                              const std::shared_ptr<Layout> &layout, int cap) {
+    if (!slot.engine || slot.layout != layout || slot.cap != cap) {
+        slot.engine.reset();
+        slot.engine = std::make_shared<GmresEngine<T>>(c, *layout, cap);
+        slot.layout = layout;
+        slot.cap = cap;
+    }
+    return *slot.engine;
+}
+
 std::vector<IrResultHost> ir_run(Context &c, const DevCsr &a64, const DevCsr &m1, const DevCsr &m2,
                                  GmresMode mode, const double *b, const IrOptionsC &o, double *x) {
     hipStream_t s = c.stream;
@@ -967,13 +994,13 @@ std::vector<IrResultHost> ir_run(Context &c, const DevCsr &a64, const DevCsr &m1
     for (int i = 0; i < nsys; ++i)
         maxrows = std::max(maxrows, L.offsets[i + 1] - L.offsets[i]);
     const int kcap = (int)std::min<std::int64_t>(cap, maxrows);
-    std::unique_ptr<GmresEngine<double>> g64;
-    std::unique_ptr<GmresEngine<float>> g32;
+    GmresEngine<double> *g64 = nullptr;
+    GmresEngine<float> *g32 = nullptr;
     DeviceArray<float> rf;
     if (o.precision == 0)
-        g64 = std::make_unique<GmresEngine<double>>(c, L, kcap);
+        g64 = &cached_engine(c, c.engine64, a64.layout, kcap);
     else {
-        g32 = std::make_unique<GmresEngine<float>>(c, L, kcap);
+        g32 = &cached_engine(c, c.engine32, a64.layout, kcap);
         rf.resize(L.n);
     }
     std::vector<IrSysState> host(nsys);

@@ -132,6 +132,29 @@ struct CanonWork {
     DeviceArray<double> out_val;
 };
 
+// Scratch of the SPAI construction, kept by the context and grown on demand
+// (one construction at a time): B's structure and values, the column work
+// lists, the column kernels' work areas and the assembly counters.
+struct SpaiWork {
+    DeviceArray<int> bad, bptr, bcol, cursor, scan_tmp, work, next, counter, ff;
+    DeviceArray<unsigned long long> tot;
+    DeviceArray<double> ftot;
+    DeviceBuffer bval, dwork, iwork;
+    int cus = 0; // compute units of the device
+};
+
+template <typename T> class GmresEngine;
+struct Layout;
+
+// The GMRES workspace of the last refinement per precision (basis, H, Gram
+// triangle, partials), kept for the next solve of the same layout and Krylov
+// capacity, as the reference's CudaGmresContext keeps its workspace.
+template <typename T> struct EngineSlot {
+    std::shared_ptr<const Layout> layout;
+    int cap = 0;
+    std::shared_ptr<GmresEngine<T>> engine;
+};
+
 struct Context {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -143,6 +166,9 @@ struct Context {
     // SpMV autotuning results by matrix shape (n, nnz, value bytes)
     std::map<std::tuple<std::int64_t, std::int64_t, int>, SpmvChoice> spmv_tuned;
     CanonWork canon;
+    SpaiWork spai;
+    EngineSlot<double> engine64;
+    EngineSlot<float> engine32;
 };
 
 // Runs `launch` (one kernel launch on c.stream), timed when the context's
