@@ -968,14 +968,25 @@ std::vector<IrResultHost> ir_run(Context &c, const DevCsr &a64, const DevCsr &m1
     const Tiles tl = L.tiles();
     const int nsys = L.nsys;
     const std::size_t nt = (std::size_t)std::max(1, L.ntiles);
-    DeviceArray<IrSysState> st(nsys);
-    DeviceArray<double> part(kIrNumSlots * 2 * nt), part4(4 * nt), flags(2 * nt), upd(nt);
-    DeviceArray<double> hist((std::size_t)nsys * kMaxOuterHistory * 4);
-    DeviceArray<int> inner((std::size_t)nsys * kMaxOuterHistory * 3);
-    DeviceArray<double> r(L.n), d(L.n);
-    HIP_CHECK(hipMemsetAsync(hist.data(), 0, hist.size() * sizeof(double), s));
-    HIP_CHECK(hipMemsetAsync(inner.data(), 0, inner.size() * sizeof(int), s));
-    IrArgs a{st.data(), part.data(), part4.data(), flags.data(), upd.data(), hist.data(),
+    IrWork &w = c.ir;
+    w.st.resize((std::size_t)nsys * sizeof(IrSysState));
+    IrSysState *const st = w.st.as<IrSysState>();
+    DeviceArray<double> &part = w.part, &part4 = w.part4, &flags = w.flags, &upd = w.upd;
+    DeviceArray<double> &hist = w.hist, &r = w.r, &d = w.d;
+    DeviceArray<int> &inner = w.inner;
+    part.resize(kIrNumSlots * 2 * nt);
+    part4.resize(4 * nt);
+    flags.resize(2 * nt);
+    upd.resize(nt);
+    const std::size_t nhist = (std::size_t)nsys * kMaxOuterHistory * 4;
+    const std::size_t ninner = (std::size_t)nsys * kMaxOuterHistory * 3;
+    hist.resize(nhist);
+    inner.resize(ninner);
+    r.resize(L.n);
+    d.resize(L.n);
+    HIP_CHECK(hipMemsetAsync(hist.data(), 0, nhist * sizeof(double), s));
+    HIP_CHECK(hipMemsetAsync(inner.data(), 0, ninner * sizeof(int), s));
+    IrArgs a{st, part.data(), part4.data(), flags.data(), upd.data(), hist.data(),
              inner.data(), 10.0 * std::ldexp(1.0, -53), o.relres_target};
     ir_setup(s, tl, a64.ptr.data(), a64.val.data(), b, a);
     // x0 = M b with M the preconditioner (src/ir.cpp:149), in the reference's
@@ -996,7 +1007,7 @@ std::vector<IrResultHost> ir_run(Context &c, const DevCsr &a64, const DevCsr &m1
     const int kcap = (int)std::min<std::int64_t>(cap, maxrows);
     GmresEngine<double> *g64 = nullptr;
     GmresEngine<float> *g32 = nullptr;
-    DeviceArray<float> rf;
+    DeviceArray<float> &rf = w.rf;
     if (o.precision == 0)
         g64 = &cached_engine(c, c.engine64, a64.layout, kcap);
     else {
@@ -1005,7 +1016,7 @@ std::vector<IrResultHost> ir_run(Context &c, const DevCsr &a64, const DevCsr &m1
     }
     std::vector<IrSysState> host(nsys);
     auto any_active = [&]() {
-        download(c, host.data(), st.data(), nsys);
+        download(c, host.data(), st, nsys);
         sync(c);
         for (const auto &h : host)
             if (h.active)
@@ -1015,11 +1026,11 @@ std::vector<IrResultHost> ir_run(Context &c, const DevCsr &a64, const DevCsr &m1
     bool running = any_active();
     for (int step = 0; step < o.outer && running; ++step) {
         if (o.precision == 0) {
-            g64->solve(m1, m2, mode, r.data(), nullptr, o.tol, d.data(), st.data());
+            g64->solve(m1, m2, mode, r.data(), nullptr, o.tol, d.data(), st);
             ir_absorb_inner<double>(s, nsys, g64->args(), a, 0);
         } else {
             ir_scale_rhs(s, tl, r.data(), rf.data(), a);
-            g32->solve(m1, m2, mode, rf.data(), nullptr, o.tol, d.data(), st.data());
+            g32->solve(m1, m2, mode, rf.data(), nullptr, o.tol, d.data(), st);
             ir_absorb_inner<float>(s, nsys, g32->args(), a, 1);
         }
         ir_update(s, tl, x, d.data(), a);
@@ -1027,11 +1038,11 @@ std::vector<IrResultHost> ir_run(Context &c, const DevCsr &a64, const DevCsr &m1
                        step + 1);
         running = any_active();
     }
-    ir_cap(s, nsys, st.data());
+    ir_cap(s, nsys, st);
     std::vector<IrResultHost> out(nsys);
-    std::vector<double> hh(hist.size());
-    std::vector<int> hi(inner.size());
-    download(c, host.data(), st.data(), nsys);
+    std::vector<double> hh(nhist);
+    std::vector<int> hi(ninner);
+    download(c, host.data(), st, nsys);
     download(c, hh.data(), hist.data(), hh.size());
     download(c, hi.data(), inner.data(), hi.size());
     sync(c);

@@ -146,6 +146,15 @@ struct SpaiWork {
 template <typename T> class GmresEngine;
 struct Layout;
 
+// Scratch of the refinement loop (per-system state, partials, history, the
+// residual and correction vectors), grown on demand.
+struct IrWork {
+    DeviceBuffer st;
+    DeviceArray<double> part, part4, flags, upd, hist, r, d;
+    DeviceArray<int> inner;
+    DeviceArray<float> rf;
+};
+
 // The GMRES workspace of the last refinement per precision (basis, H, Gram
 // triangle, partials), kept for the next solve of the same layout and Krylov
 // capacity, as the reference's CudaGmresContext keeps its workspace.
@@ -169,6 +178,7 @@ struct Context {
     SpaiWork spai;
     EngineSlot<double> engine64;
     EngineSlot<float> engine32;
+    IrWork ir;
 };
 
 // Runs `launch` (one kernel launch on c.stream), timed when the context's
