@@ -152,6 +152,24 @@ void DevCsr::ensure32(const Context &c) {
 // matrix shape (n, nnz, value type), so a rebuilt preconditioner of the same
 // shape reuses it.  Long-row matrices keep their lane-group choice.
 // BSPAI_SPMV_AUTOTUNE=0 turns tuning off (the row-length rule decides).
+const short *DevCsr::ensure_col16(const Context &c) const {
+    if (c16 == 0) {
+        c16 = -1;
+        if (nnz > 0 && layout) {
+            col16.resize((std::size_t)nnz);
+            DeviceArray<int> bad(1);
+            HIP_CHECK(hipMemsetAsync(bad.data(), 0, sizeof(int), c.stream));
+            launch_encode_col16(c.stream, layout->tiles(), ptr.data(), col.data(), col16.data(),
+                                bad.data());
+            int hb = 1;
+            download(c, &hb, bad.data(), 1);
+            sync(c);
+            c16 = hb ? -1 : 1;
+        }
+    }
+    return c16 > 0 ? col16.data() : nullptr;
+}
+
 template <typename T> SpmvChoice spmv_choice(Context &c, const DevCsr &m) {
     SpmvChoice rule{m.lanes, stream_per(m.n, m.nnz)};
     if (m.lanes != 1 || m.n == 0)
@@ -159,11 +177,22 @@ template <typename T> SpmvChoice spmv_choice(Context &c, const DevCsr &m) {
     if (const char *env = std::getenv("BSPAI_SPMV_AUTOTUNE"))
         if (std::string(env) == "0")
             return rule;
+    // BSPAI_SPMV_C16=1 takes the streaming kernel on 16-bit column offsets
+    // whenever the matrix allows, =0 the row-length rule on 32-bit columns;
+    // either bypasses the tuning (for comparisons).
+    if (const char *env = std::getenv("BSPAI_SPMV_C16")) {
+        if (std::string(env) == "1" && m.ensure_col16(c))
+            return SpmvChoice{1, rule.per, true};
+        return rule;
+    }
     const auto key = std::make_tuple(m.n, m.nnz, (int)sizeof(T));
     const auto it = c.spmv_tuned.find(key);
     if (it != c.spmv_tuned.end())
         return it->second;
-    const SpmvChoice cand[3] = {{1, 8}, {1, 16}, {-1, 8}};
+    const short *c16 = m.ensure_col16(c);
+    const SpmvChoice all[5] = {{1, 8}, {1, 16}, {-1, 8}, {1, 8, true}, {1, 16, true}};
+    const int ncand = c16 ? 5 : 3;
+    const SpmvChoice *cand = all;
     DeviceArray<T> x((std::size_t)m.n), y((std::size_t)m.n);
     HIP_CHECK(hipMemsetAsync(x.data(), 0, (std::size_t)m.n * sizeof(T), c.stream));
     hipEvent_t e0, e1;
@@ -172,10 +201,12 @@ template <typename T> SpmvChoice spmv_choice(Context &c, const DevCsr &m) {
     const Tiles tl = m.layout->tiles();
     SpmvChoice best = rule;
     float best_ms = 0;
-    for (const SpmvChoice &k : cand) {
+    for (int ci = 0; ci < ncand; ++ci) {
+        const SpmvChoice &k = cand[ci];
         auto run = [&] {
             launch_spmv<T, T, T, T>(c.stream, k.lanes, tl, nullptr, 0, m.ptr.data(),
-                                    m.col.data(), m.values<T>(), x.data(), y.data(), k.per);
+                                    m.col.data(), m.values<T>(), x.data(), y.data(), k.per,
+                                    k.c16 ? c16 : nullptr);
         };
         run();
         HIP_CHECK(hipEventRecord(e0, c.stream));
@@ -841,9 +872,13 @@ void GmresEngine<T>::solve(const DevCsr &m1, const DevCsr &m2, GmresMode mode, c
     const std::string lupd = std::string("gram_update_kernel<") + tname + ">";
     const std::string lnrm = std::string("normalize_givens_kernel<") + tname + ">";
     const double sz = sizeof(T), nn = (double)L_.n;
-    auto spmv_bytes = [&](const DevCsr &m) {
-        return (double)m.nnz * (sz + 4) + 4.0 * (double)(m.n + 1) + 2.0 * (double)m.n * sz;
+    // (the column index takes 2 bytes on 16-bit offsets, else 4)
+    auto spmv_bytes = [&](const DevCsr &m, const SpmvChoice &k) {
+        return (double)m.nnz * (sz + (k.c16 ? 2 : 4)) + 4.0 * (double)(m.n + 1) +
+               2.0 * (double)m.n * sz;
     };
+    const short *c16a = k1.c16 ? m1.ensure_col16(c_) : nullptr;
+    const short *c16b = k2.c16 ? m2.ensure_col16(c_) : nullptr;
     // Step k's kernels are gated by run[k & 1] (active and not held).
     auto run_flag = [&](int k) {
         return reinterpret_cast<const int *>(reinterpret_cast<const char *>(st) +
@@ -885,13 +920,13 @@ void GmresEngine<T>::solve(const DevCsr &m1, const DevCsr &m2, GmresMode mode, c
         ensure_steps(k);
         args_.defer = defer ? 1 : 0;
         const int *run = run_flag(k);
-        timed(c_, l1.c_str(), spmv_bytes(m1), 2.0 * (double)m1.nnz, [&] {
+        timed(c_, l1.c_str(), spmv_bytes(m1, k1), 2.0 * (double)m1.nnz, [&] {
             launch_spmv<T, T, T, T>(s, k1.lanes, tl, run, stride, m1.ptr.data(), m1.col.data(),
-                                    m1v, vptrs_[k], t_.data(), k1.per);
+                                    m1v, vptrs_[k], t_.data(), k1.per, c16a);
         });
-        timed(c_, l2.c_str(), spmv_bytes(m2), 2.0 * (double)m2.nnz, [&] {
+        timed(c_, l2.c_str(), spmv_bytes(m2, k2), 2.0 * (double)m2.nnz, [&] {
             launch_spmv<T, T, T, T>(s, k2.lanes, tl, run, stride, m2.ptr.data(), m2.col.data(),
-                                    m2v, t_.data(), w_.data(), k2.per);
+                                    m2v, t_.data(), w_.data(), k2.per, c16b);
         });
         timed(c_, lproj.c_str(), (k + 3) * nn * sz, 2.0 * (2 * k + 1) * nn,
               [&] { gmres_project<T>(s, tl, args_, k); });

@@ -119,6 +119,7 @@ class KernelTimer {
 // streaming kernel).
 struct SpmvChoice {
     int lanes = 1, per = 8;
+    bool c16 = false; // streaming kernel on 16-bit column offsets
 };
 
 // Scratch of the coordinate-entry canonicalization, kept by the context and
@@ -219,7 +220,12 @@ struct DevCsr {
     bool has32 = false, has64 = true;
     int lanes = 1; // SpMV lanes per row
     std::shared_ptr<Layout> layout;
+    // 16-bit column offsets for the streaming SpMV (made on first use):
+    // c16 0 = not tried, 1 = available, -1 = an offset does not fit.
+    mutable DeviceArray<short> col16;
+    mutable int c16 = 0;
     void ensure32(const Context &c);
+    const short *ensure_col16(const Context &c) const;
     template <typename T> const T *values() const;
 };
 template <> inline const double *DevCsr::values<double>() const { return val.data(); }

@@ -108,10 +108,16 @@ constexpr int kSpmvSub = kTileRows / kSpmvRows;
 // flag (optional): a tile runs only if flag[sys * flag_stride] != 0.
 // stream_per (lanes = 1): entries per thread per chunk, 8 or 16 (for matrices
 // averaging more than 8 entries per row); the result does not depend on it.
+// col16 (lanes = 1, optional): the columns as 16-bit offsets to each 256-row
+// block's first row (launch_encode_col16); the result does not depend on it.
 template <typename TV, typename TX, typename TA, typename TY>
 void launch_spmv(hipStream_t s, int lanes, const Tiles &tiles, const int *flag, int flag_stride,
                  const int *ptr, const int *col, const TV *val, const TX *x, TY *y,
-                 int stream_per = 8);
+                 int stream_per = 8, const short *col16 = nullptr);
+// col16[e] = col[e] - (first row of the SpMV row block holding entry e);
+// *bad (zeroed by the caller) is set when an offset does not fit 16 bits.
+void launch_encode_col16(hipStream_t s, const Tiles &tiles, const int *ptr, const int *col,
+                         short *col16, int *bad);
 // Entries per thread for the streaming SpMV of an n x n matrix with nnz entries.
 inline int stream_per(long long n, long long nnz) {
     return nnz > 8 * n ? 16 : 8;

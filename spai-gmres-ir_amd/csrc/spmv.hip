@@ -54,11 +54,15 @@ spmv_group_kernel(Tiles tiles, const int *__restrict__ flag, int flag_stride,
 // reference's sequential row sum (src/cpu.cpp:88-89), bit for bit.
 // PER entries per thread per chunk: 8 for rows of up to ~8 entries, 16 for
 // longer ones, so that a block's rows usually fit one chunk.
-template <typename TV, typename TX, typename TA, typename TY, int PER>
+// With C16 the columns come from 16-bit offsets to the block's first row
+// (col16[e] = col[e] - rbeg, see encode_col16_kernel): 2 bytes per entry
+// instead of 4, the same column and the same sums.
+template <typename TV, typename TX, typename TA, typename TY, int PER, bool C16>
 __global__ void __launch_bounds__(kSpmvRows)
 spmv_stream_kernel(Tiles tiles, const int *__restrict__ flag, int flag_stride,
                    const int *__restrict__ ptr, const int *__restrict__ col,
-                   const TV *__restrict__ val, const TX *__restrict__ x, TY *__restrict__ y) {
+                   const short *__restrict__ col16, const TV *__restrict__ val,
+                   const TX *__restrict__ x, TY *__restrict__ y) {
     constexpr int kStreamChunk = kSpmvRows * PER;
     __shared__ int rp[kSpmvRows + 1];
     __shared__ TA prod[kStreamChunk];
@@ -88,7 +92,10 @@ spmv_stream_kernel(Tiles tiles, const int *__restrict__ flag, int flag_stride,
 #pragma unroll
         for (int u = 0; u < per; ++u) {
             const int e = min(c0 + tid + u * kSpmvRows, c1 - 1);
-            cj[u] = col[e];
+            if constexpr (C16)
+                cj[u] = rbeg + (int)col16[e];
+            else
+                cj[u] = col[e];
             vv[u] = val[e];
         }
         TX xv[per];
@@ -111,10 +118,41 @@ spmv_stream_kernel(Tiles tiles, const int *__restrict__ flag, int flag_stride,
         y[rbeg + tid] = (TY)acc;
 }
 
+// Row blocks of the SpMV kernels: block b of tile t covers rows
+// [row0[t] + b kSpmvRows, ...).  col16[e] = col[e] - (first row of e's block)
+// when every such offset fits 16 bits; *bad is set otherwise.
+__global__ void __launch_bounds__(kSpmvRows)
+encode_col16_kernel(Tiles tiles, const int *__restrict__ ptr, const int *__restrict__ col,
+                    short *__restrict__ col16, int *__restrict__ bad) {
+    const int t = blockIdx.x / kSpmvSub, sb = blockIdx.x % kSpmvSub;
+    const int rbeg = tiles.row0[t] + sb * kSpmvRows;
+    const int rend = min(tiles.row1[t], rbeg + kSpmvRows);
+    if (rbeg >= rend)
+        return;
+    const int e1 = ptr[rend];
+    bool over = false;
+    for (int e = ptr[rbeg] + threadIdx.x; e < e1; e += kSpmvRows) {
+        const int d = col[e] - rbeg;
+        over = over || d < -32768 || d > 32767;
+        col16[e] = (short)d;
+    }
+    if (over)
+        *bad = 1;
+}
+
+void launch_encode_col16(hipStream_t s, const Tiles &tiles, const int *ptr, const int *col,
+                         short *col16, int *bad) {
+    if (tiles.ntiles == 0)
+        return;
+    hipLaunchKernelGGL(encode_col16_kernel, dim3((unsigned)tiles.ntiles * kSpmvSub),
+                       dim3(kSpmvRows), 0, s, tiles, ptr, col, col16, bad);
+    HIP_LAUNCH_CHECK();
+}
+
 template <typename TV, typename TX, typename TA, typename TY>
 void launch_spmv(hipStream_t s, int lanes, const Tiles &tiles, const int *flag, int flag_stride,
                  const int *ptr, const int *col, const TV *val, const TX *x, TY *y,
-                 int stream_per) {
+                 int stream_per, const short *col16) {
     if (tiles.ntiles == 0)
         return;
     const dim3 grid((unsigned)tiles.ntiles * kSpmvSub), block(kSpmvRows);
@@ -125,12 +163,18 @@ void launch_spmv(hipStream_t s, int lanes, const Tiles &tiles, const int *flag, 
         break;
     switch (lanes) {
     case 1:
-        if (stream_per > 8)
-            hipLaunchKernelGGL((spmv_stream_kernel<TV, TX, TA, TY, 16>), grid, block, 0, s, tiles,
-                               flag, flag_stride, ptr, col, val, x, y);
+        if (col16 && stream_per > 8)
+            hipLaunchKernelGGL((spmv_stream_kernel<TV, TX, TA, TY, 16, true>), grid, block, 0, s,
+                               tiles, flag, flag_stride, ptr, col, col16, val, x, y);
+        else if (col16)
+            hipLaunchKernelGGL((spmv_stream_kernel<TV, TX, TA, TY, 8, true>), grid, block, 0, s,
+                               tiles, flag, flag_stride, ptr, col, col16, val, x, y);
+        else if (stream_per > 8)
+            hipLaunchKernelGGL((spmv_stream_kernel<TV, TX, TA, TY, 16, false>), grid, block, 0, s,
+                               tiles, flag, flag_stride, ptr, col, col16, val, x, y);
         else
-            hipLaunchKernelGGL((spmv_stream_kernel<TV, TX, TA, TY, 8>), grid, block, 0, s, tiles,
-                               flag, flag_stride, ptr, col, val, x, y);
+            hipLaunchKernelGGL((spmv_stream_kernel<TV, TX, TA, TY, 8, false>), grid, block, 0, s,
+                               tiles, flag, flag_stride, ptr, col, col16, val, x, y);
         break;
     case -1:
         hipLaunchKernelGGL((spmv_group_kernel<1, TV, TX, TA, TY>), grid, block, 0, s, tiles, flag,
@@ -152,7 +196,7 @@ void launch_spmv(hipStream_t s, int lanes, const Tiles &tiles, const int *flag, 
 #define BSPAI_SPMV_INST(TV, TX, TA, TY)                                                           \
     template void launch_spmv<TV, TX, TA, TY>(hipStream_t, int, const Tiles &, const int *, int,  \
                                               const int *, const int *, const TV *, const TX *,   \
-                                              TY *, int);
+                                              TY *, int, const short *);
 BSPAI_SPMV_INST(double, double, double, double)
 BSPAI_SPMV_INST(float, float, float, float)
 BSPAI_SPMV_INST(float, double, double, double)

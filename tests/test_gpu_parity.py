@@ -444,6 +444,41 @@ def test_spmv_autotune_keeps_bits(monkeypatch, precision):
     assert np.array_equal(bits(np.asarray(out["0"].solution)), bits(np.asarray(out["1"].solution)))
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_spmv_col16_keeps_bits(monkeypatch, precision):
+    """The streaming SpMV on 16-bit column offsets (2 bytes per entry instead
+    of 4) gives the same columns and sums: IR solves with it forced and with
+    32-bit columns return the same bits and iteration counts (32^3
+    convection-diffusion, whose offsets to a 256-row block fit 16 bits).  A
+    matrix with a coupling 40,000 columns away cannot use the offsets, and
+    keeps 32-bit columns and the same bits."""
+    import bspai_hip
+
+    P = _problems()
+    n = 50000
+    i = np.arange(n)
+    far = P._from_coo(n, np.concatenate([i, i[1:], i[:-1], i]),
+                      np.concatenate([i, i[:-1], i[1:], (i + 40000) % n]),
+                      np.concatenate([np.full(n, 4.0), np.full(n - 1, -1.0),
+                                      np.full(n - 1, -1.2), np.full(n, 0.5)]))
+    for a, kw in ((P.convection_diffusion_3d(32),
+                   dict(tolerance=1e-8 if precision == "fp64" else 2e-5, max_iterations=200,
+                        relres_target=1e-12)),
+                  (far, dict(tolerance=1e-8 if precision == "fp64" else 1e-4,
+                             max_iterations=100, relres_target=1e-12))):
+        b = P.equal_rhs(a.n)
+        out = {}
+        for mode in ("0", "1"):
+            monkeypatch.setenv("BSPAI_SPMV_C16", mode)
+            c = bspai_hip.Context(0)
+            out[mode] = bspai_hip.spai_gmres_ir(bspai_hip.Matrix(a, c), b, precision=precision,
+                                                **kw)
+            assert out[mode].status == "success"
+        assert out["0"].inner_iterations == out["1"].inner_iterations
+        assert np.array_equal(bits(np.asarray(out["0"].solution)),
+                              bits(np.asarray(out["1"].solution)))
+
+
 @pytest.mark.parametrize("case", ["c1_fp64", "c1_mixed_floor", "cube32_mixed", "c1_mixed_cap"])
 def test_deferred_verification_same_as_every_step(monkeypatch, case):
     """A single system verifies triggered steps at the host's check points
