@@ -495,7 +495,7 @@ void spai_build_typed(Context &c, const DevCsr &a, const T *aval, T *d, double e
                                      bptr.data(), bcol.data(), (T)eps, alpha, beta, work.data(),
                                      nwork, counter.data(), mcap, qcap, ccap, dwork.as<T>(),
                                      iwork.as<int>(), o, out.pool_j.back().data(),
-                                     out.pool_c.back().as<T>(), qcap);
+                                     out.pool_c.back().as<T>(), qcap, pass == 1 ? 4 : 3);
             }, true);
             hipLaunchKernelGGL(collect_overflow_kernel, dim3(ceil_div(nwork, 256)), dim3(256), 0,
                                s, nwork, work.data(), out.c_status.data(), next.data(),

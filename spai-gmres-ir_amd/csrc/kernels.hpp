@@ -65,7 +65,7 @@ void launch_spai_block(hipStream_t s, int blocks, int n, const int *aptr, const 
                        const T *aval, const T *dscale, const int *bptr, const int *bcol, T eps,
                        long long alpha, int beta, const int *worklist, int nwork, int *cursor,
                        int mcap, int qcap, int ccap, T *dwork, int *iwork, SpaiColumnOut out,
-                       int *out_j, T *out_c, std::int64_t out_stride);
+                       int *out_j, T *out_c, std::int64_t out_stride, int occupancy);
 template <typename T>
 void launch_scale_rows(hipStream_t s, int n, const int *ptr, const int *col, const T *val, T *d,
                        int *first_bad);
