@@ -32,10 +32,8 @@ def parse():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--side", type=int, default=128)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--cpu-columns", type=int, default=60000,
-                    help="SPAI columns in the CPU-baseline sample")
-    ap.add_argument("--cpu-iters", type=int, default=32,
-                    help="GMRES iterations in the CPU-baseline sample")
+    ap.add_argument("--cpu-c1-trials", type=int, default=3,
+                    help="warm-up and measured trials of the reference on the 64x64 system")
     ap.add_argument("--extra", default="",
                     help="comma list of extra workloads to time beside the headline: "
                          "'many' (256 systems batched vs looped), 'powerlaw' (uneven columns), "
@@ -44,8 +42,10 @@ def parse():
                     help="rows of the power-law matrix timed by --extra powerlaw")
     ap.add_argument("--pmc-workload", action="store_true",
                     help="run the short workload profiled for HBM traffic counters instead of "
-                         "the benchmark: a 1 GiB device copy (counter calibration), one mixed "
-                         "SPAI build and 60 mixed GMRES iterations on the same system")
+                         "the benchmark: a 1 GiB device copy (counter calibration), one "
+                         "SPAI build and 150 GMRES iterations on the same system")
+    ap.add_argument("--pmc-precision", default="mixed", choices=["mixed", "fp64"],
+                    help="precision of the --pmc-workload build and iterations")
     return ap.parse_args()
 
 
@@ -85,14 +85,15 @@ def pmc_workload(args):
     torch.cuda.synchronize()
     a = P.convection_diffusion_3d(args.side)
     m = bspai_hip.Matrix(a, ctx)
-    s = bspai_hip.construct_spai(m, 0.3, -(-a.n // 8), 8, precision="mixed")
+    prec = args.pmc_precision
+    s = bspai_hip.construct_spai(m, 0.3, -(-a.n // 8), 8, precision=prec)
     b = torch.from_numpy(P.equal_rhs(a.n)).to(dev)
     # 150 Krylov steps (k = 0..149, mean 74.5): one outer step of the
     # benchmark's solve, so that per-launch counts compare with its averages.
     r = bspai_hip.refine_with_preconditioner(m, s, b, max_iterations=150, max_outer_steps=1,
-                                             precision="mixed", tolerance=1e-30)
+                                             precision=prec, tolerance=1e-30)
     torch.cuda.synchronize()
-    print(json.dumps({"pmc_workload": "done", "spai": s.status,
+    print(json.dumps({"pmc_workload": "done", "precision": prec, "spai": s.status,
                       "inner_iterations": r.inner_iterations}))
 
 
@@ -137,7 +138,7 @@ def main():
     # FP64 uses the reference's default 1e-8 (include/bspai/gmres.hpp:24).
     paths = {"mixed": dict(precision="mixed", tolerance=1.5e-5),
              "fp64": dict(precision="fp64", tolerance=1e-8)}
-    results, kernels, live = {}, {}, {}
+    results, kernels, live, solutions = {}, {}, {}, {}
     for name, kw in paths.items():
         last = None
         for _ in range(args.warmup):
@@ -163,6 +164,7 @@ def main():
         ctx.set_timing(False)
         last = runs[-1]
         x = last.solution.cpu().numpy()
+        solutions[name] = x
         relres = float(np.linalg.norm(b_host - A_sp @ x) / np.linalg.norm(b_host))
         ok = all(r.status == "success" for r in runs) and relres <= 1e-12
         results[name] = dict(
@@ -192,11 +194,23 @@ def main():
     if timed_live:
         roofline["share_of_timed_region"] = round(
             dominant["total_ms"] / (results["mixed"]["ms_per_solve"] * args.steps), 4)
-    # ---- CPU baseline: the oracle (scalar port) on a bounded sample
-    cpu = None
+    # ---- the FP32 preconditioner against the FP64 one (same device build
+    # the benchmark solves with)
+    s64 = bspai_hip.construct_spai(m, 0.3, -(-n // 8), 8)
+    s32 = bspai_hip.construct_spai(m, 0.3, -(-n // 8), 8, precision="mixed")
+    p64, p32 = s64.preconditioner(), s32.preconditioner()
+    ndiff, colerr = compare_rows(p32, p64)
+    precision_report = {"fp32_spai_vs_fp64_spai": {
+        "pattern_differing_columns": ndiff, "columns": n,
+        "max_column_rel_error_on_common_patterns": colerr},
+        "inner_iterations": {"fp64": results["fp64"]["inner_iterations"],
+                             "mixed": results["mixed"]["inner_iterations"]}}
+    # ---- CPU baseline: one full oracle solve of the same system, timed
+    cpu = cpu_c1 = None
     if not args.no_cpu:
-        cpu = cpu_baseline(a, b_host, ctx, m, args.cpu_columns, args.cpu_iters,
-                           results["fp64"]["inner_iterations"])
+        cpu, oracle_check = cpu_baseline(a, b_host, p64, p32, solutions)
+        precision_report["against_cpu_oracle"] = oracle_check
+        cpu_c1 = cpu_baseline_c1(args.cpu_c1_trials)
     mixed, fp64 = results["mixed"], results["fp64"]
     line = {
         "metric": "solves/s to rel-res 1e-12: mixed FP32/FP64 vs plain FP64 GPU; SpMV % HBM peak",
@@ -227,6 +241,8 @@ def main():
                                               "achieved", "unit", "frac")}
                            for r in rows[:8]] for name, rows in roof.items()},
         "cpu_baseline": cpu,
+        "cpu_baseline_c1": cpu_c1,
+        "precision_report": precision_report,
     }
     extras = [e for e in args.extra.split(",") if e]
     if extras:
@@ -364,8 +380,8 @@ PEAK_TFLOPS = {"double": 78.6, "float": 157.3}
 def kernel_rooflines(kt):
     """Roofline rows from the library's per-kernel event totals, largest share
     first: HBM-bound kernels report algorithmic GB/s against 8 TB/s; the SPAI
-    column kernel (dense QR) reports algorithmic TFLOP/s against the FP64 or
-    FP32 peak."""
+    column kernels (dense QR on the vector units) report algorithmic TFLOP/s
+    against the FP64 or FP32 peak."""
     total = sum(v["total_ms"] for v in kt.values()) or 1.0
     rows = []
     for name, v in kt.items():
@@ -373,7 +389,9 @@ def kernel_rooflines(kt):
         if name.startswith(("spai_columns_kernel", "spai_wave_kernel", "spai_block_kernel")):
             prec = "float" if "<float>" in name else "double"
             ach = v["flops"] / sec / 1e12 if sec > 0 else 0.0
-            bound, unit, peak = "mfma", "TFLOP/s", PEAK_TFLOPS[prec]
+            # vector-ALU code (no MFMA instruction): priced against the
+            # chip's FP32 / FP64 peak, which the vector units share
+            bound, unit, peak = "valu", "TFLOP/s", PEAK_TFLOPS[prec]
             per = {"flops": v["flops"] / max(1, v["launches"])}
         else:
             ach = v["bytes"] / sec / 1e9 if sec > 0 else 0.0
@@ -387,46 +405,131 @@ def kernel_rooflines(kt):
     return rows
 
 
-def cpu_baseline(a, b_host, ctx, m, ncols, iters, gpu_inner):
-    """Time the oracle on a sample and scale to solves/s.
+def log(msg):
+    """Progress on stderr (the JSON line alone goes to stdout)."""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
-    Sample: SPAI for `ncols` columns spread over the grid (columns are
-    independent; cost scales with n/ncols) plus two truncated GMRES solves of
-    k/2 and k iterations on the full system with the bit-identical P, which fix
-    the per-iteration cost model t(k) = c0*k + c1*k^2 (SpMV + Gram-Schmidt);
-    the model is evaluated at the inner iteration counts of the FP64 solve."""
+
+def compare_rows(p_a, p_b):
+    """Rows of two preconditioners P (rows of P are the columns of M): the
+    number whose patterns differ, and the largest per-row normwise relative
+    difference max|a - b| / max|b| over the rows whose patterns agree."""
+    import numpy as np
+
+    ptr_a, col_a, val_a = (np.asarray(x) for x in p_a)
+    ptr_b, col_b, val_b = (np.asarray(x) for x in p_b)
+    n = len(ptr_b) - 1
+    la, lb = np.diff(ptr_a), np.diff(ptr_b)
+    rows = np.nonzero((la == lb) & (lb > 0))[0]
+    L = lb[rows]
+    offs = np.arange(L.sum()) - np.repeat(np.cumsum(L) - L, L)
+    ia = np.repeat(ptr_a[:-1][rows], L) + offs
+    ib = np.repeat(ptr_b[:-1][rows], L) + offs
+    starts = np.cumsum(L) - L
+    same = np.add.reduceat((col_a[ia] != col_b[ib]).astype(np.int64), starts) == 0
+    num = np.maximum.reduceat(np.abs(val_a[ia] - val_b[ib]), starts)
+    den = np.maximum.reduceat(np.abs(val_b[ib]), starts)
+    rel = (num / den)[same]
+    return int(n - same.sum()), float(rel.max()) if rel.size else 0.0
+
+
+def cpu_baseline(a, b_host, p64, p32, solutions):
+    """One complete solve of the benchmark system by the oracle (the CPU FP64
+    restatement of the reference's cpu_fp64_ir path, bit-identical to it) on
+    one host core, timed with perf_counter around the public calls: SPAI
+    construction, then refinement with that preconditioner (outercap 20,
+    innercap 200, tau 1e-8, the reference's stop).  The GPU results are then
+    checked against it: FP64 P bits, FP32 P pattern differences and column
+    errors, and both solutions."""
     import numpy as np
 
     sys.path.insert(0, REPO)
     from oracle import pyoracle as O
     from bspai_hip import problems as P
-    import bspai_hip
 
     n = a.n
-    cols = np.linspace(0, n - 1, ncols).astype(np.int64)
+    log(f"cpu baseline: oracle SPAI on all {n} columns, one core")
     t0 = time.perf_counter()
-    O.spai_subset(a, cols, epsilon=0.3)
-    t_spai = (time.perf_counter() - t0) * n / ncols
-    s = bspai_hip.construct_spai(m, 0.3, -(-n // 8), 8)
-    pp, pc, pv = s.preconditioner()
-    p = P.Csr(n, pp, pc, pv)
-    k1, k2 = max(2, iters // 2), iters
+    s = O.construct_spai(a, 0.3, -(-n // 8), 8)
+    t_spai = time.perf_counter() - t0
+    log(f"cpu baseline: SPAI {t_spai:.1f} s ({s.status}); refinement")
     t0 = time.perf_counter()
-    O.gmres(a, p, b_host, 1e-30, k1)
-    t1 = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    O.gmres(a, p, b_host, 1e-30, k2)
-    t2 = time.perf_counter() - t0
-    # t(k) = c0 k + c1 k^2
-    c1 = (t2 / k2 - t1 / k1) / (k2 - k1)
-    c0 = t1 / k1 - c1 * k1
-    t_ir = sum(c0 * k + c1 * k * k for k in gpu_inner)
+    r = O.spai_gmres_ir(a, b_host, 0.3, tolerance=1e-8, max_iterations=200, max_outer_steps=20,
+                        p=P.Csr(n, s.p_ptr, s.p_col, s.p_val))
+    t_ir = time.perf_counter() - t0
+    log(f"cpu baseline: refinement {t_ir:.1f} s ({r.status}, inner {r.inner_iterations})")
     total = t_spai + t_ir
-    return {"value": round(1.0 / total, 6), "unit": "solves/s", "cores": 1, "kind": "port",
-            "sample": f"oracle SPAI on {ncols} of {n} columns (scaled by n/{ncols}) + oracle "
-                      f"GMRES truncated at {k1} and {k2} iterations on the full system "
-                      f"(t(k)=c0 k+c1 k^2 fit, evaluated at inner counts {gpu_inner}); "
-                      f"estimated {t_spai:.1f} s SPAI + {t_ir:.1f} s IR per solve"}
+    x_ref = np.asarray(r.solution)
+    relres = float(np.linalg.norm(b_host - a.to_scipy() @ x_ref) / np.linalg.norm(b_host))
+    ref_p = (s.p_ptr, s.p_col, s.p_val)
+    nd32, err32 = compare_rows(p32, ref_p)
+    check = {
+        "oracle": {"status": r.status, "outer_steps": r.outer_steps,
+                   "inner_iterations": r.inner_iterations, "relres": relres,
+                   "nbe": float(r.history[-1][2]) if len(r.history) else None},
+        "fp64_spai_bits_equal_oracle": bool(
+            np.array_equal(np.asarray(p64[0]), s.p_ptr) and np.array_equal(np.asarray(p64[1]), s.p_col)
+            and np.array_equal(np.asarray(p64[2]).view(np.uint64), s.p_val.view(np.uint64))),
+        "fp32_spai_vs_oracle": {"pattern_differing_columns": nd32,
+                                "max_column_rel_error_on_common_patterns": err32},
+        "solution_max_rel_diff_vs_oracle": {
+            k: float(np.max(np.abs(x - x_ref)) / np.max(np.abs(x_ref)))
+            for k, x in solutions.items()},
+    }
+    base = {"value": round(1.0 / total, 6), "unit": "solves/s", "cores": 1, "kind": "port",
+            "sample": f"one complete solve of the same {n}-row system by the oracle (CPU FP64 "
+                      "restatement of the reference's cpu_fp64_ir path) on one host core, timed: "
+                      f"SPAI {t_spai:.1f} s + refinement {t_ir:.1f} s (inner "
+                      f"{r.inner_iterations}); the reference binary's own SPAI scans every row "
+                      "per fit (O(n^2)) and cannot run at this size",
+            "seconds": round(total, 3), "spai_s": round(t_spai, 3), "ir_s": round(t_ir, 3),
+            "host_cpus": os.cpu_count()}
+    return base, check
+
+
+def cpu_baseline_c1(trials):
+    """The reference itself (its CPU sources compiled into oracle/_ref/ref_time)
+    on the CPU-runnable configuration, the 2-D 64x64 system (eps 0.3, outercap
+    20, innercap n), timed around spai_gmres_ir after `trials` warm-ups; the
+    oracle port when the compiled reference is absent."""
+    import json as _json
+    import subprocess
+    import tempfile
+
+    import numpy as np
+
+    from bspai_hip import problems as P
+
+    a = P.convection_diffusion_2d()
+    exe = os.path.join(REPO, "oracle", "_ref", "ref_time")
+    if os.path.exists(exe):
+        log(f"cpu baseline (64x64): the reference, {trials} + {trials} trials")
+        with tempfile.TemporaryDirectory() as d:
+            mtx = os.path.join(d, "c1.mtx")
+            P.write_matrix_market(mtx, a)
+            out = subprocess.run([exe, mtx, "0.3", "20", "0", str(trials), str(trials)],
+                                 capture_output=True, text=True, check=True, timeout=600)
+        res = _json.loads(out.stdout)
+        kind, secs = "reference", res["seconds"]
+        inner = res["inner"]
+    else:
+        sys.path.insert(0, REPO)
+        from oracle import pyoracle as O
+
+        log(f"cpu baseline (64x64): the oracle port, {trials} + {trials} trials")
+        b = P.equal_rhs(a.n)
+        secs = []
+        for t in range(2 * trials):
+            t0 = time.perf_counter()
+            r = O.spai_gmres_ir(a, b, 0.3)
+            if t >= trials:
+                secs.append(time.perf_counter() - t0)
+        kind, inner = "port", r.inner_iterations
+    med = float(np.median(secs))
+    return {"value": round(1.0 / med, 4), "unit": "solves/s", "cores": 1, "kind": kind,
+            "sample": f"2-D 64x64 convection-diffusion (n=4096), eps 0.3, outercap 20, innercap n, "
+                      f"{trials} warm-up + {trials} measured trials, median; inner {inner}",
+            "seconds": [round(x, 4) for x in secs]}
 
 
 if __name__ == "__main__":

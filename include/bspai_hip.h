@@ -15,6 +15,9 @@
  * Reference interfaces replaced (reference tree, file:line):
  *   bspai_hip_matrix_create / bspai_hip_spmv    CudaCsrSpmv      include/bspai/cuda_spmv.hpp:7-19
  *   bspai_hip_csr_from_entries                  CsrMatrix::from_entries include/bspai/cpu.hpp:19
+ *   bspai_hip_mm_parse (+ _info/_entries)       read_matrix_market include/bspai/cpu.hpp:47
+ *                                               (the parse; canonicalization is
+ *                                               bspai_hip_csr_from_entries)
  *   bspai_hip_spai_build (+ _info/_columns/...) cuda_construct_spai include/bspai/cuda_spai.hpp:47
  *                                               construct_spai    include/bspai/spai.hpp:69
  *   bspai_hip_score                             cuda_evaluate_spai_score include/bspai/cuda_spai.hpp:48-49
@@ -44,6 +47,7 @@ extern "C" {
 #define BSPAI_HIP_EINVAL 1  /* invalid argument (message in bspai_hip_last_error) */
 #define BSPAI_HIP_EDEVICE 2 /* HIP runtime / device failure */
 #define BSPAI_HIP_ENOMEM 3  /* device allocation failed */
+#define BSPAI_HIP_EFORMAT 4 /* malformed Matrix Market text (message: the reference reader's) */
 
 /* memory kinds */
 #define BSPAI_HIP_HOST 0
@@ -179,6 +183,27 @@ int bspai_hip_csr_from_entries(bspai_hip_ctx *ctx, int64_t rows, int64_t cols, i
                                const int64_t *row, const int64_t *col, const double *val,
                                int where, int32_t *ptr, int32_t *col_out, double *val_out,
                                int out_where, int64_t *nnz);
+
+/* Matrix Market coordinate reader (host code, several threads; no GPU use).
+ * Parses `len` bytes of text with the reference reader's rules and messages
+ * (read_matrix_market, src/cpu.cpp:105-161 of the reference): banner
+ * "%%MatrixMarket matrix coordinate real|integer general|symmetric", '%'
+ * comment and blank lines skipped, one dimension line, exactly `count` entry
+ * lines of "row col value" (one-based), nothing after them.  On success *out
+ * holds the entries in file order, zero-based, a symmetric file's off-diagonal
+ * entries followed by their mirror; they are not canonicalized (duplicates
+ * and zeros kept): pass them to bspai_hip_csr_from_entries.  A malformed file
+ * returns BSPAI_HIP_EFORMAT with the reference's message (a std::runtime_error
+ * there) in `message` (message_len bytes, terminated).  threads: 0 = up to 16,
+ * else that many (the body is split at line breaks). */
+typedef struct bspai_hip_mm bspai_hip_mm;
+int bspai_hip_mm_parse(const char *text, int64_t len, int threads, bspai_hip_mm **out,
+                       char *message, int message_len);
+int bspai_hip_mm_info(const bspai_hip_mm *mm, int64_t *rows, int64_t *cols, int64_t *count,
+                      int *integer_field, int *symmetric);
+/* row, col, val: room for `count` entries each (host arrays). */
+int bspai_hip_mm_entries(const bspai_hip_mm *mm, int64_t *row, int64_t *col, double *val);
+void bspai_hip_mm_destroy(bspai_hip_mm *mm);
 
 /* y = A x in FP64.  lanes: 0 = the matrix's default; 1 = streaming kernel
  * (the block's entries staged in LDS); -1 = one thread per row; 2..64 (a

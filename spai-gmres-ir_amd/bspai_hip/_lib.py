@@ -10,7 +10,7 @@ import os
 PKG_ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB_PATH = os.path.join(PKG_ROOT, "libbspai_hip.so")
 
-OK, EINVAL, EDEVICE, ENOMEM = 0, 1, 2, 3
+OK, EINVAL, EDEVICE, ENOMEM, EFORMAT = 0, 1, 2, 3, 4
 HOST, DEVICE = 0, 1
 FP64, MIXED = 0, 1
 MAX_OUTER = 24
@@ -95,6 +95,11 @@ SIGNATURES = {
     "bspai_hip_csr_from_entries": (C.c_int, [_VP, C.c_int64, C.c_int64, C.c_int64, _VP, _VP,
                                              _VP, C.c_int, _VP, _VP, _VP, C.c_int,
                                              C.POINTER(C.c_int64)]),
+    "bspai_hip_mm_parse": (C.c_int, [C.c_char_p, C.c_int64, C.c_int, _PP, C.c_char_p, C.c_int]),
+    "bspai_hip_mm_info": (C.c_int, [_VP, _I64P, _I64P, _I64P, C.POINTER(C.c_int),
+                                    C.POINTER(C.c_int)]),
+    "bspai_hip_mm_entries": (C.c_int, [_VP, _VP, _VP, _VP]),
+    "bspai_hip_mm_destroy": (None, [_VP]),
     "bspai_hip_spmv": (C.c_int, [_VP, _VP, _VP, _VP, C.c_int, C.c_int]),
     "bspai_hip_spmv_f32": (C.c_int, [_VP, _VP, _VP, _VP, C.c_int, C.c_int]),
     "bspai_hip_spai_build": (C.c_int, [_VP, _VP, C.POINTER(SpaiOptions), _PP]),

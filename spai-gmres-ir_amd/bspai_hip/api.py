@@ -46,11 +46,32 @@ def _where(a):
 
 
 def _host(a, dtype):
+    """The array the C ABI reads: `dtype`, contiguous.  Device tensors stay on
+    the device (converted by torch on its current stream when their dtype or
+    layout differs); host tensors and sequences become numpy arrays."""
     if isinstance(a, np.ndarray):
         return np.ascontiguousarray(a, dtype=dtype)
     if getattr(a, "is_cuda", False):
-        return a
+        import torch
+
+        return a.detach().to(dtype=getattr(torch, np.dtype(dtype).name)).contiguous()
+    if hasattr(a, "detach"):  # a host torch tensor
+        a = a.detach().cpu().numpy()
     return np.ascontiguousarray(np.asarray(a), dtype=dtype)
+
+
+def _len(a) -> int:
+    return len(a) if isinstance(a, np.ndarray) else int(a.numel())
+
+
+def _need(a, n: int, what: str):
+    """Raise ValueError unless `a` holds exactly n elements (a 1-D vector)."""
+    got = _len(a)
+    if got != n or (isinstance(a, np.ndarray) and a.ndim != 1) or \
+            (not isinstance(a, np.ndarray) and a.dim() != 1):
+        raise ValueError(f"{what}: expected a vector of {n} elements, got shape "
+                         f"{tuple(a.shape)}")
+    return a
 
 
 class Context:
@@ -125,8 +146,13 @@ def csr_from_entries(rows: int, cols: int, row, col, val, ctx: Context | None = 
     r = _host(row, np.int64)
     c = _host(col, np.int64)
     v = _host(val, np.float64)
-    count = len(v) if isinstance(v, np.ndarray) else v.numel()
+    count = _len(v)
+    _need(r, count, "row")
+    _need(c, count, "col")
+    _need(v, count, "val")
     where = _where(v)
+    if where == DEVICE and not (getattr(r, "is_cuda", False) and getattr(c, "is_cuda", False)):
+        raise ValueError("row, col and val must all be host arrays or all device tensors")
     if where == DEVICE:
         import torch
 
@@ -144,6 +170,56 @@ def csr_from_entries(rows: int, cols: int, row, col, val, ctx: Context | None = 
         _addr(ov), where, C.byref(nnz)), ctx.h)
     k = nnz.value
     return ptr, oc[:k], ov[:k]
+
+
+class MatrixMarketError(RuntimeError):
+    """A malformed Matrix Market text (the reference reader's std::runtime_error)."""
+
+
+def parse_matrix_market(text, threads: int = 0):
+    """Coordinate entries of a Matrix Market text, parsed on the host by the
+    library's multi-threaded reader (bspai_hip_mm_parse) with the rules and
+    messages of the reference's read_matrix_market (src/cpu.cpp:105-161).
+
+    Returns (rows, cols, row, col, val): zero-based int64 / float64 numpy
+    arrays in file order (a symmetric file's off-diagonal entries followed by
+    their mirror), not yet canonicalized.  Raises MatrixMarketError with the
+    reference's message for a malformed file."""
+    data = text.encode() if isinstance(text, str) else bytes(text)
+    L = _lib.load()
+    h = C.c_void_p()
+    msg = C.create_string_buffer(256)
+    rc = L.bspai_hip_mm_parse(data, len(data), int(threads), C.byref(h), msg, len(msg))
+    if rc == _lib.EFORMAT:
+        raise MatrixMarketError(msg.value.decode())
+    if rc != _lib.OK:
+        raise (MemoryError if rc == _lib.ENOMEM else ValueError)(msg.value.decode())
+    try:
+        rows, cols, count = C.c_int64(), C.c_int64(), C.c_int64()
+        L.bspai_hip_mm_info(h, C.byref(rows), C.byref(cols), C.byref(count), None, None)
+        r = np.empty(count.value, np.int64)
+        c = np.empty(count.value, np.int64)
+        v = np.empty(count.value, np.float64)
+        check(L.bspai_hip_mm_entries(h, r.ctypes.data, c.ctypes.data, v.ctypes.data), None)
+    finally:
+        L.bspai_hip_mm_destroy(h)
+    return rows.value, cols.value, r, c, v
+
+
+def read_matrix_market(source, ctx: Context | None = None):
+    """Canonical CSR of a Matrix Market file (a path) or text (str/bytes with
+    a newline): parsed on the host (parse_matrix_market), sorted and summed on
+    the GPU (csr_from_entries) -- reference read_matrix_market
+    (include/bspai/cpu.hpp:47).  Returns (rows, cols, ptr, col, val); raises
+    MatrixMarketError or ValueError with the reference's messages."""
+    if isinstance(source, (bytes, bytearray)) or (isinstance(source, str) and "\n" in source):
+        text = source
+    else:
+        with open(source, "rb") as f:
+            text = f.read()
+    rows, cols, r, c, v = parse_matrix_market(text)
+    ptr, col, val = csr_from_entries(rows, cols, r, c, v, ctx)
+    return rows, cols, ptr, col, val
 
 
 class Matrix:
@@ -165,15 +241,21 @@ class Matrix:
             ptr = _host(ptr, np.int32)
             col = _host(col, np.int32)
             val = _host(val, np.float64)
-            n = (len(ptr) if isinstance(ptr, np.ndarray) else ptr.numel()) - 1
-            nnz = len(val) if isinstance(val, np.ndarray) else val.numel()
+            n = _len(ptr) - 1
+            nnz = _len(val)
+            if n < 1:
+                raise ValueError("ptr must hold n + 1 >= 2 row offsets")
+            _need(col, nnz, "col")
+            where = _where(val)
+            if (_where(ptr) != where) or (_where(col) != where):
+                raise ValueError("ptr, col and val must all be host arrays or all device tensors")
             nsys, offs = 1, None
             if systems is not None:
                 offs = np.ascontiguousarray(systems, dtype=np.int64)
                 nsys = len(offs) - 1
             h = C.c_void_p()
             check(_lib.load().bspai_hip_matrix_create(
-                self.ctx.h, n, nnz, _addr(ptr), _addr(col), _addr(val), _where(val), nsys,
+                self.ctx.h, n, nnz, _addr(ptr), _addr(col), _addr(val), where, nsys,
                 None if offs is None else offs.ctypes.data, C.byref(h)), self.ctx.h)
             self.h = h
         n = C.c_int64()
@@ -199,7 +281,7 @@ class Matrix:
         return Matrix(None, self.ctx, _handle=h)
 
     def set_values(self, val):
-        v = _host(val, np.float64)
+        v = _need(_host(val, np.float64), self.nnz, "values")
         check(_lib.load().bspai_hip_matrix_set_values(self.ctx.h, self.h, _addr(v), _where(v)),
               self.ctx.h)
 
@@ -214,7 +296,7 @@ class Matrix:
 
     def multiply(self, x, lanes: int = 0):
         """y = A x in FP64 (reference CsrMatrix::multiply, src/cpu.cpp:80-94)."""
-        xs = _host(x, np.float64)
+        xs = _need(_host(x, np.float64), self.n, "x")
         if _where(xs) == DEVICE:
             import torch
 
@@ -227,7 +309,7 @@ class Matrix:
         return y
 
     def multiply_f32(self, x, lanes: int = 0):
-        xs = _host(x, np.float32)
+        xs = _need(_host(x, np.float32), self.n, "x")
         if _where(xs) == DEVICE:
             import torch
 
@@ -241,8 +323,10 @@ class Matrix:
 
     def sddmm(self, u, v, alpha: float = 1.0):
         """out[q] = alpha * u[row(q)] * v[col(q)] on the stored pattern."""
-        us = _host(u, np.float64)
-        vs = _host(v, np.float64)
+        us = _need(_host(u, np.float64), self.n, "u")
+        vs = _need(_host(v, np.float64), self.n, "v")
+        if _where(us) != _where(vs):
+            raise ValueError("u and v must both be host arrays or both device tensors")
         if _where(us) == DEVICE:
             import torch
 
@@ -367,9 +451,12 @@ def left_preconditioned_gmres(a: Matrix, p: Matrix, rhs, tolerance: float = 1e-8
 
     Returns one GmresResult (single system) or a list (batched matrix)."""
     ctx = a.ctx
-    b = _host(rhs, np.float64)
-    x0 = None if initial_guess is None else _host(initial_guess, np.float64)
+    b = _need(_host(rhs, np.float64), a.n, "rhs")
+    x0 = None if initial_guess is None else _need(_host(initial_guess, np.float64), a.n,
+                                                  "initial_guess")
     where = _where(b)
+    if x0 is not None and _where(x0) != where:
+        raise ValueError("rhs and initial_guess must both be host arrays or both device tensors")
     if where == DEVICE:
         import torch
 
@@ -440,7 +527,7 @@ def spai_gmres_ir(a: Matrix, rhs, **kw):
     max_iterations, max_outer_steps, precision ("fp64" | "mixed"),
     relres_target.  Returns an IrResult (or a list for a batched matrix)."""
     ctx = a.ctx
-    b = _host(rhs, np.float64)
+    b = _need(_host(rhs, np.float64), a.n, "rhs")
     x = _x_like(b, a.n)
     infos = (IrInfo * a.nsys)()
     opt = ir_options(**kw)
@@ -455,7 +542,7 @@ def refine_with_preconditioner(a: Matrix, p, rhs, **kw):
     src/ir.cpp:138-204).  p: a Matrix, or a SpaiResult (its P is used)."""
     ctx = a.ctx
     pm = p.matrix() if isinstance(p, SpaiResult) else p
-    b = _host(rhs, np.float64)
+    b = _need(_host(rhs, np.float64), a.n, "rhs")
     x = _x_like(b, a.n)
     infos = (IrInfo * a.nsys)()
     opt = ir_options(**kw)
@@ -468,7 +555,7 @@ def refine_with_preconditioner(a: Matrix, p, rhs, **kw):
 def adjoint_solve(at: Matrix, pt: Matrix, g, **kw):
     """Solve A^T lambda = g, right-preconditioned by P^T, under IR."""
     ctx = at.ctx
-    gs = _host(g, np.float64)
+    gs = _need(_host(g, np.float64), at.n, "g")
     lam = _x_like(gs, at.n)
     infos = (IrInfo * at.nsys)()
     opt = ir_options(**kw)

@@ -47,6 +47,9 @@ class SparseSolve(torch.autograd.Function):
         x = res.solution
         ctx.handle = handle
         ctx.opts = opts
+        # the solve runs in FP64 (the library converts other dtypes); the
+        # gradients are returned in the inputs' own dtypes
+        ctx.dtypes = (values.dtype, b.dtype)
         ctx.spai = (spai, sk)
         ctx.save_for_backward(x)
         return x
@@ -74,7 +77,8 @@ class SparseSolve(torch.autograd.Function):
         lam = adj.solution
         h.last_adjoint = adj
         grad_vals = m.sddmm(lam, x, alpha=-1.0)
-        return grad_vals, lam, None, None
+        vdt, bdt = ctx.dtypes
+        return grad_vals.to(vdt), lam.to(bdt), None, None
 
 
 class SolverHandle:

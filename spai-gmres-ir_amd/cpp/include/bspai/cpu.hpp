@@ -1,7 +1,9 @@
-// Host CSR container and Matrix Market reader of the HIP backend's C++ API.
-// Same names, members and semantics as the reference's include/bspai/cpu.hpp
-// (Entry, CsrMatrix::{from_entries, multiply, transpose}, read_matrix_market),
-// implemented independently.
+// Host CSR container and Matrix Market reader of the HIP back end's C++ API.
+// It declares exactly what the reference's include/bspai/cpu.hpp declares
+// (Entry, CsrMatrix::{from_entries, rows, cols, row_offsets, column_indices,
+// values, multiply, transpose}, read_matrix_market), so code written against
+// either header compiles against the other; the implementation
+// (cpp/src/cpu.cpp) is this project's own.
 #ifndef BSPAI_HIP_CPP_CPU_HPP
 #define BSPAI_HIP_CPP_CPU_HPP
 
@@ -17,17 +19,11 @@ struct Entry {
     double value;
 };
 
-// Canonical CSR: columns sorted within a row, duplicates summed in input
+// Canonical CSR: columns ascending within a row, duplicates summed in input
 // order, exact zeros dropped, nonfinite values rejected.
 class CsrMatrix {
   public:
     static CsrMatrix from_entries(std::size_t rows, std::size_t cols, std::vector<Entry> entries);
-    // Extension (no reference counterpart): wraps arrays that are already
-    // canonical, e.g. the output of hip_csr_from_entries; only the sizes are
-    // checked.
-    static CsrMatrix from_canonical(std::size_t rows, std::size_t cols,
-                                    std::vector<std::size_t> offsets,
-                                    std::vector<std::size_t> columns, std::vector<double> values);
     std::size_t rows() const { return rows_; }
     std::size_t cols() const { return cols_; }
     const std::vector<std::size_t> &row_offsets() const { return offsets_; }
@@ -43,11 +39,9 @@ class CsrMatrix {
     std::vector<double> values_;
 };
 
-// Matrix Market coordinate real/integer, general/symmetric.
+// Matrix Market coordinate real/integer, general/symmetric; malformed input
+// throws std::runtime_error with the reference reader's messages.
 CsrMatrix read_matrix_market(std::istream &input);
-// The reader's parse and checks alone: the dimensions and the entries it
-// hands to CsrMatrix::from_entries (symmetric files expanded).
-std::vector<Entry> parse_matrix_market(std::istream &input, std::size_t &rows, std::size_t &cols);
 
 } // namespace bspai
 #endif

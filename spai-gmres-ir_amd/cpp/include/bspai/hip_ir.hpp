@@ -32,15 +32,6 @@ struct HipSpaiResult {
     HipTelemetry telemetry;
 };
 
-// CsrMatrix::from_entries with the sort and the duplicate sums on the GPU
-// (bspai_hip_csr_from_entries): the same matrix, and the same
-// std::invalid_argument messages for bad entries.
-CsrMatrix hip_csr_from_entries(std::size_t rows, std::size_t cols,
-                               const std::vector<Entry> &entries, int device = 0);
-// read_matrix_market with the canonicalization on the GPU (the text is parsed
-// and checked on the host, as the reference's reader does).
-CsrMatrix hip_read_matrix_market(std::istream &input, int device = 0);
-
 // GPU SPAI construction (reference construct_spai / cuda_construct_spai).
 HipSpaiResult hip_construct_spai(const CsrMatrix &a, const SpaiOptions &options,
                                  HipPrecision precision = HipPrecision::fp64, int device = 0);
@@ -50,16 +41,20 @@ GmresResult hip_left_preconditioned_gmres(const CsrMatrix &a, const CsrMatrix &p
                                           const GmresOptions &options,
                                           const std::vector<double> &initial_guess = {},
                                           int device = 0);
-// GPU-resident refinement with a given preconditioner.
+// GPU-resident refinement with a given preconditioner.  The outer loop stops
+// on the reference's test (normwise backward error <= 10u, src/ir.cpp:182);
+// relres_target > 0 also requires ||b - Ax||_2 <= relres_target ||b||_2.
 HipIrResult hip_refine_with_preconditioner(const CsrMatrix &a, const CsrMatrix &p,
                                            const std::vector<double> &rhs,
                                            const IrOptions &options,
                                            HipPrecision precision = HipPrecision::fp64,
-                                           int device = 0);
-// GPU SPAI + GPU-resident IR (reference cuda_gpu_spai_gmres_ir).
+                                           int device = 0, double relres_target = 0.0);
+// GPU SPAI + GPU-resident IR (reference cuda_gpu_spai_gmres_ir); the stop as
+// above.
 HipIrResult hip_gpu_spai_gmres_ir(const CsrMatrix &a, const std::vector<double> &rhs,
                                   const IrOptions &options,
-                                  HipPrecision precision = HipPrecision::fp64, int device = 0);
+                                  HipPrecision precision = HipPrecision::fp64, int device = 0,
+                                  double relres_target = 0.0);
 
 } // namespace bspai
 #endif

@@ -69,7 +69,7 @@ int prec(HipPrecision p) {
     return p == HipPrecision::mixed ? BSPAI_HIP_MIXED : BSPAI_HIP_FP64;
 }
 
-bspai_hip_ir_options ir_opts(const IrOptions &o, HipPrecision p) {
+bspai_hip_ir_options ir_opts(const IrOptions &o, HipPrecision p, double relres_target) {
     bspai_hip_ir_options r{};
     r.spai_epsilon = o.spai.epsilon;
     r.spai_alpha = (std::int64_t)o.spai.alpha;
@@ -78,8 +78,9 @@ bspai_hip_ir_options ir_opts(const IrOptions &o, HipPrecision p) {
     r.gmres_max_iterations = (std::int64_t)o.gmres.max_iterations;
     r.max_outer_steps = (std::int64_t)o.max_outer_steps;
     r.precision = prec(p);
-    // Mixed precision also requires ||b-Ax||_2 <= 1e-12 ||b||_2.
-    r.relres_target = p == HipPrecision::mixed ? 1e-12 : 0.0;
+    // The reference's stop (normwise backward error <= 10u, src/ir.cpp:182)
+    // alone, unless the caller also asks for ||b-Ax||_2 <= target ||b||_2.
+    r.relres_target = relres_target;
     return r;
 }
 
@@ -164,40 +165,6 @@ void fill_ir(const bspai_hip_ir_info &i, std::vector<double> x, IrResult &r) {
 
 } // namespace
 
-CsrMatrix hip_csr_from_entries(std::size_t rows, std::size_t cols,
-                               const std::vector<Entry> &entries, int device) {
-    const Ctx c(device);
-    const std::size_t count = entries.size();
-    std::vector<std::int64_t> r(count), cl(count);
-    std::vector<double> v(count);
-    for (std::size_t i = 0; i < count; ++i) {
-        r[i] = (std::int64_t)entries[i].row; // beyond INT64_MAX: negative, out of range
-        cl[i] = (std::int64_t)entries[i].col;
-        v[i] = entries[i].value;
-    }
-    std::vector<std::int32_t> ptr(rows + 1), col(std::max<std::size_t>(count, 1));
-    std::vector<double> val(std::max<std::size_t>(count, 1));
-    std::int64_t nnz = 0;
-    const int rc = bspai_hip_csr_from_entries(
-        c.h, (std::int64_t)rows, (std::int64_t)cols, (std::int64_t)count, r.data(), cl.data(),
-        v.data(), BSPAI_HIP_HOST, ptr.data(), col.data(), val.data(), BSPAI_HIP_HOST, &nnz);
-    if (rc == BSPAI_HIP_EINVAL)
-        throw std::invalid_argument(bspai_hip_last_error(c.h));
-    c.check(rc);
-    col.resize((std::size_t)nnz);
-    val.resize((std::size_t)nnz);
-    return CsrMatrix::from_canonical(rows, cols,
-                                     std::vector<std::size_t>(ptr.begin(), ptr.end()),
-                                     std::vector<std::size_t>(col.begin(), col.end()),
-                                     std::move(val));
-}
-
-CsrMatrix hip_read_matrix_market(std::istream &input, int device) {
-    std::size_t rows = 0, cols = 0;
-    const std::vector<Entry> entries = parse_matrix_market(input, rows, cols);
-    return hip_csr_from_entries(rows, cols, entries, device);
-}
-
 HipSpaiResult hip_construct_spai(const CsrMatrix &a, const SpaiOptions &o, HipPrecision p,
                                  int device) {
     HipSpaiResult out(a.rows());
@@ -256,7 +223,8 @@ GmresResult hip_left_preconditioned_gmres(const CsrMatrix &a, const CsrMatrix &p
 
 HipIrResult hip_refine_with_preconditioner(const CsrMatrix &a, const CsrMatrix &p,
                                            const std::vector<double> &rhs, const IrOptions &o,
-                                           HipPrecision precision, int device) {
+                                           HipPrecision precision, int device,
+                                           double relres_target) {
     HipIrResult out(a.rows());
     if (!valid(a, rhs, o) || p.rows() != a.rows() || p.cols() != a.cols()) {
         out.solver.message = "Invalid dimensions, finite inputs, inner options, or outer cap";
@@ -264,7 +232,7 @@ HipIrResult hip_refine_with_preconditioner(const CsrMatrix &a, const CsrMatrix &
     }
     Ctx c(device);
     Mat ma(c, a), mp(c, p);
-    const bspai_hip_ir_options io = ir_opts(o, precision);
+    const bspai_hip_ir_options io = ir_opts(o, precision, relres_target);
     bspai_hip_ir_info info{};
     std::vector<double> x(a.rows());
     c.check(bspai_hip_ir_solve(c.h, ma.h, mp.h, rhs.data(), &io, x.data(), BSPAI_HIP_HOST,
@@ -275,7 +243,8 @@ HipIrResult hip_refine_with_preconditioner(const CsrMatrix &a, const CsrMatrix &
 }
 
 HipIrResult hip_gpu_spai_gmres_ir(const CsrMatrix &a, const std::vector<double> &rhs,
-                                  const IrOptions &o, HipPrecision precision, int device) {
+                                  const IrOptions &o, HipPrecision precision, int device,
+                                  double relres_target) {
     HipIrResult out(a.rows());
     if (!valid(a, rhs, o)) {
         out.solver.message = "Invalid dimensions, finite inputs, inner options, or outer cap";
@@ -302,7 +271,7 @@ HipIrResult hip_gpu_spai_gmres_ir(const CsrMatrix &a, const std::vector<double> 
     bspai_hip_matrix *pm = nullptr;
     c.check(bspai_hip_spai_matrix(c.h, b.h, 0, &pm));
     Mat mp(c, pm);
-    const bspai_hip_ir_options io = ir_opts(o, precision);
+    const bspai_hip_ir_options io = ir_opts(o, precision, relres_target);
     bspai_hip_ir_info info{};
     std::vector<double> x(a.rows());
     c.check(bspai_hip_ir_solve(c.h, ma.h, mp.h, rhs.data(), &io, x.data(), BSPAI_HIP_HOST,

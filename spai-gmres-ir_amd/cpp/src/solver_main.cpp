@@ -5,6 +5,7 @@
 // solver CLI (src/solver_main.cpp:60-144): 0 converged, 1 solver failure or
 // runtime error, 2 invalid argument / out of range.  The right side is
 // b_i = 1/sqrt(n); alpha = ceil(n/8); beta = 8.
+#include "bspai/hip_io.hpp"
 #include "bspai/hip_ir.hpp"
 
 #include <cmath>

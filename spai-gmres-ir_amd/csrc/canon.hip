@@ -185,4 +185,118 @@ std::int64_t csr_from_entries(Context &c, std::int64_t rows, std::int64_t cols,
     return nnz;
 }
 
+// ------------------------------------------------------------ CSR checks
+// A matrix handed over in device memory is checked on the device before use
+// (the SPAI and SpMV kernels index by it): row offsets first, then -- only
+// when they are consistent, so that no row walk leaves the arrays -- every
+// row's columns and values.
+
+namespace {
+
+enum : int {
+    kBadPtr = 1,      // offsets not 0 .. nnz, or decreasing
+    kBadRange = 2,    // column outside [0, n)
+    kBadOrder = 4,    // columns not strictly increasing within a row
+    kBadValue = 8,    // nonfinite value
+    kBadSystem = 16,  // column outside the row's system (batched layouts)
+};
+
+__global__ void __launch_bounds__(256)
+csr_ptr_check_kernel(int n, int nnz, const int *__restrict__ ptr, int *__restrict__ bad) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i > n)
+        return;
+    const int p = ptr[i];
+    bool ok = p >= 0 && p <= nnz;
+    if (i == 0)
+        ok = ok && p == 0;
+    if (i == n)
+        ok = ok && p == nnz;
+    if (i < n)
+        ok = ok && p <= ptr[i + 1];
+    if (!ok)
+        atomicOr(bad, kBadPtr);
+}
+
+// One thread per row; sys (nsys + 1 offsets) bounds each row's columns.
+__global__ void __launch_bounds__(256)
+csr_rows_check_kernel(int n, const int *__restrict__ ptr, const int *__restrict__ col,
+                      const double *__restrict__ val, int nsys, const int *__restrict__ sys,
+                      int *__restrict__ bad) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n)
+        return;
+    int lo = 0, hi = n;
+    if (nsys > 1) { // the system s with sys[s] <= r < sys[s + 1]
+        int a = 0, b = nsys;
+        while (b - a > 1) {
+            const int m = (a + b) / 2;
+            if (sys[m] <= r)
+                a = m;
+            else
+                b = m;
+        }
+        lo = sys[a];
+        hi = sys[a + 1];
+    }
+    int flags = 0, prev = -1;
+    for (int q = ptr[r]; q < ptr[r + 1]; ++q) {
+        const int c = col[q];
+        if (c < 0 || c >= n)
+            flags |= kBadRange;
+        else if (c < lo || c >= hi)
+            flags |= kBadSystem;
+        if (c <= prev)
+            flags |= kBadOrder;
+        if (!isfinite(val[q]))
+            flags |= kBadValue;
+        prev = c;
+    }
+    if (flags)
+        atomicOr(bad, flags);
+}
+
+const char *csr_problem(int flags) {
+    if (flags & kBadPtr)
+        return "row pointers must span [0, nnz] and be nondecreasing";
+    if (flags & kBadRange)
+        return "column index out of range";
+    if (flags & kBadOrder)
+        return "column indices must be strictly increasing within each row";
+    if (flags & kBadSystem)
+        return "entry outside its row's system (a batch must be block-diagonal)";
+    if (flags & kBadValue)
+        return "matrix values must be finite";
+    return nullptr;
+}
+
+} // namespace
+
+void check_device_csr(Context &c, std::int64_t n, std::int64_t nnz, const int *ptr,
+                      const int *col, const double *val,
+                      const std::vector<std::int64_t> &sys_offsets) {
+    hipStream_t s = c.stream;
+    const int nsys = (int)sys_offsets.size() - 1;
+    DeviceArray<int> flag(1), sys(nsys + 1);
+    std::vector<int> hs(sys_offsets.begin(), sys_offsets.end());
+    HIP_CHECK(hipMemsetAsync(flag.data(), 0, sizeof(int), s));
+    HIP_CHECK(hipMemcpyAsync(sys.data(), hs.data(), hs.size() * sizeof(int),
+                             hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(csr_ptr_check_kernel, dim3(ceil_div(n + 1, 256)), dim3(256), 0, s, (int)n,
+                       (int)nnz, ptr, flag.data());
+    HIP_LAUNCH_CHECK();
+    int h = 0;
+    HIP_CHECK(hipMemcpyAsync(&h, flag.data(), sizeof h, hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    if (!h) {
+        hipLaunchKernelGGL(csr_rows_check_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, s, (int)n,
+                           ptr, col, val, nsys, sys.data(), flag.data());
+        HIP_LAUNCH_CHECK();
+        HIP_CHECK(hipMemcpyAsync(&h, flag.data(), sizeof h, hipMemcpyDeviceToHost, s));
+        HIP_CHECK(hipStreamSynchronize(s));
+    }
+    if (const char *m = csr_problem(h))
+        throw std::invalid_argument(m);
+}
+
 } // namespace bspai_hip

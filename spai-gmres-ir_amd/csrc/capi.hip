@@ -23,10 +23,27 @@ struct bspai_hip_spai {
 
 namespace {
 
+// Every entry point runs on the context's device and gives the caller's
+// current device back on return, as the reference's DeviceGuard does
+// (src/cuda_gmres.cu:78-87).
+struct DeviceRestore {
+    int prev = -1;
+    ~DeviceRestore() {
+        if (prev >= 0)
+            (void)hipSetDevice(prev);
+    }
+};
+
 template <typename F> int guard(bspai_hip_ctx *ctx, F &&f) {
+    DeviceRestore restore;
     try {
-        if (ctx)
-            HIP_CHECK(hipSetDevice(ctx->c.device));
+        if (ctx) {
+            int cur = -1;
+            HIP_CHECK(hipGetDevice(&cur));
+            restore.prev = cur;
+            if (cur != ctx->c.device)
+                HIP_CHECK(hipSetDevice(ctx->c.device));
+        }
         f();
         return BSPAI_HIP_OK;
     } catch (const std::invalid_argument &e) {
@@ -344,17 +361,6 @@ int bspai_hip_matrix_create(bspai_hip_ctx *ctx, int64_t n, int64_t nnz, const in
         a.val.resize(std::max<int64_t>(nnz, 1));
         const hipMemcpyKind kind =
             where == BSPAI_HIP_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
-        if (where == BSPAI_HIP_HOST) {
-            require(ptr[0] == 0 && ptr[n] == nnz, "row pointers must span [0, nnz]");
-            for (int64_t r = 0; r < n; ++r)
-                require(ptr[r] <= ptr[r + 1], "row pointers must be nondecreasing");
-            for (int64_t q = 0; q < nnz; ++q)
-                require(col[q] >= 0 && col[q] < n && std::isfinite(val[q]),
-                        "column index out of range or nonfinite value");
-        }
-        HIP_CHECK(hipMemcpyAsync(a.ptr.data(), ptr, (n + 1) * sizeof(int), kind, c.stream));
-        HIP_CHECK(hipMemcpyAsync(a.col.data(), col, nnz * sizeof(int), kind, c.stream));
-        HIP_CHECK(hipMemcpyAsync(a.val.data(), val, nnz * sizeof(double), kind, c.stream));
         std::vector<std::int64_t> offs;
         if (nsys == 1)
             offs = {0, n};
@@ -364,6 +370,34 @@ int bspai_hip_matrix_create(bspai_hip_ctx *ctx, int64_t n, int64_t nnz, const in
             for (int s = 0; s < nsys; ++s)
                 require(offs[s] < offs[s + 1], "systems must be nonempty");
         }
+        if (where == BSPAI_HIP_HOST) {
+            // The canonical form every kernel relies on (the reference's
+            // CsrMatrix is canonical by construction): offsets 0..nnz
+            // nondecreasing, columns strictly increasing within a row and
+            // inside the row's system, finite values.
+            require(ptr[0] == 0 && ptr[n] == nnz,
+                    "row pointers must span [0, nnz] and be nondecreasing");
+            int sys = 0;
+            for (int64_t r = 0; r < n; ++r) {
+                require(ptr[r] <= ptr[r + 1],
+                        "row pointers must span [0, nnz] and be nondecreasing");
+                while (r >= offs[sys + 1])
+                    ++sys;
+                for (int64_t q = ptr[r]; q < ptr[r + 1]; ++q) {
+                    require(col[q] >= 0 && col[q] < n, "column index out of range");
+                    require(q == ptr[r] || col[q] > col[q - 1],
+                            "column indices must be strictly increasing within each row");
+                    require(col[q] >= offs[sys] && col[q] < offs[sys + 1],
+                            "entry outside its row's system (a batch must be block-diagonal)");
+                    require(std::isfinite(val[q]), "matrix values must be finite");
+                }
+            }
+        }
+        HIP_CHECK(hipMemcpyAsync(a.ptr.data(), ptr, (n + 1) * sizeof(int), kind, c.stream));
+        HIP_CHECK(hipMemcpyAsync(a.col.data(), col, nnz * sizeof(int), kind, c.stream));
+        HIP_CHECK(hipMemcpyAsync(a.val.data(), val, nnz * sizeof(double), kind, c.stream));
+        if (where == BSPAI_HIP_DEVICE)
+            check_device_csr(c, n, nnz, a.ptr.data(), a.col.data(), a.val.data(), offs);
         a.layout = std::make_shared<Layout>();
         a.layout->build(c, n, offs);
         a.lanes = choose_lanes(n, nnz);
@@ -637,11 +671,16 @@ int bspai_hip_gmres(bspai_hip_ctx *ctx, const bspai_hip_matrix *a, const bspai_h
     return guard(ctx, [&] {
         require(a && p && rhs && opt && x && info, "null argument");
         require(a->a.n == p->a.n, "Invalid dimensions, initial guess, tolerance, or iteration cap");
-        require(std::isfinite(opt->tolerance) && opt->tolerance > 0 &&
-                    opt->max_iterations >= 0 && opt->max_iterations <= a->a.n,
-                "Invalid dimensions, initial guess, tolerance, or iteration cap");
         Context &c = ctx->c;
         const Layout &L = *a->a.layout;
+        // The cap applies to every system: it may not exceed the smallest
+        // one (src/gmres.cpp:115 for a single system).
+        std::int64_t minrows = a->a.n;
+        for (int s = 0; s < L.nsys; ++s)
+            minrows = std::min(minrows, L.offsets[s + 1] - L.offsets[s]);
+        require(std::isfinite(opt->tolerance) && opt->tolerance > 0 &&
+                    opt->max_iterations >= 0 && opt->max_iterations <= minrows,
+                "Invalid dimensions, initial guess, tolerance, or iteration cap");
         const int cap = opt->max_iterations ? (int)opt->max_iterations : (int)a->a.n;
         std::int64_t maxrows = 0;
         for (int s = 0; s < L.nsys; ++s)

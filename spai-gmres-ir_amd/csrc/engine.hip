@@ -972,7 +972,7 @@ template <typename T> std::vector<GmresOutcome> GmresEngine<T>::outcomes() {
         out[i] = GmresOutcome{h.status,     h.converged,  h.iterations, h.init_orig,
                               h.init_rel,   h.final_orig, h.final_rel};
         if (h.active) { // cap reached without an accept (loop ended)
-            out[i].status = h.iterations < h.nrows ? kGmIterationLimit : kGmNonconvergence;
+            out[i].status = h.scap < h.nrows ? kGmIterationLimit : kGmNonconvergence;
         }
     }
     return out;

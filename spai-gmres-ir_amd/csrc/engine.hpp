@@ -330,6 +330,14 @@ void spai_score_on_device(hipStream_t s, double rr, double dot, double den, int 
 // Returns nnz.  Throws std::invalid_argument with the reference's messages
 // for an index out of range, a non-finite value or a non-finite duplicate
 // sum; the outputs are then unspecified.
+// Checks a device CSR before use: offsets 0..nnz nondecreasing, then every
+// row's columns in range, strictly increasing and inside the row's system,
+// and finite values.  Throws std::invalid_argument naming the first problem
+// class found.  ptr/col/val are device arrays; sys_offsets: nsys + 1 offsets.
+void check_device_csr(Context &c, std::int64_t n, std::int64_t nnz, const int *ptr,
+                      const int *col, const double *val,
+                      const std::vector<std::int64_t> &sys_offsets);
+
 std::int64_t csr_from_entries(Context &c, std::int64_t rows, std::int64_t cols,
                               std::int64_t count, const std::int64_t *row,
                               const std::int64_t *col, const double *val, int *ptr, int *out_col,

@@ -150,7 +150,7 @@ __device__ void givens_step(SysState &st, double *H, double *cs, double *sn, dou
         bad = bad || !isfinite(cs[k]) || !isfinite(sn[k]) || !isfinite(g[k]) ||
               !isfinite(g[k + 1]) || !isfinite(st.est);
         st.zero_pivot = 0;
-        st.triggered = breakdown || st.est <= st.tol || k + 1 == cap;
+        st.triggered = breakdown || st.est <= st.tol || k + 1 == st.scap;
         count = k + 1;
     } else {
         // Zero pivot: re-verify the previous candidate (or x0), never a new y.
@@ -1018,8 +1018,8 @@ decide_kernel(Tiles tiles, KrylovArgs<T> a, int phase, int right) {
         } else if (st.breakdown) {
             st.status = kGmBreakdown;
             st.active = 0;
-        } else if (st.iterations == a.cap) {
-            st.status = a.cap < st.nrows ? kGmIterationLimit : kGmNonconvergence;
+        } else if (st.iterations == st.scap) {
+            st.status = st.scap < st.nrows ? kGmIterationLimit : kGmNonconvergence;
             st.active = 0;
         }
         if (st.active) // not accepted: the next step (a held one resumes) runs
@@ -1062,6 +1062,10 @@ __global__ void gmres_reset_kernel(int nsys, KrylovArgs<T> k, const IrSysState *
     SysState st = {};
     st.tol = tol;
     st.nrows = nrows[s];
+    // Each system stops at its own cap: the workspace's cap (the caller's, or
+    // the largest system) bounded by its own dimension, as a single solve of
+    // that system would be (src/gmres.cpp:128-130).
+    st.scap = k.cap < st.nrows ? k.cap : st.nrows;
     st.participating = ir ? ir[s].active : 1;
     st.active = st.participating;
     st.run[0] = st.run[1] = 1;

@@ -69,6 +69,7 @@ struct SysState {
     double den, beta, hnorm, est;
     double init_orig, init_rel, final_orig, final_rel;
     int nrows;
+    int scap; // this system's Krylov cap: min(workspace cap, nrows)
     int participating, active, status, converged, iterations;
     int triggered, ccount, zero_pivot, breakdown;
     int rot_bad; // a previous rotation made column k non-finite

@@ -17,6 +17,13 @@ oracle/_ref via `make -C oracle ref`).  Fixtures written:
                               (tests/fixtures/tiny_nonsymmetric.mtx of the reference)
   cli_c1_head.txt             first 6 lines of `cpu_fp64_ir c1.mtx` stdout + x hash
   cli_dup_overflow.txt        `cpu_fp64_ir dup.mtx` exit code and stderr, duplicate sum > DBL_MAX
+  mm_cases.json               Matrix Market texts (the reference's accept and reject
+                              lists, tests/test_cpu.cpp:36-82, plus edge cases of its
+                              rules) with the reference read_matrix_market's outcome
+                              (canonical CSR, or exception kind and message) and, for
+                              the rejected ones, the reference CLI's exit code and stderr
+
+`python make_goldens.py mm` writes mm_cases.json only.
 """
 from __future__ import annotations
 
@@ -98,7 +105,114 @@ def cli_dup_overflow(tmp):
         f.write(f"exit={r.returncode}\n" + r.stdout + r.stderr)
 
 
+_MM_HEAD = "%%MatrixMarket matrix coordinate real general\n"
+MM_CASES = [
+    # accepted (reference tests/test_cpu.cpp:36-60)
+    ("tiny nonsymmetric", "%%MatrixMarket matrix coordinate real general\n% tiny nonsymmetric "
+     "fixture\n3 3 7\n3 3 3\n1 2 1\n2 1 2\n1 1 4\n2 2 5\n3 2 1\n2 3 1\n"),
+    ("integer duplicates", "%%MatrixMarket matrix coordinate integer general\n3 4 5\n"
+     "1 2 3\n3 4 5\n1 2 -3\n3 4 2\n2 1 0\n"),
+    ("symmetric expansion", "%%MatrixMarket matrix coordinate real symmetric\n3 3 4\n"
+     "1 1 2\n2 1 3\n2 2 4\n3 2 5\n"),
+    ("zero nnz", "%%MatrixMarket matrix coordinate real general\n2 3 0\n"),
+    # rejected (reference tests/test_cpu.cpp:67-82)
+    ("invalid banner", "not a matrix market file\n"),
+    ("array storage", "%%MatrixMarket matrix array real general\n2 2\n1\n"),
+    ("complex field", "%%MatrixMarket matrix coordinate complex general\n1 1 1\n1 1 2 3\n"),
+    ("pattern field", "%%MatrixMarket matrix coordinate pattern general\n1 1 1\n1 1\n"),
+    ("skew symmetry", "%%MatrixMarket matrix coordinate real skew-symmetric\n2 2 1\n2 1 1\n"),
+    ("nonsquare symmetric", "%%MatrixMarket matrix coordinate real symmetric\n2 3 0\n"),
+    ("zero index", _MM_HEAD + "2 2 1\n0 1 1\n"),
+    ("row out of range", _MM_HEAD + "2 2 1\n3 1 1\n"),
+    ("column out of range", _MM_HEAD + "2 2 1\n1 3 1\n"),
+    ("truncated", _MM_HEAD + "2 2 2\n1 1 1\n"),
+    ("trailing", _MM_HEAD + "2 2 1\n1 1 1\n2 2 2\n"),
+    ("nan value", _MM_HEAD + "2 2 1\n1 1 NaN\n"),
+    ("overflowing value", _MM_HEAD + "2 2 1\n1 1 1e999\n"),
+    ("negative count", _MM_HEAD + "2 2 -1\n"),
+    # edge cases of the same rules
+    ("empty file", ""),
+    ("banner only", _MM_HEAD),
+    ("short banner", "%%MatrixMarket matrix coordinate real\n1 1 1\n1 1 1\n"),
+    ("extra banner field", "%%MatrixMarket matrix coordinate real general x\n1 1 1\n1 1 1\n"),
+    ("lowercase magic", "%%matrixmarket matrix coordinate real general\n1 1 1\n1 1 1\n"),
+    ("short dimensions", _MM_HEAD + "2 2\n"),
+    ("extra dimension field", _MM_HEAD + "2 2 1 7\n1 1 1\n"),
+    ("zero rows", _MM_HEAD + "0 2 0\n"),
+    ("signed dimension", _MM_HEAD + "+2 2 0\n"),
+    ("huge dimension", _MM_HEAD + "99999999999999999999 2 0\n"),
+    ("huge index", _MM_HEAD + "2 2 1\n18446744073709551616 1 1\n"),
+    ("short entry", _MM_HEAD + "2 2 1\n1 1\n"),
+    ("extra entry field", _MM_HEAD + "2 2 1\n1 1 1 2\n"),
+    ("word value", _MM_HEAD + "2 2 1\n1 1 abc\n"),
+    ("value with suffix", _MM_HEAD + "2 2 1\n1 1 1.5x\n"),
+    ("infinite value", _MM_HEAD + "2 2 1\n1 1 inf\n"),
+    ("underflowing value", _MM_HEAD + "2 2 1\n1 1 1e-400\n"),
+    ("subnormal value", _MM_HEAD + "2 2 1\n1 1 1e-310\n"),
+    ("hex value", _MM_HEAD + "2 2 1\n1 2 0x1p3\n"),
+    ("integer field exponent", "%%MatrixMarket matrix coordinate integer general\n2 2 1\n"
+     "1 1 1e3\n"),
+    ("integer field signs", "%%MatrixMarket matrix coordinate integer general\n2 2 2\n"
+     "1 1 +7\n2 2 -3\n"),
+    ("comments blanks crlf tabs", "%%MatrixMarket matrix coordinate real general\r\n% c\r\n"
+     "\r\n  \t\n%% more\n 2\t2 3 \r\n%x\n1 2 0.5\r\n\n\t2 1 -0.25\n   % y\n2 2 4"),
+    ("vertical tab line", _MM_HEAD + "\v\n2 2 0\n"),
+    ("comment after count", _MM_HEAD + "2 2 1\n1 1 1\n% done\n\n"),
+    ("explicit zero and cancellation", _MM_HEAD + "2 2 4\n1 1 0\n2 2 1\n2 2 -1\n1 2 -0.0\n"),
+    ("duplicate overflow", _MM_HEAD + "2 2 3\n1 1 1e308\n1 1 1e308\n2 2 1\n"),
+    ("symmetric duplicates", "%%MatrixMarket matrix coordinate real symmetric\n2 2 3\n"
+     "2 1 1\n1 2 2\n2 2 3\n"),
+    ("first error wins", _MM_HEAD + "3 3 3\n1 1 1\n4 1 1\n1 1 x\n"),
+]
+
+
+def run_ref_mm(text, tmp):
+    """The reference read_matrix_market on `text`: ("ok", rows, cols, ptr, col, val-bits)
+    or ("error", kind, message)."""
+    mtx = os.path.join(tmp, "case.mtx")
+    out = os.path.join(tmp, "case.bin")
+    with open(mtx, "wb") as f:
+        f.write(text.encode())
+    subprocess.run([os.path.join(BIN, "ref_mm"), mtx, out], check=True)
+    raw = open(out, "rb").read()
+    kind = struct.unpack("<q", raw[:8])[0]
+    if kind:
+        return {"error": ["runtime_error", "invalid_argument", "exception"][kind - 1],
+                "message": raw[8:].decode()}
+    rows, cols, nnz = struct.unpack("<3q", raw[8:32])
+    body = np.frombuffer(raw[32:], dtype="<i8")
+    ptr = body[: rows + 1]
+    col = body[rows + 1: rows + 1 + nnz]
+    val = body[rows + 1 + nnz:]
+    return {"rows": rows, "cols": cols, "ptr": ptr.tolist(), "col": col.tolist(),
+            "val_bits": [format(int(v) & (2**64 - 1), "016x") for v in val]}
+
+
+def mm_cases(tmp):
+    cases = []
+    for name, text in MM_CASES:
+        c = {"name": name, "text": text, "reference": run_ref_mm(text, tmp)}
+        if "error" in c["reference"]:
+            mtx = os.path.join(tmp, "cli.mtx")
+            with open(mtx, "wb") as f:
+                f.write(text.encode())
+            r = subprocess.run([os.path.join(BIN, "cpu_fp64_ir"), mtx], capture_output=True,
+                               text=True)
+            c["cli"] = {"exit": r.returncode, "stderr": r.stderr}
+        cases.append(c)
+    with open(os.path.join(HERE, "mm_cases.json"), "w") as f:
+        json.dump(cases, f, indent=1)
+        f.write("\n")
+
+
 def main():
+    if sys.argv[1:] == ["mm"]:
+        subprocess.run(["make", "-s", "-C", os.path.join(REPO, "oracle"), "ref",
+                        f"REF={REF}"], check=True)
+        with tempfile.TemporaryDirectory() as tmp:
+            mm_cases(tmp)
+        print("mm_cases.json written to", HERE)
+        return
     subprocess.run(["make", "-s", "-C", os.path.join(REPO, "oracle"), "all", "ref",
                     f"REF={REF}"], check=True)
     snap = subprocess.run([os.path.join(BIN, "cpu_release_snapshot")], check=True,
@@ -143,6 +257,7 @@ def main():
         with open(os.path.join(HERE, "cli_bad_outercap.txt"), "w") as f:
             f.write(f"exit={r.returncode}\n" + r.stdout + r.stderr)
         cli_dup_overflow(tmp)
+        mm_cases(tmp)
     print("goldens written to", HERE)
 
 

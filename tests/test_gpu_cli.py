@@ -108,3 +108,39 @@ def test_cli_failure_exit_1():
     assert r.returncode == 1
     kv, _ = parse(r.stdout)
     assert kv["status"] == "construction_failure" and kv["converged"] == "0"
+
+
+REF_CLI_HIP = os.path.join(REPO, "oracle", "_ref", "ref_cli_hip")
+
+
+@pytest.mark.skipif(not os.path.exists(REF_CLI_HIP),
+                    reason="reference CLI with the HIP branch not built (needs the reference tree)")
+def test_reference_cli_with_hip_backend():
+    """The reference's own solver CLI (src/solver_main.cpp) with the
+    BSPAI_SOLVER_HIP branch a maintainer adds, linked against the reference's
+    CPU sources, this repository's C++ adapter and libbspai_hip.so: the drop-in
+    in use.  Its transcripts match the reference CPU CLI's (tiny fixture: the
+    exact solution; the 64x64 system: the header lines and outer steps)."""
+    from bspai_hip import problems as P
+
+    r = subprocess.run([REF_CLI_HIP, os.path.join(GOLDEN, "tiny_nonsymmetric.mtx"), "0.5"],
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    kv, x = parse(r.stdout)
+    ref_kv, ref_x = parse(open(os.path.join(GOLDEN, "cli_tiny.txt")).read())
+    for key in ("n", "nnz", "epsilon", "tau", "status", "converged", "outer_steps"):
+        assert kv[key] == ref_kv[key], key
+    assert kv["backend"] == "GPU" and kv["precision"] == "FP64"
+    assert np.max(np.abs(x - ref_x)) <= 1e-15
+    head = open(os.path.join(GOLDEN, "cli_c1_head.txt")).read()
+    ref_kv, _ = parse(head)
+    with tempfile.TemporaryDirectory() as d:
+        mtx = os.path.join(d, "c1.mtx")
+        P.write_matrix_market(mtx, P.convection_diffusion_2d())
+        r = subprocess.run([REF_CLI_HIP, mtx], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    kv, x = parse(r.stdout)
+    for key in ("n", "nnz", "epsilon", "tau", "status", "converged", "outer_steps"):
+        assert kv[key] == ref_kv[key], key
+    assert float(kv["backward_error"]) <= 10 * 2.0 ** -53
+    assert len(x) == 4096
