@@ -448,6 +448,9 @@ def cpu_baseline(a, b_host, p64, p32, solutions):
     from bspai_hip import problems as P
 
     n = a.n
+    O.set_progress(lambda phase, done, value: log(
+        f"cpu baseline: SPAI columns {done}/{value}" if phase == 0 else
+        f"cpu baseline: refinement step {done} done ({value} inner iterations)"))
     log(f"cpu baseline: oracle SPAI on all {n} columns, one core")
     t0 = time.perf_counter()
     s = O.construct_spai(a, 0.3, -(-n // 8), 8)
@@ -457,6 +460,7 @@ def cpu_baseline(a, b_host, p64, p32, solutions):
     r = O.spai_gmres_ir(a, b_host, 0.3, tolerance=1e-8, max_iterations=200, max_outer_steps=20,
                         p=P.Csr(n, s.p_ptr, s.p_col, s.p_val))
     t_ir = time.perf_counter() - t0
+    O.set_progress(None)
     log(f"cpu baseline: refinement {t_ir:.1f} s ({r.status}, inner {r.inner_iterations})")
     total = t_spai + t_ir
     x_ref = np.asarray(r.solution)

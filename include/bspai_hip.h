@@ -147,6 +147,11 @@ typedef struct {
     double flops;        /* algorithmic flops, summed over the launches */
 } bspai_hip_kernel_time;
 int bspai_hip_ctx_set_timing(bspai_hip_ctx *ctx, int enable);
+/* Arnoldi-step graphs (hipGraph) the context's GMRES solves have captured and
+ * replayed so far.  Each replay runs several Arnoldi steps between two host
+ * check points; BSPAI_GMRES_GRAPH=0 in the environment, or timing enabled,
+ * launches the same kernels one by one on the stream instead. */
+int bspai_hip_ctx_graph_stats(const bspai_hip_ctx *ctx, int64_t *captured, int64_t *replayed);
 int bspai_hip_ctx_kernel_times(bspai_hip_ctx *ctx, bspai_hip_kernel_time *out, int capacity,
                                int *count);
 

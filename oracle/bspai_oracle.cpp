@@ -37,6 +37,12 @@
 #include <string>
 #include <vector>
 
+// Optional progress report of the long-running calls (the benchmark's CPU
+// baseline prints it): phase 0 = SPAI columns done, 1 = refinement steps done
+// (value = that step's inner iterations).  Set by oracle_set_progress.
+typedef void (*OracleProgress)(int phase, std::int64_t done, std::int64_t value);
+static OracleProgress g_progress = nullptr;
+
 namespace {
 
 using idx = std::int64_t;
@@ -439,6 +445,8 @@ Spai construct_spai(const Csr &a, double eps, idx alpha, idx beta, bool trace) {
     r.status = SPAI_SUCCESS;
     r.cols.resize(a.n);
     for (idx k = 0; k < a.n; ++k) {
+        if (g_progress && k > 0 && k % (1 << 19) == 0)
+            g_progress(0, k, a.n);
         r.cols[k] = construct_column(a, bt, r.scaling, k, eps, alpha, beta, trace);
         const Column &c = r.cols[k];
         if (c.status != SPAI_SUCCESS && r.status == SPAI_SUCCESS) {
@@ -815,6 +823,8 @@ void refine(Ir &out, const Csr &a, const Csr &p, const std::vector<double> &b, d
             const bool unchanged = next == out.solution;
             out.solution.swap(next);
             ++out.outer_steps;
+            if (g_progress)
+                g_progress(1, out.outer_steps, (std::int64_t)in.iterations);
             if (done(out.history.back())) {
                 out.status = IR_SUCCESS;
                 out.converged = true;
@@ -840,6 +850,9 @@ thread_local std::string g_error;
 
 // ======================================================================= C API
 extern "C" {
+void oracle_set_progress(OracleProgress fn) {
+    g_progress = fn;
+}
 
 struct OracleSpai {
     Spai s;

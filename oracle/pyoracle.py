@@ -92,8 +92,22 @@ def lib():
         L.oracle_csr_get.argtypes = [P, i64p, i64p, f64p]
         L.oracle_csr_free.argtypes = [P]
         L.oracle_last_error.restype = C.c_char_p
+        L.oracle_set_progress.argtypes = [C.c_void_p]
         _lib = L
     return _lib
+
+
+_PROGRESS = C.CFUNCTYPE(None, C.c_int, C.c_int64, C.c_int64)
+_progress_keep = None
+
+
+def set_progress(fn) -> None:
+    """fn(phase, done, value) after every 2^19 SPAI columns (phase 0, value =
+    n) and every refinement step (phase 1, value = its inner iterations);
+    None turns it off."""
+    global _progress_keep
+    _progress_keep = None if fn is None else _PROGRESS(fn)
+    lib().oracle_set_progress(C.cast(_progress_keep, C.c_void_p) if fn else None)
 
 
 def _p(a, ctype):

@@ -84,6 +84,7 @@ SIGNATURES = {
     "bspai_hip_last_error": (C.c_char_p, [_VP]),
     "bspai_hip_synchronize": (C.c_int, [_VP]),
     "bspai_hip_ctx_set_timing": (C.c_int, [_VP, C.c_int]),
+    "bspai_hip_ctx_graph_stats": (C.c_int, [_VP, _I64P, _I64P]),
     "bspai_hip_ctx_kernel_times": (C.c_int, [_VP, _VP, C.c_int, C.POINTER(C.c_int)]),
     "bspai_hip_matrix_create": (C.c_int, [_VP, C.c_int64, C.c_int64, _VP, _VP, _VP, C.c_int,
                                           C.c_int, _VP, _PP]),

@@ -106,6 +106,12 @@ class Context:
         mode = 2 if enable == "spai" else int(enable)
         check(_lib.load().bspai_hip_ctx_set_timing(self.h, mode), self.h)
 
+    def graph_stats(self):
+        """(captured, replayed): Arnoldi-step graphs of this context's solves."""
+        cap, rep = C.c_int64(), C.c_int64()
+        check(_lib.load().bspai_hip_ctx_graph_stats(self.h, C.byref(cap), C.byref(rep)), self.h)
+        return cap.value, rep.value
+
     def kernel_times(self):
         """Per-kernel totals since timing was enabled:
         {label: dict(launches, total_ms, avg_us, bytes, flops)}."""

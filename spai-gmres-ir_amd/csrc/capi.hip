@@ -325,6 +325,16 @@ int bspai_hip_ctx_set_timing(bspai_hip_ctx *ctx, int enable) {
     });
 }
 
+int bspai_hip_ctx_graph_stats(const bspai_hip_ctx *ctx, int64_t *captured, int64_t *replayed) {
+    if (!ctx)
+        return BSPAI_HIP_EINVAL;
+    if (captured)
+        *captured = ctx->c.graphs_captured;
+    if (replayed)
+        *replayed = ctx->c.graphs_replayed;
+    return BSPAI_HIP_OK;
+}
+
 int bspai_hip_ctx_kernel_times(bspai_hip_ctx *ctx, bspai_hip_kernel_time *out, int capacity,
                                int *count) {
     if (!ctx || !count || capacity < 0 || (capacity > 0 && !out))
@@ -375,12 +385,14 @@ int bspai_hip_matrix_create(bspai_hip_ctx *ctx, int64_t n, int64_t nnz, const in
             // CsrMatrix is canonical by construction): offsets 0..nnz
             // nondecreasing, columns strictly increasing within a row and
             // inside the row's system, finite values.
+            // (all offsets first, so that no row walk leaves the arrays)
             require(ptr[0] == 0 && ptr[n] == nnz,
                     "row pointers must span [0, nnz] and be nondecreasing");
-            int sys = 0;
-            for (int64_t r = 0; r < n; ++r) {
+            for (int64_t r = 0; r < n; ++r)
                 require(ptr[r] <= ptr[r + 1],
                         "row pointers must span [0, nnz] and be nondecreasing");
+            int sys = 0;
+            for (int64_t r = 0; r < n; ++r) {
                 while (r >= offs[sys + 1])
                     ++sys;
                 for (int64_t q = ptr[r]; q < ptr[r + 1]; ++q) {

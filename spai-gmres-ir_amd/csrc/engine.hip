@@ -756,7 +756,14 @@ GmresEngine<T>::GmresEngine(Context &c, const Layout &L, int cap) : c_(c), L_(L)
     args_.q = q_.data();
     args_.h = h_.data();
     args_.defer = 0;
+    args_.kdev = nullptr;
     ensure_steps(0);
+}
+
+template <typename T> GmresEngine<T>::~GmresEngine() {
+    for (StepGraph &g : graphs_)
+        if (g.exec)
+            (void)hipGraphExecDestroy(g.exec);
 }
 
 template <typename T> void GmresEngine<T>::ensure_steps(int k) {
@@ -915,40 +922,127 @@ void GmresEngine<T>::solve(const DevCsr &m1, const DevCsr &m2, GmresMode mode, c
     // two schedules can be compared.)
     const char *every = std::getenv("BSPAI_GMRES_VERIFY_EVERY_STEP");
     bool defer = L_.nsys == 1 && !(every && std::string(every) == "1");
-    for (int k = 0; k < cap_; ++k) {
-        ensure_basis(std::min(cap_ + 1, k + 2));
-        ensure_steps(k);
-        args_.defer = defer ? 1 : 0;
+    // One Arnoldi step (src/gmres.cpp:172-254): the two products, the compact
+    // Gram-Schmidt passes and the Givens step, then (unless deferred) the
+    // verification.  a.kdev == nullptr: k is the step; else k is the offset
+    // to the device's step counter, and xs picks V_k on the device.
+    auto step = [&](const KrylovArgs<T> &a, int k, int kmax, bool verify_now, XSel xs) {
         const int *run = run_flag(k);
+        const T *vk = a.kdev ? nullptr : vptrs_[k];
         timed(c_, l1.c_str(), spmv_bytes(m1, k1), 2.0 * (double)m1.nnz, [&] {
             launch_spmv<T, T, T, T>(s, k1.lanes, tl, run, stride, m1.ptr.data(), m1.col.data(),
-                                    m1v, vptrs_[k], t_.data(), k1.per, c16a);
+                                    m1v, vk, t_.data(), k1.per, c16a, xs);
         });
         timed(c_, l2.c_str(), spmv_bytes(m2, k2), 2.0 * (double)m2.nnz, [&] {
             launch_spmv<T, T, T, T>(s, k2.lanes, tl, run, stride, m2.ptr.data(), m2.col.data(),
                                     m2v, t_.data(), w_.data(), k2.per, c16b);
         });
         timed(c_, lproj.c_str(), (k + 3) * nn * sz, 2.0 * (2 * k + 1) * nn,
-              [&] { gmres_project<T>(s, tl, args_, k); });
+              [&] { gmres_project<T>(s, tl, a, k, kmax); });
         timed(c_, lred.c_str(), 2.0 * (2 * k + 1) * tl.ntiles * 8.0, 0.0,
-              [&] { gmres_gram_reduce<T>(s, tl, args_, k); });
-        timed(c_, (k <= kGramSolveLdsMaxK ? lsol_lds : lsol).c_str(), 0.0, 0.0,
-              [&] { gmres_gram_solve<T>(s, tl, args_, k); });
+              [&] { gmres_gram_reduce<T>(s, tl, a, k, kmax); });
+        timed(c_, (kmax <= kGramSolveLdsMaxK ? lsol_lds : lsol).c_str(), 0.0, 0.0,
+              [&] { gmres_gram_solve<T>(s, tl, a, k, kmax); });
         timed(c_, lupd.c_str(), (k + 3) * nn * sz, 2.0 * (k + 1) * nn,
-              [&] { gmres_gram_update<T>(s, tl, args_, k); });
+              [&] { gmres_gram_update<T>(s, tl, a, k); });
         timed(c_, lnrm.c_str(), 2.0 * nn * sz, 0.0,
-              [&] { gmres_normalize_givens<T>(s, tl, args_, k); });
-        if (!defer)
+              [&] { gmres_normalize_givens<T>(s, tl, a, k); });
+        if (verify_now)
             verify();
-        if ((k + 1) % sync_every == 0 || k + 1 == cap_) {
+    };
+    // Replayed graphs: kGraphSteps steps captured once per (operators,
+    // workspace, schedule) and replayed between the host's check points.
+    // Their launch shapes hold any step up to the cap (blocks beyond the
+    // step exit at once), so the workspace is laid out for the cap up front.
+    const char *genv = std::getenv("BSPAI_GMRES_GRAPH");
+    const bool graphs = !(genv && std::string(genv) == "0") && !c_.timer.on &&
+                        cap_ >= kGraphSteps && sync_every % kGraphSteps == 0;
+    graphs_replayed = 0;
+    if (graphs) {
+        ensure_basis(cap_ + 1);
+        ensure_steps(cap_ - 1);
+        if (kdev_.size() == 0)
+            kdev_.resize(1);
+    }
+    auto graph_for = [&](bool verify_each) -> hipGraphExec_t {
+        KrylovArgs<T> ga = args_;
+        ga.defer = verify_each ? 0 : 1;
+        ga.kdev = kdev_.data();
+        std::vector<const void *> key = {
+            m1.ptr.data(), m1.col.data(), m1v, c16a, m2.ptr.data(), m2.col.data(), m2v, c16b,
+            rhs, prhs, x, vdev_.data(), H_.data(), gram_.data(), gp_.data(), w_.data(),
+            t_.data(), ax_.data(), pax_.data(), xz_.data(),
+            reinterpret_cast<const void *>((std::intptr_t)(k1.lanes * 64 + k1.per)),
+            reinterpret_cast<const void *>((std::intptr_t)(k2.lanes * 64 + k2.per)),
+            reinterpret_cast<const void *>((std::intptr_t)mode)};
+        StepGraph &g = graphs_[verify_each ? 1 : 0];
+        if (g.exec && g.key == key)
+            return g.exec;
+        if (g.exec) {
+            HIP_CHECK(hipGraphExecDestroy(g.exec));
+            g.exec = nullptr;
+        }
+        // the graph's arguments are read once here; the captured kernels see
+        // ga.defer and the device step counter
+        const KrylovArgs<T> saved = args_;
+        args_ = ga; // verify() launches with args_
+        hipGraph_t graph = nullptr;
+        HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        try {
+            for (int o = 0; o < kGraphSteps; ++o)
+                step(ga, o, cap_ - 1, verify_each,
+                     XSel{reinterpret_cast<const void *const *>(vdev_.data()), kdev_.data(), o,
+                          cap_});
+            gmres_advance_step(s, kdev_.data(), kGraphSteps);
+        } catch (...) {
+            (void)hipStreamEndCapture(s, &graph);
+            if (graph)
+                (void)hipGraphDestroy(graph);
+            args_ = saved;
+            throw;
+        }
+        HIP_CHECK(hipStreamEndCapture(s, &graph));
+        args_ = saved;
+        const hipError_t e = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        HIP_CHECK(e);
+        g.key = std::move(key);
+        ++graphs_captured;
+        ++c_.graphs_captured;
+        return g.exec;
+    };
+    int k = 0;
+    while (k < cap_) {
+        args_.defer = defer ? 1 : 0;
+        const int k0 = k;
+        if (graphs && k % kGraphSteps == 0) {
+            ensure_basis(cap_ + 1); // (laid out for the cap already)
+            hipGraphExec_t g = graph_for(!defer);
+            if (kdev_val_ != k)
+                gmres_set_step(s, kdev_.data(), k);
+            HIP_CHECK(hipGraphLaunch(g, s));
+            kdev_val_ = k + kGraphSteps;
+            ++graphs_replayed;
+            ++c_.graphs_replayed;
+            k = std::min(cap_, k + kGraphSteps);
+        } else {
+            ensure_basis(std::min(cap_ + 1, k + 2));
+            ensure_steps(k);
+            step(args_, k, k, !defer, XSel{});
+            ++k;
+        }
+        (void)k0;
+        if (k % sync_every == 0 || k == cap_) {
             download(c_, host.data(), st_.data(), L_.nsys);
             sync(c_);
             if (defer && host[0].active && host[0].triggered) {
+                // the held step's verification; the loop resumes after it
+                args_.defer = 1;
                 verify();
                 download(c_, host.data(), st_.data(), L_.nsys);
                 sync(c_);
                 defer = false;
-                k = host[0].iterations - 1; // the loop resumes at the held step + 1
+                k = host[0].iterations;
             }
             bool any = false;
             for (const SysState &h : host)

@@ -180,6 +180,8 @@ struct Context {
     EngineSlot<double> engine64;
     EngineSlot<float> engine32;
     IrWork ir;
+    // Arnoldi-step graphs captured and replayed by this context's solves
+    long long graphs_captured = 0, graphs_replayed = 0;
 };
 
 // Runs `launch` (one kernel launch on c.stream), timed when the context's
@@ -269,6 +271,9 @@ enum GmresMode { kLeft = 0, kRight = 1 };
 template <typename T> class GmresEngine {
   public:
     GmresEngine(Context &c, const Layout &L, int cap);
+    ~GmresEngine();
+    GmresEngine(const GmresEngine &) = delete;
+    GmresEngine &operator=(const GmresEngine &) = delete;
     // m1 then m2 applied per Arnoldi step: left (A, P), right (P^T, A^T).
     // rhs in T; x (FP64) receives the solution; ir (optional) masks systems.
     void solve(const DevCsr &m1, const DevCsr &m2, GmresMode mode, const T *rhs, const double *x0,
@@ -277,8 +282,23 @@ template <typename T> class GmresEngine {
     KrylovArgs<T> args() const { return args_; }
     int cap() const { return cap_; }
     int sync_every = 4;
+    // Arnoldi steps per replayed graph (BSPAI_GMRES_GRAPH=0 launches every
+    // kernel on the stream instead); graphs_replayed counts the replays of
+    // the last solve.
+    static constexpr int kGraphSteps = 4;
+    long long graphs_replayed = 0, graphs_captured = 0;
 
   private:
+    // A captured group of kGraphSteps Arnoldi steps (with or without the
+    // per-step verification), valid while its key -- every pointer and
+    // launch choice baked into its nodes -- is unchanged.
+    struct StepGraph {
+        hipGraphExec_t exec = nullptr;
+        std::vector<const void *> key;
+    };
+    StepGraph graphs_[2];
+    DeviceArray<int> kdev_; // the step the next replayed graph starts at
+    int kdev_val_ = -1;     // its value as the host last set or advanced it
     void ensure_basis(int count);
     // Capacity of H, the Gram triangle and the projection partials for
     // Krylov steps 0..k (grown by doubling, contents kept).

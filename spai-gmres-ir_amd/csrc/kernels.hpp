@@ -110,10 +110,17 @@ constexpr int kSpmvSub = kTileRows / kSpmvRows;
 // averaging more than 8 entries per row); the result does not depend on it.
 // col16 (lanes = 1, optional): the columns as 16-bit offsets to each 256-row
 // block's first row (launch_encode_col16); the result does not depend on it.
+// xs (optional): x is taken on the device as tab[min(*idx + off, last)],
+// e.g. the basis vector of the step a replayed graph is at; x is then unused.
+struct XSel {
+    const void *const *tab = nullptr;
+    const int *idx = nullptr;
+    int off = 0, last = 0;
+};
 template <typename TV, typename TX, typename TA, typename TY>
 void launch_spmv(hipStream_t s, int lanes, const Tiles &tiles, const int *flag, int flag_stride,
                  const int *ptr, const int *col, const TV *val, const TX *x, TY *y,
-                 int stream_per = 8, const short *col16 = nullptr);
+                 int stream_per = 8, const short *col16 = nullptr, XSel xs = XSel{});
 // col16[e] = col[e] - (first row of the SpMV row block holding entry e);
 // *bad (zeroed by the caller) is set when an offset does not fit 16 bits.
 void launch_encode_col16(hipStream_t s, const Tiles &tiles, const int *ptr, const int *col,

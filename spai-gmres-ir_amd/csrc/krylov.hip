@@ -255,6 +255,9 @@ __device__ inline void st16(T *p, const typename Vec16<T>::type &v, int lim = 16
 template <typename T, bool kUniform>
 __global__ void __launch_bounds__(kProjWaves *kWave)
 project_kernel(Tiles tiles, KrylovArgs<T> a, int k) {
+    k = step_index(a, k);
+    if (k >= a.cap) // a replayed graph's steps beyond the cap do nothing
+        return;
     using V16 = typename Vec16<T>::type;
     constexpr int E = 16 / sizeof(T);
     constexpr int G = kTileRows / (kWave * E); // 16-byte accesses per lane per vector
@@ -357,6 +360,9 @@ project_kernel(Tiles tiles, KrylovArgs<T> a, int k) {
 template <typename T>
 __global__ void __launch_bounds__(kReduceWaves *kWave)
 gram_reduce_kernel(Tiles tiles, KrylovArgs<T> a, int k) {
+    k = step_index(a, k);
+    if (k >= a.cap) // a replayed graph's steps beyond the cap do nothing
+        return;
     const int s = blockIdx.x;
     if (!a.st[s].active || !a.st[s].run[k & 1])
         return;
@@ -389,6 +395,9 @@ constexpr int kRotChunk = 512;
 constexpr int kLoadBatch = 16; // divides kWave
 template <typename T>
 __global__ void __launch_bounds__(kWave) gram_solve_kernel(Tiles tiles, KrylovArgs<T> a, int k) {
+    k = step_index(a, k);
+    if (k >= a.cap) // a replayed graph's steps beyond the cap do nothing
+        return;
     __shared__ double lcs[kRotChunk], lsn[kRotChunk], lrow[kRotChunk];
     __shared__ double tile[kWave][kWave + 1]; // the block's own triangle, [i][lane]
     const int s = blockIdx.x;
@@ -555,6 +564,9 @@ __device__ inline void gram_steps(double (&acc)[S], const double *Lc, int &i, in
 template <typename T, int S>
 __global__ void __launch_bounds__(kSolveThreads)
 gram_solve_lds_kernel(Tiles tiles, KrylovArgs<T> a, int k) {
+    k = step_index(a, k);
+    if (k >= a.cap) // a replayed graph's steps beyond the cap do nothing
+        return;
     constexpr int kSolvePad = S * kWave, kChunk = kSolveChunk<S>;
     __shared__ double Lc[kSolvePad + kChunk + kSolvePad];
     __shared__ double hs[S * kWave], lcs[S * kWave], lsn[S * kWave];
@@ -734,6 +746,9 @@ __device__ inline void upd_apply(T (&x)[G][E], const typename Vec16<T>::type (&b
 template <typename T>
 __global__ void __launch_bounds__(kUpdateThreads)
 gram_update_kernel(Tiles tiles, KrylovArgs<T> a, int k) {
+    k = step_index(a, k);
+    if (k >= a.cap) // a replayed graph's steps beyond the cap do nothing
+        return;
     using V16 = typename Vec16<T>::type;
     constexpr int E = 16 / sizeof(T);
     constexpr int G = kTileRows / (kUpdateThreads * E);
@@ -803,6 +818,9 @@ gram_update_kernel(Tiles tiles, KrylovArgs<T> a, int k) {
 template <typename T>
 __global__ void __launch_bounds__(kTileThreads)
 normalize_givens_kernel(Tiles tiles, KrylovArgs<T> a, int k) {
+    k = step_index(a, k);
+    if (k >= a.cap) // a replayed graph's steps beyond the cap do nothing
+        return;
     __shared__ double lds[kTileThreads / kWave];
     BSPAI_SYS_TILES
     SysState &st = a.st[s];
@@ -1095,27 +1113,47 @@ void gmres_gram_relayout(hipStream_t s, int nsys, const double *src, std::int64_
     HIP_LAUNCH_CHECK();
 }
 
+__global__ void set_step_kernel(int *kdev, int k) {
+    *kdev = k;
+}
+__global__ void advance_step_kernel(int *kdev, int d) {
+    *kdev += d;
+}
+
 // ------------------------------------------------------------------ launchers
 #define BSPAI_GRID dim3(tl.ntiles), dim3(kTileThreads), 0, s
 
-template <typename T> void gmres_project(hipStream_t s, const Tiles &tl, const KrylovArgs<T> &a, int k) {
-    const dim3 grid(tl.ntiles, ceil_div(k + 1, kProjVecs)), block(kProjWaves * kWave);
+void gmres_set_step(hipStream_t s, int *kdev, int k) {
+    hipLaunchKernelGGL(set_step_kernel, dim3(1), dim3(1), 0, s, kdev, k);
+    HIP_LAUNCH_CHECK();
+}
+void gmres_advance_step(hipStream_t s, int *kdev, int d) {
+    hipLaunchKernelGGL(advance_step_kernel, dim3(1), dim3(1), 0, s, kdev, d);
+    HIP_LAUNCH_CHECK();
+}
+
+template <typename T>
+void gmres_project(hipStream_t s, const Tiles &tl, const KrylovArgs<T> &a, int k, int kmax) {
+    const dim3 grid(tl.ntiles, ceil_div(kmax + 1, kProjVecs)), block(kProjWaves * kWave);
     if (tl.uniform)
         hipLaunchKernelGGL((project_kernel<T, true>), grid, block, 0, s, tl, a, k);
     else
         hipLaunchKernelGGL((project_kernel<T, false>), grid, block, 0, s, tl, a, k);
     HIP_LAUNCH_CHECK();
 }
-template <typename T> void gmres_gram_reduce(hipStream_t s, const Tiles &tl, const KrylovArgs<T> &a, int k) {
-    hipLaunchKernelGGL(gram_reduce_kernel<T>, dim3(tl.nsys, ceil_div(2 * k + 1, kReduceWaves)),
+template <typename T>
+void gmres_gram_reduce(hipStream_t s, const Tiles &tl, const KrylovArgs<T> &a, int k, int kmax) {
+    hipLaunchKernelGGL(gram_reduce_kernel<T>, dim3(tl.nsys, ceil_div(2 * kmax + 1, kReduceWaves)),
                        dim3(kReduceWaves * kWave), 0, s, tl, a, k);
     HIP_LAUNCH_CHECK();
 }
-template <typename T> void gmres_gram_solve(hipStream_t s, const Tiles &tl, const KrylovArgs<T> &a, int k) {
-    if (k < 4 * kWave)
+template <typename T>
+void gmres_gram_solve(hipStream_t s, const Tiles &tl, const KrylovArgs<T> &a, int k, int kmax) {
+    // the variant that holds the largest step the launch may see
+    if (kmax < 4 * kWave)
         hipLaunchKernelGGL((gram_solve_lds_kernel<T, 4>), dim3(tl.nsys), dim3(kSolveThreads), 0,
                            s, tl, a, k);
-    else if (k <= kGramSolveLdsMaxK)
+    else if (kmax <= kGramSolveLdsMaxK)
         hipLaunchKernelGGL((gram_solve_lds_kernel<T, 8>), dim3(tl.nsys), dim3(kSolveThreads), 0,
                            s, tl, a, k);
     else
@@ -1175,9 +1213,11 @@ void gmres_diff_partial(hipStream_t s, const Tiles &tl, SysState *st, double *pa
 }
 
 #define BSPAI_KRYLOV_INST(T)                                                                      \
-    template void gmres_project<T>(hipStream_t, const Tiles &, const KrylovArgs<T> &, int);       \
-    template void gmres_gram_reduce<T>(hipStream_t, const Tiles &, const KrylovArgs<T> &, int);   \
-    template void gmres_gram_solve<T>(hipStream_t, const Tiles &, const KrylovArgs<T> &, int);    \
+    template void gmres_project<T>(hipStream_t, const Tiles &, const KrylovArgs<T> &, int, int);  \
+    template void gmres_gram_reduce<T>(hipStream_t, const Tiles &, const KrylovArgs<T> &, int,    \
+                                       int);                                                      \
+    template void gmres_gram_solve<T>(hipStream_t, const Tiles &, const KrylovArgs<T> &, int,     \
+                                      int);                                                       \
     template void gmres_gram_update<T>(hipStream_t, const Tiles &, const KrylovArgs<T> &, int);   \
     template void gmres_normalize_givens<T>(hipStream_t, const Tiles &, const KrylovArgs<T> &,    \
                                             int);                                                 \

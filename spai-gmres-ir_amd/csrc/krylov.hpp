@@ -105,7 +105,15 @@ template <typename T> struct KrylovArgs {
     // 1: a triggered step holds the system (run cleared) until the host
     // launches the verification; 0: the verification follows every step.
     int defer;
+    // Replayed graphs: the step kernels take their step as *kdev + the
+    // offset they were launched with (nullptr: the argument is the step).
+    const int *kdev;
 };
+
+// The Arnoldi step a step kernel works on (see KrylovArgs::kdev).
+template <typename T> __device__ inline int step_index(const KrylovArgs<T> &a, int k) {
+    return a.kdev ? *a.kdev + k : k;
+}
 
 struct IrSysState {
     double ainf, bnorm, rinf, r2, nbe, cbe, xinf;
@@ -130,12 +138,22 @@ struct IrArgs {
 // Copies the Gram triangles of nsys systems from a K0-step to a K-step layout.
 void gmres_gram_relayout(hipStream_t s, int nsys, const double *src, std::int64_t sstride, int K0,
                          double *dst, std::int64_t dstride, int K);
-template <typename T> void gmres_project(hipStream_t s, const Tiles &tl, const KrylovArgs<T> &a, int k);
-template <typename T> void gmres_gram_reduce(hipStream_t s, const Tiles &tl, const KrylovArgs<T> &a, int k);
-template <typename T> void gmres_gram_solve(hipStream_t s, const Tiles &tl, const KrylovArgs<T> &a, int k);
+// k: the step (a.kdev == nullptr) or the offset to *a.kdev; kmax: the
+// largest step the launch may work on, which sizes the grids (k itself for
+// a direct launch, the Krylov cap - 1 for a replayed graph).
+template <typename T>
+void gmres_project(hipStream_t s, const Tiles &tl, const KrylovArgs<T> &a, int k, int kmax);
+template <typename T>
+void gmres_gram_reduce(hipStream_t s, const Tiles &tl, const KrylovArgs<T> &a, int k, int kmax);
+template <typename T>
+void gmres_gram_solve(hipStream_t s, const Tiles &tl, const KrylovArgs<T> &a, int k, int kmax);
 template <typename T> void gmres_gram_update(hipStream_t s, const Tiles &tl, const KrylovArgs<T> &a, int k);
 template <typename T>
 void gmres_normalize_givens(hipStream_t s, const Tiles &tl, const KrylovArgs<T> &a, int k);
+// *kdev = k (one thread): the step a replayed graph starts at.
+void gmres_set_step(hipStream_t s, int *kdev, int k);
+// *kdev += d (one thread): the last node of a replayed graph.
+void gmres_advance_step(hipStream_t s, int *kdev, int d);
 template <typename T> void gmres_candidate(hipStream_t s, const Tiles &tl, const KrylovArgs<T> &a);
 template <typename T>
 void gmres_start_vector(hipStream_t s, const Tiles &tl, const KrylovArgs<T> &a, const T *r0);

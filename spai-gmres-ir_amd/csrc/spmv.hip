@@ -28,11 +28,14 @@ template <int LANES, typename TV, typename TX, typename TA, typename TY>
 __global__ void __launch_bounds__(kSpmvRows)
 spmv_group_kernel(Tiles tiles, const int *__restrict__ flag, int flag_stride,
                   const int *__restrict__ ptr, const int *__restrict__ col,
-                  const TV *__restrict__ val, const TX *__restrict__ x, TY *__restrict__ y) {
+                  const TV *__restrict__ val, const TX *__restrict__ x, TY *__restrict__ y,
+                  XSel xs) {
     const int b = xcd_block(blockIdx.x, gridDim.x);
     const int t = b / kSpmvSub, sb = b % kSpmvSub;
     if (flag && !flag[(std::int64_t)tiles.sys[t] * flag_stride])
         return;
+    if (xs.tab)
+        x = static_cast<const TX *>(xs.tab[min(*xs.idx + xs.off, xs.last)]);
     const int rbeg = tiles.row0[t] + sb * kSpmvRows;
     const int rend = min(tiles.row1[t], rbeg + kSpmvRows);
     const int groups = kSpmvRows / LANES;
@@ -62,7 +65,7 @@ __global__ void __launch_bounds__(kSpmvRows)
 spmv_stream_kernel(Tiles tiles, const int *__restrict__ flag, int flag_stride,
                    const int *__restrict__ ptr, const int *__restrict__ col,
                    const short *__restrict__ col16, const TV *__restrict__ val,
-                   const TX *__restrict__ x, TY *__restrict__ y) {
+                   const TX *__restrict__ x, TY *__restrict__ y, XSel xs) {
     constexpr int kStreamChunk = kSpmvRows * PER;
     __shared__ int rp[kSpmvRows + 1];
     __shared__ TA prod[kStreamChunk];
@@ -70,6 +73,8 @@ spmv_stream_kernel(Tiles tiles, const int *__restrict__ flag, int flag_stride,
     const int t = b / kSpmvSub, sb = b % kSpmvSub;
     if (flag && !flag[(std::int64_t)tiles.sys[t] * flag_stride])
         return;
+    if (xs.tab)
+        x = static_cast<const TX *>(xs.tab[min(*xs.idx + xs.off, xs.last)]);
     const int rbeg = tiles.row0[t] + sb * kSpmvRows;
     const int rend = min(tiles.row1[t], rbeg + kSpmvRows);
     if (rbeg >= rend)
@@ -152,33 +157,33 @@ void launch_encode_col16(hipStream_t s, const Tiles &tiles, const int *ptr, cons
 template <typename TV, typename TX, typename TA, typename TY>
 void launch_spmv(hipStream_t s, int lanes, const Tiles &tiles, const int *flag, int flag_stride,
                  const int *ptr, const int *col, const TV *val, const TX *x, TY *y,
-                 int stream_per, const short *col16) {
+                 int stream_per, const short *col16, XSel xs) {
     if (tiles.ntiles == 0)
         return;
     const dim3 grid((unsigned)tiles.ntiles * kSpmvSub), block(kSpmvRows);
 #define BSPAI_SPMV_CASE(L)                                                                        \
     case L:                                                                                      \
         hipLaunchKernelGGL((spmv_group_kernel<L, TV, TX, TA, TY>), grid, block, 0, s, tiles,     \
-                           flag, flag_stride, ptr, col, val, x, y);                              \
+                           flag, flag_stride, ptr, col, val, x, y, xs);                          \
         break;
     switch (lanes) {
     case 1:
         if (col16 && stream_per > 8)
             hipLaunchKernelGGL((spmv_stream_kernel<TV, TX, TA, TY, 16, true>), grid, block, 0, s,
-                               tiles, flag, flag_stride, ptr, col, col16, val, x, y);
+                               tiles, flag, flag_stride, ptr, col, col16, val, x, y, xs);
         else if (col16)
             hipLaunchKernelGGL((spmv_stream_kernel<TV, TX, TA, TY, 8, true>), grid, block, 0, s,
-                               tiles, flag, flag_stride, ptr, col, col16, val, x, y);
+                               tiles, flag, flag_stride, ptr, col, col16, val, x, y, xs);
         else if (stream_per > 8)
             hipLaunchKernelGGL((spmv_stream_kernel<TV, TX, TA, TY, 16, false>), grid, block, 0, s,
-                               tiles, flag, flag_stride, ptr, col, col16, val, x, y);
+                               tiles, flag, flag_stride, ptr, col, col16, val, x, y, xs);
         else
             hipLaunchKernelGGL((spmv_stream_kernel<TV, TX, TA, TY, 8, false>), grid, block, 0, s,
-                               tiles, flag, flag_stride, ptr, col, col16, val, x, y);
+                               tiles, flag, flag_stride, ptr, col, col16, val, x, y, xs);
         break;
     case -1:
         hipLaunchKernelGGL((spmv_group_kernel<1, TV, TX, TA, TY>), grid, block, 0, s, tiles, flag,
-                           flag_stride, ptr, col, val, x, y);
+                           flag_stride, ptr, col, val, x, y, xs);
         break;
         BSPAI_SPMV_CASE(2)
         BSPAI_SPMV_CASE(4)
@@ -196,7 +201,7 @@ void launch_spmv(hipStream_t s, int lanes, const Tiles &tiles, const int *flag, 
 #define BSPAI_SPMV_INST(TV, TX, TA, TY)                                                           \
     template void launch_spmv<TV, TX, TA, TY>(hipStream_t, int, const Tiles &, const int *, int,  \
                                               const int *, const int *, const TV *, const TX *,   \
-                                              TY *, int, const short *);
+                                              TY *, int, const short *, XSel);
 BSPAI_SPMV_INST(double, double, double, double)
 BSPAI_SPMV_INST(float, float, float, float)
 BSPAI_SPMV_INST(float, double, double, double)

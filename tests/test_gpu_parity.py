@@ -530,3 +530,36 @@ def test_gmres_second_inner_solve_same_input(ctx):
     assert abs(g.iterations - ref.iterations) <= 1
     assert np.linalg.norm(np.asarray(g.solution) - ref.solution) <= 1e-6 * np.linalg.norm(
         ref.solution)
+
+
+@pytest.mark.parametrize("case", ["c1_fp64", "c1_mixed", "cube32_mixed", "batched"])
+def test_graph_replay_same_bits(monkeypatch, case):
+    """Arnoldi steps replayed from captured graphs (the step read from device
+    memory, launch shapes sized for the cap) against the same kernels
+    launched one by one: identical statuses, counts and solution bits."""
+    import bspai_hip
+
+    P = _problems()
+    if case == "batched":
+        sb = P.many_systems(count=12, lmax=80)
+        a, b, systems = sb.matrix, sb.rhs, sb.offsets
+    else:
+        a = P.convection_diffusion_3d(32) if case.startswith("cube") else P.convection_diffusion_2d()
+        b, systems = P.equal_rhs(a.n), None
+    prec = "fp64" if case.endswith("fp64") or case == "batched" else "mixed"
+    kw = dict(precision=prec, tolerance=1e-8 if prec == "fp64" else 1e-4, max_iterations=200,
+              relres_target=1e-12 if prec == "mixed" else 0.0)
+    out = {}
+    for mode in ("0", "1"):
+        monkeypatch.setenv("BSPAI_GMRES_GRAPH", mode)
+        ctx = bspai_hip.Context(0)
+        m = bspai_hip.Matrix(a, ctx, systems=systems)
+        r = bspai_hip.spai_gmres_ir(m, b, **kw)
+        rs = r if isinstance(r, list) else [r]
+        out[mode] = ([(x.status, x.outer_steps, x.inner_iterations) for x in rs],
+                     bits(np.asarray(rs[0].solution)), ctx.graph_stats())
+    assert out["0"][0] == out["1"][0]
+    assert all(s == "success" for s, _, _ in out["1"][0])
+    assert np.array_equal(out["0"][1], out["1"][1])
+    assert out["0"][2] == (0, 0)
+    assert out["1"][2][1] > 0  # graphs were replayed
