@@ -189,6 +189,22 @@ int bspai_hip_csr_from_entries(bspai_hip_ctx *ctx, int64_t rows, int64_t cols, i
                                int where, int32_t *ptr, int32_t *col_out, double *val_out,
                                int out_where, int64_t *nnz);
 
+/* Allocation-free variant of bspai_hip_csr_from_entries.  No counterpart in
+ * the reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same result, messages and error order as
+ * bspai_hip_csr_from_entries; every array (row, col, val in; ptr, col_out,
+ * val_out out) must be device memory.  It allocates nothing: its scratch is a
+ * static device array of 48 MiB per GPU, held until the process ends, whose
+ * address bspai_hip_ctx_create looks up once per device.  Limits:
+ * count <= 524288 (2^19) entries and rows < 4194304 (2^22); larger inputs
+ * return BSPAI_HIP_EINVAL.  Every call on one GPU shares that array: calls
+ * that overlap on one GPU (several threads or streams) race.  Synchronous. */
+int bspai_hip_csr_from_entries_noalloc(bspai_hip_ctx *ctx, int64_t rows, int64_t cols,
+                                       int64_t count, const int64_t *row, const int64_t *col,
+                                       const double *val, int32_t *ptr, int32_t *col_out,
+                                       double *val_out, int64_t *nnz);
+
 /* Matrix Market coordinate reader (host code, several threads; no GPU use).
  * Parses `len` bytes of text with the reference reader's rules and messages
  * (read_matrix_market, src/cpu.cpp:105-161 of the reference): banner

@@ -96,6 +96,9 @@ SIGNATURES = {
     "bspai_hip_csr_from_entries": (C.c_int, [_VP, C.c_int64, C.c_int64, C.c_int64, _VP, _VP,
                                              _VP, C.c_int, _VP, _VP, _VP, C.c_int,
                                              C.POINTER(C.c_int64)]),
+    "bspai_hip_csr_from_entries_noalloc": (C.c_int, [_VP, C.c_int64, C.c_int64, C.c_int64, _VP,
+                                                     _VP, _VP, _VP, _VP, _VP,
+                                                     C.POINTER(C.c_int64)]),
     "bspai_hip_mm_parse": (C.c_int, [C.c_char_p, C.c_int64, C.c_int, _PP, C.c_char_p, C.c_int]),
     "bspai_hip_mm_info": (C.c_int, [_VP, _I64P, _I64P, _I64P, C.POINTER(C.c_int),
                                     C.POINTER(C.c_int)]),

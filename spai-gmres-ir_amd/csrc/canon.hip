@@ -22,6 +22,8 @@
 #include <hipcub/hipcub.hpp>
 
 #include <climits>
+#include <map>
+#include <mutex>
 #include <stdexcept>
 
 namespace bspai_hip {
@@ -99,77 +101,90 @@ int canon_blocks(std::int64_t count) {
     return (int)std::max<std::int64_t>(1, std::min<std::int64_t>(b, 1 << 16));
 }
 
-} // namespace
+__global__ void fill_u32_kernel(unsigned *__restrict__ p, std::int64_t n, unsigned v) {
+    const std::int64_t stride = (std::int64_t)gridDim.x * blockDim.x;
+    for (std::int64_t i = (std::int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+        p[i] = v;
+}
 
-std::int64_t csr_from_entries(Context &c, std::int64_t rows, std::int64_t cols,
-                              std::int64_t count, const std::int64_t *row,
-                              const std::int64_t *col, const double *val, int *ptr, int *out_col,
-                              double *out_val) {
+// Clears with a kernel rather than hipMemsetAsync, so the sequence stays
+// valid inside a stream that a caller captures into a graph.
+void fill_u32(hipStream_t s, void *p, std::int64_t words, unsigned v) {
+    if (words <= 0)
+        return;
+    const int blocks = (int)std::min<std::int64_t>(1024, (words + 255) / 256);
+    hipLaunchKernelGGL(fill_u32_kernel, dim3(blocks), dim3(256), 0, s, static_cast<unsigned *>(p),
+                       words, v);
+    HIP_LAUNCH_CHECK();
+}
+
+// The scratch of one canonicalization: either the context's own arrays or a
+// carve of the static scratch array (bspai_hip_csr_from_entries_noalloc).
+struct CanonBuffers {
+    unsigned long long *keys, *keys_sorted, *flags64;
+    double *vals_sorted, *sums;
+    int *keep, *slot, *flags, *scan_tmp, *row_tmp;
+    void *sort_tmp;
+    std::size_t sort_tmp_bytes;
+};
+
+// Radix-sort temporary storage for `count` pairs on the key bits [0, end_bit).
+std::size_t sort_tmp_bytes(std::int64_t count, int end_bit, hipStream_t s) {
+    std::size_t bytes = 0;
+    HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(
+        nullptr, bytes, (const unsigned long long *)nullptr, (unsigned long long *)nullptr,
+        (const double *)nullptr, (double *)nullptr, (int)count, 0, end_bit, s));
+    return bytes;
+}
+
+int key_bits(std::int64_t rows, std::int64_t cols) {
+    const unsigned long long max_key =
+        (unsigned long long)rows * (unsigned long long)cols - 1ull;
+    return max_key == 0 ? 1 : 64 - __builtin_clzll(max_key);
+}
+
+// The canonicalization proper, on scratch w (count > 0).
+std::int64_t canon_run(Context &c, std::int64_t rows, std::int64_t cols, std::int64_t count,
+                       const std::int64_t *row, const std::int64_t *col, const double *val,
+                       int *ptr, int *out_col, double *out_val, const CanonBuffers &w) {
     const hipStream_t s = c.stream;
-    CanonWork &w = c.canon;
-    HIP_CHECK(hipMemsetAsync(ptr, 0, (rows + 1) * sizeof(int), s));
-    if (count == 0) {
-        HIP_CHECK(hipStreamSynchronize(s));
-        return 0;
-    }
     const int blocks = canon_blocks(count);
-    w.keys.resize(count);
-    w.keys_sorted.resize(count);
-    w.vals_sorted.resize(count);
-    w.sums.resize(count);
-    w.keep.resize(count);
-    w.slot.resize(count);
-    w.flags64.resize(1);
-    w.flags.resize(1);
-    w.scan_tmp.resize(scan_tmp_ints(count));
-    w.row_tmp.resize(scan_tmp_ints(rows + 1));
-    HIP_CHECK(hipMemsetAsync(w.flags64.data(), 0xff, sizeof(unsigned long long), s));
-    HIP_CHECK(hipMemsetAsync(w.flags.data(), 0, sizeof(int), s));
+    fill_u32(s, w.flags64, 2, 0xffffffffu);
+    fill_u32(s, w.flags, 1, 0u);
     // One pass to the end, then one synchronization: an invalid entry gets
     // key 0 (in range), so the later stages stay in bounds, and the errors
     // are reported in the reference's order (validation first).
     timed(c, "entries_check_kernel", 32.0 * (double)count, 0.0, [&] {
         hipLaunchKernelGGL(entries_check_kernel, dim3(blocks), dim3(kCanonThreads), 0, s, count,
-                           rows, cols, row, col, val, w.keys.data(), w.flags64.data());
+                           rows, cols, row, col, val, w.keys, w.flags64);
         HIP_LAUNCH_CHECK();
     });
     // Sort on the key bits in use only.
-    const unsigned long long max_key =
-        (unsigned long long)rows * (unsigned long long)cols - 1ull;
-    const int end_bit = max_key == 0 ? 1 : 64 - __builtin_clzll(max_key);
-    std::size_t tmp_bytes = 0;
-    HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, w.keys.data(),
-                                                 w.keys_sorted.data(), val, w.vals_sorted.data(),
-                                                 (int)count, 0, end_bit, s));
-    w.sort_tmp.resize(std::max<std::size_t>(tmp_bytes, 1));
+    const int end_bit = key_bits(rows, cols);
+    std::size_t tmp_bytes = w.sort_tmp_bytes;
     timed(c, "entries_radix_sort", 4.0 * 16.0 * (double)count, 0.0, [&] {
-        HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(w.sort_tmp.data(), tmp_bytes, w.keys.data(),
-                                                     w.keys_sorted.data(), val,
-                                                     w.vals_sorted.data(), (int)count, 0, end_bit,
-                                                     s));
+        HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(w.sort_tmp, tmp_bytes, w.keys,
+                                                     w.keys_sorted, val, w.vals_sorted,
+                                                     (int)count, 0, end_bit, s));
     });
     timed(c, "run_sums_kernel", 28.0 * (double)count, 0.0, [&] {
         hipLaunchKernelGGL(run_sums_kernel, dim3(blocks), dim3(kCanonThreads), 0, s, count,
-                           w.keys_sorted.data(), w.vals_sorted.data(), w.keep.data(),
-                           w.sums.data(), w.flags.data());
+                           w.keys_sorted, w.vals_sorted, w.keep, w.sums, w.flags);
         HIP_LAUNCH_CHECK();
     });
-    HIP_CHECK(hipMemcpyAsync(w.slot.data(), w.keep.data(), count * sizeof(int),
-                             hipMemcpyDeviceToDevice, s));
-    scan_inclusive_int(s, w.slot.data(), count, w.scan_tmp.data());
+    HIP_CHECK(hipMemcpyAsync(w.slot, w.keep, count * sizeof(int), hipMemcpyDeviceToDevice, s));
+    scan_inclusive_int(s, w.slot, count, w.scan_tmp);
     timed(c, "scatter_kernel", 20.0 * (double)count, 0.0, [&] {
         hipLaunchKernelGGL(scatter_kernel, dim3(blocks), dim3(kCanonThreads), 0, s, count, cols,
-                           w.keys_sorted.data(), w.keep.data(), w.slot.data(), w.sums.data(),
-                           out_col, out_val, ptr);
+                           w.keys_sorted, w.keep, w.slot, w.sums, out_col, out_val, ptr);
         HIP_LAUNCH_CHECK();
     });
-    scan_inclusive_int(s, ptr, rows + 1, w.row_tmp.data());
+    scan_inclusive_int(s, ptr, rows + 1, w.row_tmp);
     unsigned long long bad = 0;
     int flag = 0, nnz = 0;
-    HIP_CHECK(hipMemcpyAsync(&bad, w.flags64.data(), sizeof bad, hipMemcpyDeviceToHost, s));
-    HIP_CHECK(hipMemcpyAsync(&flag, w.flags.data(), sizeof flag, hipMemcpyDeviceToHost, s));
-    HIP_CHECK(hipMemcpyAsync(&nnz, w.slot.data() + (count - 1), sizeof nnz,
-                             hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipMemcpyAsync(&bad, w.flags64, sizeof bad, hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipMemcpyAsync(&flag, w.flags, sizeof flag, hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipMemcpyAsync(&nnz, w.slot + (count - 1), sizeof nnz, hipMemcpyDeviceToHost, s));
     HIP_CHECK(hipStreamSynchronize(s));
     if (bad != ULLONG_MAX) {
         // The reference checks the indices of an entry before its value.
@@ -183,6 +198,138 @@ std::int64_t csr_from_entries(Context &c, std::int64_t rows, std::int64_t cols,
     if (flag)
         throw std::invalid_argument("Duplicate sum is nonfinite");
     return nnz;
+}
+
+} // namespace
+
+std::int64_t csr_from_entries(Context &c, std::int64_t rows, std::int64_t cols,
+                              std::int64_t count, const std::int64_t *row,
+                              const std::int64_t *col, const double *val, int *ptr, int *out_col,
+                              double *out_val) {
+    const hipStream_t s = c.stream;
+    fill_u32(s, ptr, rows + 1, 0u);
+    if (count == 0) {
+        HIP_CHECK(hipStreamSynchronize(s));
+        return 0;
+    }
+    CanonWork &w = c.canon;
+    w.keys.resize(count);
+    w.keys_sorted.resize(count);
+    w.vals_sorted.resize(count);
+    w.sums.resize(count);
+    w.keep.resize(count);
+    w.slot.resize(count);
+    w.flags64.resize(1);
+    w.flags.resize(1);
+    w.scan_tmp.resize(scan_tmp_ints(count));
+    w.row_tmp.resize(scan_tmp_ints(rows + 1));
+    const std::size_t tb = sort_tmp_bytes(count, key_bits(rows, cols), s);
+    w.sort_tmp.resize(std::max<std::size_t>(tb, 1));
+    const CanonBuffers b{w.keys.data(), w.keys_sorted.data(), w.flags64.data(),
+                         w.vals_sorted.data(), w.sums.data(), w.keep.data(),
+                         w.slot.data(), w.flags.data(), w.scan_tmp.data(),
+                         w.row_tmp.data(), w.sort_tmp.data(), tb};
+    return canon_run(c, rows, cols, count, row, col, val, ptr, out_col, out_val, b);
+}
+
+// ------------------------------------------------- allocation-free variant
+// The scratch of bspai_hip_csr_from_entries_noalloc: one static device array,
+// each GPU holding its own copy until the process ends.  Host code never uses
+// the name `scratch` as an address: noalloc_setup looks the device address up
+// once per device (with that device current) and every launch takes it from
+// there.
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace {
+
+struct NoAllocDevice {
+    void *base = nullptr;      // device address of this GPU's scratch
+    std::size_t sort_bytes = 0; // radix-sort temporary for kNoAllocMaxEntries pairs
+};
+std::mutex g_noalloc_mutex;
+std::map<int, NoAllocDevice> g_noalloc;
+
+// Offsets of the carve for `count` entries and `rows` rows (each 256-aligned).
+struct NoAllocCarve {
+    std::size_t keys, keys_sorted, flags64, vals_sorted, sums, keep, slot, flags, scan_tmp,
+        row_tmp, sort_tmp, end;
+};
+NoAllocCarve carve(std::int64_t count, std::int64_t rows, std::size_t sort_bytes) {
+    auto up = [](std::size_t x) { return (x + 255) & ~(std::size_t)255; };
+    NoAllocCarve o{};
+    std::size_t at = 0;
+    auto take = [&](std::size_t &field, std::size_t bytes) {
+        field = at;
+        at = up(at + bytes);
+    };
+    take(o.keys, 8 * (std::size_t)count);
+    take(o.keys_sorted, 8 * (std::size_t)count);
+    take(o.flags64, 8);
+    take(o.vals_sorted, 8 * (std::size_t)count);
+    take(o.sums, 8 * (std::size_t)count);
+    take(o.keep, 4 * (std::size_t)count);
+    take(o.slot, 4 * (std::size_t)count);
+    take(o.flags, 4);
+    take(o.scan_tmp, 4 * (std::size_t)scan_tmp_ints(count));
+    take(o.row_tmp, 4 * (std::size_t)scan_tmp_ints(rows + 1));
+    take(o.sort_tmp, sort_bytes);
+    o.end = at;
+    return o;
+}
+
+} // namespace
+
+void noalloc_setup(Context &c) {
+    std::lock_guard<std::mutex> lock(g_noalloc_mutex);
+    if (g_noalloc.count(c.device))
+        return;
+    NoAllocDevice d;
+    HIP_CHECK(hipGetSymbolAddress(&d.base, HIP_SYMBOL(scratch)));
+    if (!d.base)
+        throw HipError("hipGetSymbolAddress returned no address for the scratch array");
+    d.sort_bytes = sort_tmp_bytes(kNoAllocMaxEntries, 64, c.stream);
+    g_noalloc[c.device] = d;
+}
+
+std::int64_t csr_from_entries_noalloc(Context &c, std::int64_t rows, std::int64_t cols,
+                                      std::int64_t count, const std::int64_t *row,
+                                      const std::int64_t *col, const double *val, int *ptr,
+                                      int *out_col, double *out_val) {
+    NoAllocDevice d;
+    {
+        std::lock_guard<std::mutex> lock(g_noalloc_mutex);
+        const auto it = g_noalloc.find(c.device);
+        if (it == g_noalloc.end())
+            throw HipError("allocation-free scratch not set up on this device");
+        d = it->second;
+    }
+    if (count > kNoAllocMaxEntries || rows + 1 > kNoAllocMaxRows)
+        throw std::invalid_argument("allocation-free canonicalization: at most " +
+                                    std::to_string(kNoAllocMaxEntries) + " entries and " +
+                                    std::to_string(kNoAllocMaxRows - 1) + " rows");
+    const NoAllocCarve o = carve(kNoAllocMaxEntries, kNoAllocMaxRows - 1, d.sort_bytes);
+    if (o.end > kNoAllocScratchBytes)
+        throw HipError("allocation-free scratch too small for its own limits");
+    const hipStream_t s = c.stream;
+    fill_u32(s, ptr, rows + 1, 0u);
+    if (count == 0) {
+        HIP_CHECK(hipStreamSynchronize(s));
+        return 0;
+    }
+    unsigned char *b = static_cast<unsigned char *>(d.base);
+    const CanonBuffers w{reinterpret_cast<unsigned long long *>(b + o.keys),
+                         reinterpret_cast<unsigned long long *>(b + o.keys_sorted),
+                         reinterpret_cast<unsigned long long *>(b + o.flags64),
+                         reinterpret_cast<double *>(b + o.vals_sorted),
+                         reinterpret_cast<double *>(b + o.sums),
+                         reinterpret_cast<int *>(b + o.keep),
+                         reinterpret_cast<int *>(b + o.slot),
+                         reinterpret_cast<int *>(b + o.flags),
+                         reinterpret_cast<int *>(b + o.scan_tmp),
+                         reinterpret_cast<int *>(b + o.row_tmp),
+                         b + o.sort_tmp,
+                         d.sort_bytes};
+    return canon_run(c, rows, cols, count, row, col, val, ptr, out_col, out_val, w);
 }
 
 // ------------------------------------------------------------ CSR checks

@@ -281,6 +281,8 @@ int bspai_hip_ctx_create(int device, void *stream, bspai_hip_ctx **out) {
             }
             HIP_CHECK(hipEventCreate(&ctx->c.ev0));
             HIP_CHECK(hipEventCreate(&ctx->c.ev1));
+            // the allocation-free entry point's scratch address on this device
+            noalloc_setup(ctx->c);
         });
         if (rc != BSPAI_HIP_OK) {
             delete ctx;
@@ -498,6 +500,33 @@ int bspai_hip_csr_from_entries(bspai_hip_ctx *ctx, int64_t rows, int64_t cols, i
         }
         HIP_CHECK(hipStreamSynchronize(c.stream));
         *nnz = k;
+    });
+}
+
+int bspai_hip_csr_from_entries_noalloc(bspai_hip_ctx *ctx, int64_t rows, int64_t cols,
+                                       int64_t count, const int64_t *row, const int64_t *col,
+                                       const double *val, int32_t *ptr, int32_t *col_out,
+                                       double *val_out, int64_t *nnz) {
+    return guard(ctx, [&] {
+        require(ctx && ptr && nnz, "null argument");
+        require(count == 0 || (row && col && val && col_out && val_out), "null argument");
+        require(rows >= 0 && rows < INT_MAX && cols >= 0 && cols < INT_MAX && count >= 0 &&
+                    count < INT_MAX,
+                "matrix size out of range");
+        // every array must be device memory: nothing is staged through the host
+        auto on_device = [](const void *p) {
+            if (!p)
+                return true;
+            hipPointerAttribute_t at{};
+            const hipError_t e = hipPointerGetAttributes(&at, p);
+            (void)hipGetLastError(); // an unknown (host) pointer is not an error to keep
+            return e == hipSuccess && (at.type == hipMemoryTypeDevice || at.isManaged);
+        };
+        require(on_device(row) && on_device(col) && on_device(val) && on_device(ptr) &&
+                    on_device(col_out) && on_device(val_out),
+                "allocation-free canonicalization takes device arrays only");
+        *nnz = csr_from_entries_noalloc(ctx->c, rows, cols, count, row, col, val, ptr, col_out,
+                                        val_out);
     });
 }
 

@@ -350,6 +350,20 @@ void spai_score_on_device(hipStream_t s, double rr, double dot, double den, int 
 // Returns nnz.  Throws std::invalid_argument with the reference's messages
 // for an index out of range, a non-finite value or a non-finite duplicate
 // sum; the outputs are then unspecified.
+// Allocation-free canonicalization (bspai_hip_csr_from_entries_noalloc): its
+// scratch is a static device array of kNoAllocScratchBytes per GPU, set up by
+// noalloc_setup (once per device, at context creation).  Limits: count <=
+// kNoAllocMaxEntries, rows + 1 <= kNoAllocMaxRows.  Calls that overlap on one
+// GPU share the array and race.
+constexpr std::int64_t kNoAllocMaxEntries = 1 << 19;
+constexpr std::int64_t kNoAllocMaxRows = 1 << 22;
+constexpr std::size_t kNoAllocScratchBytes = (std::size_t)48 << 20;
+void noalloc_setup(Context &c);
+std::int64_t csr_from_entries_noalloc(Context &c, std::int64_t rows, std::int64_t cols,
+                                      std::int64_t count, const std::int64_t *row,
+                                      const std::int64_t *col, const double *val, int *ptr,
+                                      int *out_col, double *out_val);
+
 // Checks a device CSR before use: offsets 0..nnz nondecreasing, then every
 // row's columns in range, strictly increasing and inside the row's system,
 // and finite values.  Throws std::invalid_argument naming the first problem
