@@ -404,16 +404,18 @@ void spai_build_typed(Context &c, const DevCsr &a, const T *aval, T *d, double e
     bcol.resize(a.nnz > 0 ? a.nnz : 1);
     cursor.resize(n + 1);
     scan_tmp.resize(scan_tmp_ints(n + 1));
-    w.bval.resize((wave_fp32 && a.nnz > 0 ? a.nnz : 1) * sizeof(T));
+    // B's values: the FP32 kernels score candidates from them
+    const bool need_bval = std::is_same<T, float>::value;
+    w.bval.resize((need_bval && a.nnz > 0 ? a.nnz : 1) * sizeof(T));
     T *const bval = w.bval.as<T>();
     HIP_CHECK(hipMemsetAsync(bptr.data(), 0, (n + 1) * sizeof(int), s));
     launch_count_cols(s, a.nnz, a.col.data(), bptr.data());
     scan_inclusive_int(s, bptr.data(), n + 1, scan_tmp.data());
     HIP_CHECK(hipMemcpyAsync(cursor.data(), bptr.data(), (n + 1) * sizeof(int),
                              hipMemcpyDeviceToDevice, s));
-    launch_transpose_fill<T>(s, n, a.ptr.data(), a.col.data(), wave_fp32 ? aval : nullptr,
+    launch_transpose_fill<T>(s, n, a.ptr.data(), a.col.data(), need_bval ? aval : nullptr,
                              bptr.data(), cursor.data(), bcol.data(),
-                             wave_fp32 ? bval : nullptr);
+                             need_bval ? bval : nullptr);
     // ---- columns, in capacity passes
     out.c_status.resize(n);
     out.c_fits.resize(n);
@@ -464,8 +466,9 @@ void spai_build_typed(Context &c, const DevCsr &a, const T *aval, T *d, double e
             qcap = (int)std::min<std::int64_t>(qcap, n);
             ccap = (int)std::min<std::int64_t>(ccap, n);
             const int words = (int)((n + 31) / 32);
-            const std::int64_t bd = spai_block_doubles(mcap, qcap, ccap);
-            const std::int64_t bi = spai_block_ints(mcap, qcap, ccap, words, n);
+            const std::int64_t pcap = spai_pair_cap(ccap, n, a.nnz);
+            const std::int64_t bd = spai_block_doubles(mcap, qcap, ccap, pcap);
+            const std::int64_t bi = spai_block_ints(mcap, qcap, ccap, pcap, words, n);
             const std::int64_t per_block = bd * (std::int64_t)sizeof(T) + bi * 4;
             std::int64_t blocks = std::min<std::int64_t>(nwork, cus * 4LL);
             // Large buckets hold tens of MB per workgroup: allow up to a quarter
@@ -492,10 +495,12 @@ void spai_build_typed(Context &c, const DevCsr &a, const T *aval, T *d, double e
             pass_labels.push_back(label);
             timed(c, label.c_str(), 0.0, 0.0, [&] {
                 launch_spai_block<T>(s, (int)blocks, n, a.ptr.data(), a.col.data(), aval, d,
-                                     bptr.data(), bcol.data(), (T)eps, alpha, beta, work.data(),
+                                     bptr.data(), bcol.data(), need_bval ? bval : nullptr,
+                                     (T)eps, alpha, beta, work.data(),
                                      nwork, counter.data(), mcap, qcap, ccap, dwork.as<T>(),
                                      iwork.as<int>(), o, out.pool_j.back().data(),
-                                     out.pool_c.back().as<T>(), qcap, pass == 1 ? 4 : 3);
+                                     out.pool_c.back().as<T>(), qcap, pass == 1 ? 4 : 3,
+                                     pcap);
             }, true);
             hipLaunchKernelGGL(collect_overflow_kernel, dim3(ceil_div(nwork, 256)), dim3(256), 0,
                                s, nwork, work.data(), out.c_status.data(), next.data(),
@@ -952,19 +957,17 @@ void GmresEngine<T>::solve(const DevCsr &m1, const DevCsr &m2, GmresMode mode, c
     };
     // Replayed graphs: kGraphSteps steps captured once per (operators,
     // workspace, schedule) and replayed between the host's check points.
-    // Their launch shapes hold any step up to the cap (blocks beyond the
-    // step exit at once), so the workspace is laid out for the cap up front.
+    // Their launch shapes hold any step the workspace holds (blocks beyond
+    // the step exit at once); the workspace still grows on demand (doubling
+    // from 64 steps), and a growth moves H, the Gram triangle and the
+    // partials, so the graph is captured again for the larger capacity.
     const char *genv = std::getenv("BSPAI_GMRES_GRAPH");
     const bool graphs = !(genv && std::string(genv) == "0") && !c_.timer.on &&
                         cap_ >= kGraphSteps && sync_every % kGraphSteps == 0;
     graphs_replayed = 0;
-    if (graphs) {
-        ensure_basis(cap_ + 1);
-        ensure_steps(cap_ - 1);
-        if (kdev_.size() == 0)
-            kdev_.resize(1);
-    }
-    auto graph_for = [&](bool verify_each) -> hipGraphExec_t {
+    if (graphs && kdev_.size() == 0)
+        kdev_.resize(1);
+    auto graph_for = [&](bool verify_each, int kmax) -> hipGraphExec_t {
         KrylovArgs<T> ga = args_;
         ga.defer = verify_each ? 0 : 1;
         ga.kdev = kdev_.data();
@@ -974,7 +977,8 @@ void GmresEngine<T>::solve(const DevCsr &m1, const DevCsr &m2, GmresMode mode, c
             t_.data(), ax_.data(), pax_.data(), xz_.data(),
             reinterpret_cast<const void *>((std::intptr_t)(k1.lanes * 64 + k1.per)),
             reinterpret_cast<const void *>((std::intptr_t)(k2.lanes * 64 + k2.per)),
-            reinterpret_cast<const void *>((std::intptr_t)mode)};
+            reinterpret_cast<const void *>((std::intptr_t)mode),
+            reinterpret_cast<const void *>((std::intptr_t)kmax)};
         StepGraph &g = graphs_[verify_each ? 1 : 0];
         if (g.exec && g.key == key)
             return g.exec;
@@ -990,7 +994,7 @@ void GmresEngine<T>::solve(const DevCsr &m1, const DevCsr &m2, GmresMode mode, c
         HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
         try {
             for (int o = 0; o < kGraphSteps; ++o)
-                step(ga, o, cap_ - 1, verify_each,
+                step(ga, o, kmax, verify_each,
                      XSel{reinterpret_cast<const void *const *>(vdev_.data()), kdev_.data(), o,
                           cap_});
             gmres_advance_step(s, kdev_.data(), kGraphSteps);
@@ -1016,8 +1020,10 @@ void GmresEngine<T>::solve(const DevCsr &m1, const DevCsr &m2, GmresMode mode, c
         args_.defer = defer ? 1 : 0;
         const int k0 = k;
         if (graphs && k % kGraphSteps == 0) {
-            ensure_basis(cap_ + 1); // (laid out for the cap already)
-            hipGraphExec_t g = graph_for(!defer);
+            // room for the graph's steps k .. k + kGraphSteps - 1
+            ensure_basis(std::min(cap_ + 1, k + kGraphSteps + 1));
+            ensure_steps(std::min(cap_ - 1, k + kGraphSteps - 1));
+            hipGraphExec_t g = graph_for(!defer, ksteps_ - 1);
             if (kdev_val_ != k)
                 gmres_set_step(s, kdev_.data(), k);
             HIP_CHECK(hipGraphLaunch(g, s));

@@ -55,17 +55,27 @@ void launch_spai_wave(hipStream_t s, int blocks, const int *aptr, const int *aco
 int spai_wave_max_n();
 int spai_wave_qcap();
 // Workgroup-per-column kernel for the columns that outgrow the lane kernel.
-std::int64_t spai_block_doubles(int mcap, int qcap, int ccap);
-std::int64_t spai_block_ints(int mcap, int qcap, int ccap, int words, int n);
+// pcap: (FP32) the (row, candidate) pairs one fit may hold (see spai_pair_cap)
+std::int64_t spai_block_doubles(int mcap, int qcap, int ccap, std::int64_t pcap);
+std::int64_t spai_block_ints(int mcap, int qcap, int ccap, std::int64_t pcap, int words, int n);
+// Pair capacity of a workgroup pass: 8 per candidate of the cap, or every
+// stored entry of the matrix (no fit can need more) once the caps reach n.
+inline std::int64_t spai_pair_cap(int ccap, int n, std::int64_t nnz) {
+    const std::int64_t want = ccap >= n ? nnz : 8LL * ccap;
+    return want < nnz ? (want > 0 ? want : 1) : (nnz > 0 ? nnz : 1);
+}
 // p[b*stride + offset + i] = value for b < blocks, i < count.
 void launch_fill_int(hipStream_t s, std::int64_t blocks, std::int64_t stride, std::int64_t offset,
                      int count, int *p, int value);
+// bval: B's values aligned with bcol (the FP32 build scores candidates from
+// them); may be null for FP64.
 template <typename T>
 void launch_spai_block(hipStream_t s, int blocks, int n, const int *aptr, const int *acol,
-                       const T *aval, const T *dscale, const int *bptr, const int *bcol, T eps,
-                       long long alpha, int beta, const int *worklist, int nwork, int *cursor,
-                       int mcap, int qcap, int ccap, T *dwork, int *iwork, SpaiColumnOut out,
-                       int *out_j, T *out_c, std::int64_t out_stride, int occupancy);
+                       const T *aval, const T *dscale, const int *bptr, const int *bcol,
+                       const T *bval, T eps, long long alpha, int beta, const int *worklist,
+                       int nwork, int *cursor, int mcap, int qcap, int ccap, T *dwork,
+                       int *iwork, SpaiColumnOut out, int *out_j, T *out_c,
+                       std::int64_t out_stride, int occupancy, std::int64_t pcap);
 template <typename T>
 void launch_scale_rows(hipStream_t s, int n, const int *ptr, const int *col, const T *val, T *d,
                        int *first_bad);

@@ -1476,22 +1476,42 @@ int spai_wave_max_n() {
 //     repeated sorted merges of the lane kernel, and the beta smallest
 //     (score, index) pairs are found by block-wide argmin rounds.
 // Per workgroup: T area  R[Q][M] | G[Q][M] | rhs[M] | v[M] | res[M] | score[C] | coef[Q] |
-//                         dot[Q+1]
-//                int area I[M] | J[Q] | Jt[Q] | cand[C] | clamp[C] | sel[Q] | bitmap[W] |
-//                         pos[n]
+//                         dot[Q+1] | vhead[Q] | pv[P]
+//                int area I[M] | J[Q] | Jt[Q] | cand[C] | clamp[C] | sel[Q] | Jf[Q] | vlen[Q] |
+//                         cnt[C+1] | pr[P] | wpre[W] | bitmap[W] | inI[W] | pos[n]
 // G keeps the gathered B(I,J) for the residual; pos maps a column index to its
-// position in I (-1 elsewhere), replacing binary searches.
+// position in I (-1 elsewhere), replacing binary searches, and the bitmap inI
+// marks the same rows: the scores test membership there first (62.5 KB per
+// workgroup at n = 500,000, against pos's 2 MB), so that only the few entries
+// of a candidate's row that lie in I read pos.
+//
+// FP32 (the mixed path's build) refits incrementally.  The reference refits
+// every growth from scratch (src/spai.cpp:134: least_squares on the whole
+// B(I,J)), which for the power-law matrices' columns (q in the hundreds, I in
+// the thousands) is O(m q^2) per fit and O(m q^3 / beta) per column.  Here a
+// row keeps the slot it got when it entered I (new rows are appended), a
+// column of B(I,J) keeps its position in the factorization (Jf: new columns
+// appended), and the Householder reflectors stay in R below the diagonal
+// (v_j[j] in vhead[j], support slots j..vlen[j]).  Because a new row of I is
+// zero in every old column (I is the union of the old columns' rows), the old
+// reflectors leave it alone: a growth applies the old reflectors to the new
+// columns only -- one wavefront per new column, all reflectors in order, no
+// workgroup barrier -- and factors the new columns.  The least-squares
+// problem, and so the coefficients, are the reference's up to FP32 rounding;
+// the summation order differs, which the FP32 build allows.
 constexpr int kBlk = 256;
 
+// pcap: (FP32) room for the (row, candidate) pairs of one fit
 struct BlockCaps {
     int mcap, qcap, ccap, words;
+    std::int64_t pcap;
 };
 
-std::int64_t spai_block_doubles(int M, int Q, int C) {
-    return 2LL * Q * M + 3LL * M + C + 2LL * Q + 1;
+std::int64_t spai_block_doubles(int M, int Q, int C, std::int64_t P) {
+    return 2LL * Q * M + 3LL * M + C + 3LL * Q + 1 + P;
 }
-std::int64_t spai_block_ints(int M, int Q, int C, int words, int n) {
-    return (std::int64_t)M + 3LL * Q + 2LL * C + words + n;
+std::int64_t spai_block_ints(int M, int Q, int C, std::int64_t P, int words, int n) {
+    return (std::int64_t)M + 5LL * Q + 3LL * C + 1 + P + 3LL * words + n;
 }
 
 // Block-wide helpers (blockDim.x == kBlk).
@@ -1554,6 +1574,42 @@ __device__ int bitmap_compact(unsigned *bm, int w0, int w1, int *out, int cap, i
     }
     __syncthreads();
     return total;
+}
+
+// As bitmap_compact, but the words keep their bits and wpre[w - w0] gets the
+// number of set bits in the words before w, so that the position of a set
+// bit x in out[] is wpre[(x >> 5) - w0] + popcount of the lower bits of its
+// word (a rank dictionary).  The caller clears the words (bitmap_clear).
+__device__ int bitmap_rank_list(const unsigned *bm, int w0, int w1, int *out, int cap, int *sh,
+                                int *wpre) {
+    const int tid = threadIdx.x;
+    const int nw = w1 >= w0 ? w1 - w0 + 1 : 0;
+    const int per = (nw + kBlk - 1) / kBlk;
+    const int a = w0 + tid * per, b = min(w0 + nw, a + per);
+    int cnt = 0;
+    for (int w = a; w < b; ++w)
+        cnt += __popc(bm[w]);
+    int total;
+    int pos = block_scan_excl(cnt, sh, total);
+    for (int w = a; w < b; ++w) {
+        wpre[w - w0] = pos;
+        unsigned bits = bm[w];
+        while (bits) {
+            const int bit = __ffs(bits) - 1;
+            bits &= bits - 1;
+            if (pos < cap)
+                out[pos] = w * 32 + bit;
+            ++pos;
+        }
+    }
+    __syncthreads();
+    return total;
+}
+
+__device__ void bitmap_clear(unsigned *bm, int w0, int w1) {
+    for (int w = w0 + (int)threadIdx.x; w <= w1; w += kBlk)
+        bm[w] = 0u;
+    __syncthreads();
 }
 
 // Marks column x; the caller keeps the word range [lo, hi] it touched and
@@ -1629,7 +1685,8 @@ template <typename T, int kOcc>
 __global__ void __launch_bounds__(kBlk) __attribute__((amdgpu_waves_per_eu(kOcc)))
 spai_block_kernel(int n, const int *__restrict__ aptr, const int *__restrict__ acol,
                   const T *__restrict__ aval, const T *__restrict__ dscale,
-                  const int *__restrict__ bptr, const int *__restrict__ bcol, T eps,
+                  const int *__restrict__ bptr, const int *__restrict__ bcol,
+                  const T *__restrict__ bval, T eps,
                   long long alpha, int beta, const int *__restrict__ worklist, int nwork,
                   int *__restrict__ cursor, BlockCaps caps, T *__restrict__ dwork,
                   int *__restrict__ iwork, std::int64_t blk_d, std::int64_t blk_i,
@@ -1642,15 +1699,24 @@ spai_block_kernel(int n, const int *__restrict__ aptr, const int *__restrict__ a
     __shared__ double s_dred[kBlk];
     __shared__ T s_scal[4];
     constexpr bool kExact = sizeof(T) == sizeof(double);
+    // (FP32) the back substitution's working vector, when q fits
+    constexpr int kYLds = kExact ? 1 : 2048;
+    __shared__ T s_y[kYLds];
     const int tid = threadIdx.x, wave = tid / kWave, lane = tid % kWave;
     const int M = caps.mcap, Q = caps.qcap, C = caps.ccap;
     T *D = dwork + (std::int64_t)blockIdx.x * blk_d;
     int *Iw = iwork + (std::int64_t)blockIdx.x * blk_i;
     T *R = D, *G = D + (std::int64_t)Q * M, *rhs = G + (std::int64_t)Q * M, *v = rhs + M,
-      *res = v + M, *sc = res + M, *coef = sc + C, *dot = coef + Q;
+      *res = v + M, *sc = res + M, *coef = sc + C, *dot = coef + Q, *vhead = dot + Q + 1;
+    T *pv = vhead + Q; // (FP32) values of the (row, candidate) pairs
     int *I = Iw, *J = I + M, *Jt = J + Q, *cand = Jt + Q, *clampf = cand + C, *sel = clampf + C;
-    unsigned *bm = reinterpret_cast<unsigned *>(sel + Q);
-    int *pos = reinterpret_cast<int *>(bm + caps.words); // all -1 between columns
+    int *Jf = sel + Q, *vlen = Jf + Q; // (FP32) factorization order, reflector supports
+    // (FP32) pairs per candidate (counts, then offsets), their rows, and the
+    // candidate bitmap's rank dictionary (set bits before each word)
+    int *cnt = vlen + Q, *pr = cnt + C + 1, *wpre = pr + caps.pcap;
+    unsigned *bm = reinterpret_cast<unsigned *>(wpre + caps.words);
+    unsigned *inI = bm + caps.words;                       // rows of I; clear between columns
+    int *pos = reinterpret_cast<int *>(inI + caps.words); // all -1 between columns
     int mapped = 0; // entries of I currently recorded in pos
 
     for (;;) {
@@ -1665,192 +1731,485 @@ spai_block_kernel(int n, const int *__restrict__ aptr, const int *__restrict__ a
         int status = kSpaiNotAttempted;
         int fits = 0, growths = 0, clamps = 0, q = 1, qdone = 0;
         double last_res = __builtin_nan(""), last_piv = __builtin_nan(""), qr_flops = 0;
-        if (tid == 0)
+        int mslots = 0, qf = 0;   // (FP32) slots of I and columns factored so far
+        T piv_min = T(INFINITY);  // (FP32) smallest |R_jj| so far
+        if (tid == 0) {
             J[0] = k;
+            Jf[0] = k;
+        }
         int *jout = out_j + (std::int64_t)slot * out_stride;
         T *cout_ = out_c + (std::int64_t)slot * out_stride;
         __syncthreads();
         for (;;) {
-            // ---- shadow I = union of cols(A row j), j in J (bitmap, ordered)
-            if (tid == 0) {
-                s_wlo = INT_MAX;
-                s_whi = -1;
-            }
-            __syncthreads();
-            {
-                int lo = INT_MAX, hi = -1;
-                for (int jj = wave; jj < q; jj += kBlk / kWave) {
-                    const int j = J[jj];
-                    for (int e = aptr[j] + lane; e < aptr[j + 1]; e += kWave)
-                        mark(bm, acol[e], lo, hi);
-                }
-                publish_range(lo, hi, &s_wlo, &s_whi);
-            }
-            __syncthreads();
-            for (int r = tid; r < mapped; r += kBlk) // the previous fit's shadow
-                pos[I[r]] = -1;
-            __syncthreads();
-            mapped = 0;
-            const int m = bitmap_compact(bm, s_wlo, s_whi, I, M, sh);
-            if (m > M) {
-                status = kSpaiOverflow;
-                break;
-            }
-            if (m < q) {
-                status = kSpaiRankDeficient;
-                break;
-            }
-            for (int r = tid; r < m; r += kBlk)
-                pos[I[r]] = r;
-            mapped = m;
-            __syncthreads();
-            // ---- gather R(i, c) = a_{J[c], I[i]} d_{J[c]}, rhs = e_k(I)
-            if (tid == 0)
-                s_pk = -1;
-            for (int c = wave; c < q; c += kBlk / kWave)
-                for (int r = lane; r < m; r += kWave) {
-                    R[(std::int64_t)c * M + r] = T(0);
-                    G[(std::int64_t)c * M + r] = T(0);
-                }
-            __syncthreads();
-            for (int r = tid; r < m; r += kBlk) {
-                rhs[r] = I[r] == k ? T(1) : T(0);
-                if (I[r] == k)
-                    s_pk = r;
-            }
-            for (int c = wave; c < q; c += kBlk / kWave) {
-                const int j = J[c];
-                const T dj = dscale[j];
-                for (int e = aptr[j] + lane; e < aptr[j + 1]; e += kWave) {
-                    const std::int64_t at = (std::int64_t)c * M + pos[acol[e]];
-                    const T x = aval[e] * dj;
-                    R[at] = x;
-                    G[at] = x;
-                }
-            }
-            __syncthreads();
-            const int pk = s_pk;
-            if (pk < 0) {
-                status = kSpaiNumericalFailure;
-                break;
-            }
-            // ---- Householder least squares (src/spai.cpp:30-98)
-            T rmax = 0;
-            for (int r = tid; r < m; r += kBlk) {
-                T sum = 0;
-                for (int c0 = 0; c0 < q; c0 += kBatch) {
-                    T xb[kBatch];
-#pragma unroll
-                    for (int u = 0; u < kBatch; ++u)
-                        xb[u] = R[(std::int64_t)min(c0 + u, q - 1) * M + r];
-#pragma unroll
-                    for (int u = 0; u < kBatch; ++u)
-                        if (c0 + u < q)
-                            sum += fabs(xb[u]);
-                }
-                rmax = sum > rmax ? sum : rmax;
-            }
-            s_red[tid] = rmax;
-            __syncthreads();
-            if (tid == 0) {
-                T mx = 0;
-                for (int t = 0; t < kBlk; ++t)
-                    mx = s_red[t] > mx ? s_red[t] : mx;
-                s_scal[3] = mx;
-            }
-            __syncthreads();
-            const T inf_norm = s_scal[3];
-            T min_pivot = T(INFINITY);
+            int m = 0, pk = -1;
+            T inf_norm = 0, min_pivot = T(INFINITY);
             int fit_status = kSpaiSuccess;
-            for (int j = 0; j < q; ++j) {
-                const int len = m - j;
-                T *colj = R + (std::int64_t)j * M + j;
-                T length;
-                if (kExact) { // the reference's hypot chain (norm2)
+            if constexpr (kExact) {
+                // ---- shadow I = union of cols(A row j), j in J (bitmap, ordered)
+                if (tid == 0) {
+                    s_wlo = INT_MAX;
+                    s_whi = -1;
+                }
+                __syncthreads();
+                {
+                    int lo = INT_MAX, hi = -1;
+                    for (int jj = wave; jj < q; jj += kBlk / kWave) {
+                        const int j = J[jj];
+                        for (int e = aptr[j] + lane; e < aptr[j + 1]; e += kWave)
+                            mark(bm, acol[e], lo, hi);
+                    }
+                    publish_range(lo, hi, &s_wlo, &s_whi);
+                }
+                __syncthreads();
+                for (int r = tid; r < mapped; r += kBlk) { // the previous fit's shadow
+                    pos[I[r]] = -1;
+                    atomicAnd(inI + (I[r] >> 5), ~(1u << (I[r] & 31)));
+                }
+                __syncthreads();
+                mapped = 0;
+                m = bitmap_compact(bm, s_wlo, s_whi, I, M, sh);
+                if (m > M) {
+                    status = kSpaiOverflow;
+                    break;
+                }
+                if (m < q) {
+                    status = kSpaiRankDeficient;
+                    break;
+                }
+                for (int r = tid; r < m; r += kBlk) {
+                    pos[I[r]] = r;
+                    atomicOr(inI + (I[r] >> 5), 1u << (I[r] & 31));
+                }
+                mapped = m;
+                __syncthreads();
+                // ---- gather R(i, c) = a_{J[c], I[i]} d_{J[c]}, rhs = e_k(I)
+                if (tid == 0)
+                    s_pk = -1;
+                for (int c = wave; c < q; c += kBlk / kWave)
+                    for (int r = lane; r < m; r += kWave) {
+                        R[(std::int64_t)c * M + r] = T(0);
+                        G[(std::int64_t)c * M + r] = T(0);
+                    }
+                __syncthreads();
+                for (int r = tid; r < m; r += kBlk) {
+                    rhs[r] = I[r] == k ? T(1) : T(0);
+                    if (I[r] == k)
+                        s_pk = r;
+                }
+                for (int c = wave; c < q; c += kBlk / kWave) {
+                    const int j = J[c];
+                    const T dj = dscale[j];
+                    for (int e = aptr[j] + lane; e < aptr[j + 1]; e += kWave) {
+                        const std::int64_t at = (std::int64_t)c * M + pos[acol[e]];
+                        const T x = aval[e] * dj;
+                        R[at] = x;
+                        G[at] = x;
+                    }
+                }
+                __syncthreads();
+                pk = s_pk;
+                if (pk < 0) {
+                    status = kSpaiNumericalFailure;
+                    break;
+                }
+                // ---- Householder least squares (src/spai.cpp:30-98)
+                T rmax = 0;
+                for (int r = tid; r < m; r += kBlk) {
+                    T sum = 0;
+                    for (int c0 = 0; c0 < q; c0 += kBatch) {
+                        T xb[kBatch];
+    #pragma unroll
+                        for (int u = 0; u < kBatch; ++u)
+                            xb[u] = R[(std::int64_t)min(c0 + u, q - 1) * M + r];
+    #pragma unroll
+                        for (int u = 0; u < kBatch; ++u)
+                            if (c0 + u < q)
+                                sum += fabs(xb[u]);
+                    }
+                    rmax = sum > rmax ? sum : rmax;
+                }
+                s_red[tid] = rmax;
+                __syncthreads();
+                if (tid == 0) {
+                    T mx = 0;
+                    for (int t = 0; t < kBlk; ++t)
+                        mx = s_red[t] > mx ? s_red[t] : mx;
+                    s_scal[3] = mx;
+                }
+                __syncthreads();
+                inf_norm = s_scal[3];
+                for (int j = 0; j < q; ++j) {
+                    const int len = m - j;
+                    T *colj = R + (std::int64_t)j * M + j;
+                    T length;
+                    if (kExact) { // the reference's hypot chain (norm2)
+                        for (int r = tid; r < len; r += kBlk)
+                            v[r] = colj[r];
+                        __syncthreads();
+                        if (tid == 0)
+                            s_scal[0] = hypot_chain(v, len);
+                        __syncthreads();
+                        length = s_scal[0];
+                    } else { // FP32: sum of squares in FP64, fixed-order tree
+                        double acc = 0;
+                        for (int r = tid; r < len; r += kBlk) {
+                            const T x = colj[r];
+                            v[r] = x;
+                            acc += (double)x * (double)x;
+                        }
+                        length = (T)sqrt(block_sum_fixed(acc, s_dred));
+                    }
+                    if (!isfinite(length)) {
+                        fit_status = kSpaiNumericalFailure;
+                        break;
+                    }
+                    if (length == 0) {
+                        fit_status = kSpaiRankDeficient;
+                        break;
+                    }
+                    const T diagonal = -copysign(length, v[0]);
+                    __syncthreads(); // v[0] read before it is rescaled
+                    for (int r = tid; r < len; r += kBlk) {
+                        T x = v[r] / length;
+                        if (r == 0)
+                            x += copysign(T(1), x);
+                        v[r] = x;
+                    }
+                    __syncthreads();
+                    T vnorm;
+                    if (kExact) {
+                        if (tid == 0)
+                            s_scal[1] = hypot_chain(v, len);
+                        __syncthreads();
+                        vnorm = s_scal[1];
+                    } else {
+                        double acc = 0;
+                        for (int r = tid; r < len; r += kBlk)
+                            acc += (double)v[r] * (double)v[r];
+                        vnorm = (T)sqrt(block_sum_fixed(acc, s_dred));
+                    }
                     for (int r = tid; r < len; r += kBlk)
-                        v[r] = colj[r];
+                        v[r] = v[r] / vnorm;
                     __syncthreads();
+                    // slots j+1..q (q = right side)
+                    if (kExact) {
+                        // one thread per slot, rows in order; loads in batches
+                        for (int c = j + 1 + tid; c <= q; c += kBlk) {
+                            const T *x = (c < q ? R + (std::int64_t)c * M : rhs) + j;
+                            T d = 0;
+                            for (int r0 = 0; r0 < len; r0 += kBatch) {
+                                T vb[kBatch], xb[kBatch];
+    #pragma unroll
+                                for (int u = 0; u < kBatch; ++u) {
+                                    const int r = min(r0 + u, len - 1);
+                                    vb[u] = v[r];
+                                    xb[u] = x[r];
+                                }
+    #pragma unroll
+                                for (int u = 0; u < kBatch; ++u)
+                                    if (r0 + u < len)
+                                        d += vb[u] * xb[u];
+                            }
+                            dot[c] = d;
+                        }
+                    } else {
+                        // FP32: one wave per slot, lanes over rows (loads batched),
+                        // FP64 partials combined by a fixed butterfly
+                        for (int c = j + 1 + wave; c <= q; c += kBlk / kWave) {
+                            const T *x = (c < q ? R + (std::int64_t)c * M : rhs) + j;
+                            double d = 0;
+                            for (int r0 = lane; r0 < len; r0 += kWave * kRowB) {
+                                T vb[kRowB], xb[kRowB];
+    #pragma unroll
+                                for (int u = 0; u < kRowB; ++u) {
+                                    const int r = min(r0 + u * kWave, len - 1);
+                                    vb[u] = v[r];
+                                    xb[u] = x[r];
+                                }
+    #pragma unroll
+                                for (int u = 0; u < kRowB; ++u)
+                                    if (r0 + u * kWave < len)
+                                        d += (double)vb[u] * (double)xb[u];
+                            }
+                            d = wave_sum(d);
+                            if (lane == 0)
+                                dot[c] = (T)d;
+                        }
+                    }
                     if (tid == 0)
-                        s_scal[0] = hypot_chain(v, len);
+                        s_flag = 0;
                     __syncthreads();
-                    length = s_scal[0];
-                } else { // FP32: sum of squares in FP64, fixed-order tree
+                    bool bad = false;
+                    for (int c = j + 1 + wave; c <= q; c += kBlk / kWave) {
+                        T *x = (c < q ? R + (std::int64_t)c * M : rhs) + j;
+                        const T dc = dot[c];
+                        for (int r0 = lane; r0 < len; r0 += kWave * kRowB) {
+                            T vb[kRowB], xb[kRowB];
+    #pragma unroll
+                            for (int u = 0; u < kRowB; ++u) {
+                                const int r = min(r0 + u * kWave, len - 1);
+                                vb[u] = v[r];
+                                xb[u] = x[r];
+                            }
+    #pragma unroll
+                            for (int u = 0; u < kRowB; ++u)
+                                if (r0 + u * kWave < len) {
+                                    const T y = xb[u] - T(2) * vb[u] * dc;
+                                    x[r0 + u * kWave] = y;
+                                    bad = bad || !isfinite(y);
+                                }
+                        }
+                    }
+                    if (bad)
+                        s_flag = 1;
+                    __syncthreads();
+                    if (s_flag) {
+                        fit_status = kSpaiNumericalFailure;
+                        break;
+                    }
+                    if (tid == 0)
+                        colj[0] = diagonal;
+                    const T ad = fabs(diagonal);
+                    min_pivot = ad < min_pivot ? ad : min_pivot;
+                    __syncthreads();
+                }
+                if (fit_status == kSpaiSuccess) {
+                    if (tid == 0) {
+                        int st = kSpaiSuccess;
+                        for (int j = q - 1; j >= 0; --j) {
+                            T value = rhs[j];
+                            for (int c0 = j + 1; c0 < q; c0 += kBatch) {
+                                T rb[kBatch], cb[kBatch];
+    #pragma unroll
+                                for (int u = 0; u < kBatch; ++u) {
+                                    const int c = min(c0 + u, q - 1);
+                                    rb[u] = R[(std::int64_t)c * M + j];
+                                    cb[u] = coef[c];
+                                }
+    #pragma unroll
+                                for (int u = 0; u < kBatch; ++u)
+                                    if (c0 + u < q)
+                                        value -= rb[u] * cb[u];
+                            }
+                            const T dg = R[(std::int64_t)j * M + j];
+                            if (!isfinite(dg)) {
+                                st = kSpaiNumericalFailure;
+                                break;
+                            }
+                            if (dg == 0) {
+                                st = kSpaiRankDeficient;
+                                break;
+                            }
+                            const T sj = value / dg;
+                            if (!isfinite(sj)) {
+                                st = kSpaiNumericalFailure;
+                                break;
+                            }
+                            coef[j] = sj;
+                        }
+                        s_fail = st;
+                    }
+                    __syncthreads();
+                    fit_status = s_fail;
+                }
+                if (fit_status != kSpaiSuccess) {
+                    status = fit_status;
+                    break;
+                }
+            } else {
+                // ---- (FP32) incremental fit: see the layout notes above.  New
+                // rows: those of the columns added since the last fit that have
+                // no slot yet, appended in ascending order.
+                if (tid == 0) {
+                    s_wlo = INT_MAX;
+                    s_whi = -1;
+                }
+                __syncthreads();
+                {
+                    int lo = INT_MAX, hi = -1;
+                    for (int jj = qf + wave; jj < q; jj += kBlk / kWave) {
+                        const int j = Jf[jj];
+                        for (int e = aptr[j] + lane; e < aptr[j + 1]; e += kWave) {
+                            const int row = acol[e];
+                            if (pos[row] < 0)
+                                mark(bm, row, lo, hi);
+                        }
+                    }
+                    publish_range(lo, hi, &s_wlo, &s_whi);
+                }
+                __syncthreads();
+                const int mold = mslots;
+                m = mold + bitmap_compact(bm, s_wlo, s_whi, I + mold, M - mold, sh);
+                if (m > M) {
+                    status = kSpaiOverflow;
+                    break;
+                }
+                if (m < q) {
+                    status = kSpaiRankDeficient;
+                    break;
+                }
+                for (int r = mold + tid; r < m; r += kBlk) {
+                    pos[I[r]] = r;
+                    atomicOr(inI + (I[r] >> 5), 1u << (I[r] & 31));
+                }
+                mapped = m;
+                mslots = m;
+                // Old columns are zero at the new slots; new columns are gathered
+                // whole; the transformed right side is e_k at the new slots (the
+                // old reflectors do not reach them).
+                for (int c = wave; c < q; c += kBlk / kWave)
+                    for (int r = (c < qf ? mold : 0) + lane; r < m; r += kWave) {
+                        R[(std::int64_t)c * M + r] = T(0);
+                        G[(std::int64_t)c * M + r] = T(0);
+                    }
+                for (int r = mold + tid; r < m; r += kBlk)
+                    rhs[r] = I[r] == k ? T(1) : T(0);
+                __syncthreads();
+                for (int c = qf + wave; c < q; c += kBlk / kWave) {
+                    const int j = Jf[c];
+                    const T dj = dscale[j];
+                    for (int e = aptr[j] + lane; e < aptr[j + 1]; e += kWave) {
+                        const std::int64_t at = (std::int64_t)c * M + pos[acol[e]];
+                        const T x = aval[e] * dj;
+                        R[at] = x;
+                        G[at] = x;
+                    }
+                }
+                __syncthreads();
+                pk = pos[k];
+                if (pk < 0) {
+                    status = kSpaiNumericalFailure;
+                    break;
+                }
+                // ||B(I,J)||_inf from the gathered copy (every column, every slot)
+                {
+                    T rmax = 0;
+                    for (int r = tid; r < m; r += kBlk) {
+                        T sum = 0;
+                        for (int c0 = 0; c0 < q; c0 += kBatch) {
+                            T xb[kBatch];
+    #pragma unroll
+                            for (int u = 0; u < kBatch; ++u)
+                                xb[u] = G[(std::int64_t)min(c0 + u, q - 1) * M + r];
+    #pragma unroll
+                            for (int u = 0; u < kBatch; ++u)
+                                if (c0 + u < q)
+                                    sum += fabs(xb[u]);
+                        }
+                        rmax = sum > rmax ? sum : rmax;
+                    }
+                    s_red[tid] = rmax;
+                    __syncthreads();
+                    if (tid == 0) {
+                        T mx = 0;
+                        for (int t = 0; t < kBlk; ++t)
+                            mx = s_red[t] > mx ? s_red[t] : mx;
+                        s_scal[3] = mx;
+                    }
+                    __syncthreads();
+                    inf_norm = s_scal[3];
+                }
+                // The old reflectors on the new columns: one wave per column,
+                // reflectors in order (v_j^T x in FP64, fixed butterfly).
+                {
+                    bool bad = false;
+                    for (int c = qf + wave; c < q; c += kBlk / kWave) {
+                        T *x = R + (std::int64_t)c * M;
+                        for (int j = 0; j < qf; ++j) {
+                            const int len = vlen[j];
+                            const T *vj = R + (std::int64_t)j * M;
+                            const T h0 = vhead[j];
+                            double d = 0;
+                            for (int r0 = j + lane; r0 < len; r0 += kWave * kRowB) {
+                                T vb[kRowB], xb[kRowB];
+    #pragma unroll
+                                for (int u = 0; u < kRowB; ++u) {
+                                    const int r = min(r0 + u * kWave, len - 1);
+                                    vb[u] = r == j ? h0 : vj[r];
+                                    xb[u] = x[r];
+                                }
+    #pragma unroll
+                                for (int u = 0; u < kRowB; ++u)
+                                    if (r0 + u * kWave < len)
+                                        d += (double)vb[u] * (double)xb[u];
+                            }
+                            const T dc = (T)wave_sum(d);
+                            for (int r0 = j + lane; r0 < len; r0 += kWave * kRowB) {
+                                T vb[kRowB], xb[kRowB];
+    #pragma unroll
+                                for (int u = 0; u < kRowB; ++u) {
+                                    const int r = min(r0 + u * kWave, len - 1);
+                                    vb[u] = r == j ? h0 : vj[r];
+                                    xb[u] = x[r];
+                                }
+    #pragma unroll
+                                for (int u = 0; u < kRowB; ++u)
+                                    if (r0 + u * kWave < len) {
+                                        const T y = xb[u] - T(2) * vb[u] * dc;
+                                        x[r0 + u * kWave] = y;
+                                        bad = bad || !isfinite(y);
+                                    }
+                            }
+                            wave_sync(); // x as rewritten, before the next dot
+                        }
+                    }
+                    if (tid == 0)
+                        s_flag = 0;
+                    __syncthreads();
+                    if (bad)
+                        s_flag = 1;
+                    __syncthreads();
+                    if (s_flag) {
+                        status = kSpaiNumericalFailure;
+                        break;
+                    }
+                }
+                // Factor the new columns j = qf .. q-1 over slots j .. m-1; the
+                // trailing updates reach the new columns and the right side only.
+                for (int j = qf; j < q; ++j) {
+                    const int len = m - j;
+                    T *colj = R + (std::int64_t)j * M + j;
                     double acc = 0;
                     for (int r = tid; r < len; r += kBlk) {
                         const T x = colj[r];
                         v[r] = x;
                         acc += (double)x * (double)x;
                     }
-                    length = (T)sqrt(block_sum_fixed(acc, s_dred));
-                }
-                if (!isfinite(length)) {
-                    fit_status = kSpaiNumericalFailure;
-                    break;
-                }
-                if (length == 0) {
-                    fit_status = kSpaiRankDeficient;
-                    break;
-                }
-                const T diagonal = -copysign(length, v[0]);
-                __syncthreads(); // v[0] read before it is rescaled
-                for (int r = tid; r < len; r += kBlk) {
-                    T x = v[r] / length;
-                    if (r == 0)
-                        x += copysign(T(1), x);
-                    v[r] = x;
-                }
-                __syncthreads();
-                T vnorm;
-                if (kExact) {
-                    if (tid == 0)
-                        s_scal[1] = hypot_chain(v, len);
-                    __syncthreads();
-                    vnorm = s_scal[1];
-                } else {
-                    double acc = 0;
-                    for (int r = tid; r < len; r += kBlk)
-                        acc += (double)v[r] * (double)v[r];
-                    vnorm = (T)sqrt(block_sum_fixed(acc, s_dred));
-                }
-                for (int r = tid; r < len; r += kBlk)
-                    v[r] = v[r] / vnorm;
-                __syncthreads();
-                // slots j+1..q (q = right side)
-                if (kExact) {
-                    // one thread per slot, rows in order; loads in batches
-                    for (int c = j + 1 + tid; c <= q; c += kBlk) {
-                        const T *x = (c < q ? R + (std::int64_t)c * M : rhs) + j;
-                        T d = 0;
-                        for (int r0 = 0; r0 < len; r0 += kBatch) {
-                            T vb[kBatch], xb[kBatch];
-#pragma unroll
-                            for (int u = 0; u < kBatch; ++u) {
-                                const int r = min(r0 + u, len - 1);
-                                vb[u] = v[r];
-                                xb[u] = x[r];
-                            }
-#pragma unroll
-                            for (int u = 0; u < kBatch; ++u)
-                                if (r0 + u < len)
-                                    d += vb[u] * xb[u];
-                        }
-                        dot[c] = d;
+                    const T length = (T)sqrt(block_sum_fixed(acc, s_dred));
+                    if (!isfinite(length)) {
+                        fit_status = kSpaiNumericalFailure;
+                        break;
                     }
-                } else {
-                    // FP32: one wave per slot, lanes over rows (loads batched),
-                    // FP64 partials combined by a fixed butterfly
+                    if (length == 0) {
+                        fit_status = kSpaiRankDeficient;
+                        break;
+                    }
+                    const T diagonal = -copysign(length, v[0]);
+                    __syncthreads(); // v[0] read before it is rescaled
+                    double acc2 = 0;
+                    for (int r = tid; r < len; r += kBlk) {
+                        T x = v[r] / length;
+                        if (r == 0)
+                            x += copysign(T(1), x);
+                        v[r] = x;
+                        acc2 += (double)x * (double)x;
+                    }
+                    const T vnorm = (T)sqrt(block_sum_fixed(acc2, s_dred));
+                    for (int r = tid; r < len; r += kBlk)
+                        v[r] = v[r] / vnorm;
+                    __syncthreads();
                     for (int c = j + 1 + wave; c <= q; c += kBlk / kWave) {
                         const T *x = (c < q ? R + (std::int64_t)c * M : rhs) + j;
                         double d = 0;
                         for (int r0 = lane; r0 < len; r0 += kWave * kRowB) {
                             T vb[kRowB], xb[kRowB];
-#pragma unroll
+    #pragma unroll
                             for (int u = 0; u < kRowB; ++u) {
                                 const int r = min(r0 + u * kWave, len - 1);
                                 vb[u] = v[r];
                                 xb[u] = x[r];
                             }
-#pragma unroll
+    #pragma unroll
                             for (int u = 0; u < kRowB; ++u)
                                 if (r0 + u * kWave < len)
                                     d += (double)vb[u] * (double)xb[u];
@@ -1859,86 +2218,97 @@ spai_block_kernel(int n, const int *__restrict__ aptr, const int *__restrict__ a
                         if (lane == 0)
                             dot[c] = (T)d;
                     }
-                }
-                if (tid == 0)
-                    s_flag = 0;
-                __syncthreads();
-                bool bad = false;
-                for (int c = j + 1 + wave; c <= q; c += kBlk / kWave) {
-                    T *x = (c < q ? R + (std::int64_t)c * M : rhs) + j;
-                    const T dc = dot[c];
-                    for (int r0 = lane; r0 < len; r0 += kWave * kRowB) {
-                        T vb[kRowB], xb[kRowB];
-#pragma unroll
-                        for (int u = 0; u < kRowB; ++u) {
-                            const int r = min(r0 + u * kWave, len - 1);
-                            vb[u] = v[r];
-                            xb[u] = x[r];
-                        }
-#pragma unroll
-                        for (int u = 0; u < kRowB; ++u)
-                            if (r0 + u * kWave < len) {
-                                const T y = xb[u] - T(2) * vb[u] * dc;
-                                x[r0 + u * kWave] = y;
-                                bad = bad || !isfinite(y);
+                    if (tid == 0)
+                        s_flag = 0;
+                    __syncthreads();
+                    bool bad = false;
+                    for (int c = j + 1 + wave; c <= q; c += kBlk / kWave) {
+                        T *x = (c < q ? R + (std::int64_t)c * M : rhs) + j;
+                        const T dc = dot[c];
+                        for (int r0 = lane; r0 < len; r0 += kWave * kRowB) {
+                            T vb[kRowB], xb[kRowB];
+    #pragma unroll
+                            for (int u = 0; u < kRowB; ++u) {
+                                const int r = min(r0 + u * kWave, len - 1);
+                                vb[u] = v[r];
+                                xb[u] = x[r];
                             }
+    #pragma unroll
+                            for (int u = 0; u < kRowB; ++u)
+                                if (r0 + u * kWave < len) {
+                                    const T y = xb[u] - T(2) * vb[u] * dc;
+                                    x[r0 + u * kWave] = y;
+                                    bad = bad || !isfinite(y);
+                                }
+                        }
                     }
+                    if (bad)
+                        s_flag = 1;
+                    // keep the reflector: v_j[0] aside, the rest below the diagonal
+                    for (int r = 1 + tid; r < len; r += kBlk)
+                        colj[r] = v[r];
+                    __syncthreads();
+                    if (s_flag) {
+                        fit_status = kSpaiNumericalFailure;
+                        break;
+                    }
+                    if (tid == 0) {
+                        colj[0] = diagonal;
+                        vhead[j] = v[0];
+                        vlen[j] = m;
+                    }
+                    const T ad = fabs(diagonal);
+                    piv_min = ad < piv_min ? ad : piv_min;
+                    __syncthreads();
                 }
-                if (bad)
-                    s_flag = 1;
-                __syncthreads();
-                if (s_flag) {
-                    fit_status = kSpaiNumericalFailure;
+                if (fit_status == kSpaiSuccess)
+                    qf = q;
+                min_pivot = piv_min;
+                // Back substitution by one wave, column by column (R is stored by
+                // columns): y_j = y_j / R_jj, then y_i -= R_ij y_j for i < j.
+                if (fit_status == kSpaiSuccess) {
+                    if (wave == 0) {
+                        T *y = q <= kYLds ? s_y : coef;
+                        for (int r = lane; r < q; r += kWave)
+                            y[r] = rhs[r];
+                        wave_sync();
+                        int st = kSpaiSuccess;
+                        for (int j = q - 1; j >= 0; --j) {
+                            const T *cj = R + (std::int64_t)j * M;
+                            const T dg = cj[j];
+                            if (!isfinite(dg)) {
+                                st = kSpaiNumericalFailure;
+                                break;
+                            }
+                            if (dg == 0) {
+                                st = kSpaiRankDeficient;
+                                break;
+                            }
+                            const T yj = y[j] / dg;
+                            if (!isfinite(yj)) {
+                                st = kSpaiNumericalFailure;
+                                break;
+                            }
+                            for (int i = lane; i < j; i += kWave)
+                                y[i] = y[i] - cj[i] * yj;
+                            wave_sync();
+                            if (lane == 0)
+                                y[j] = yj;
+                            wave_sync();
+                        }
+                        if (y != coef)
+                            for (int r = lane; r < q; r += kWave)
+                                coef[r] = y[r];
+                        if (lane == 0)
+                            s_fail = st;
+                    }
+                    __syncthreads();
+                    fit_status = s_fail;
+                }
+                if (fit_status != kSpaiSuccess) {
+                    status = fit_status;
                     break;
                 }
-                if (tid == 0)
-                    colj[0] = diagonal;
-                const T ad = fabs(diagonal);
-                min_pivot = ad < min_pivot ? ad : min_pivot;
-                __syncthreads();
-            }
-            if (fit_status == kSpaiSuccess) {
-                if (tid == 0) {
-                    int st = kSpaiSuccess;
-                    for (int j = q - 1; j >= 0; --j) {
-                        T value = rhs[j];
-                        for (int c0 = j + 1; c0 < q; c0 += kBatch) {
-                            T rb[kBatch], cb[kBatch];
-#pragma unroll
-                            for (int u = 0; u < kBatch; ++u) {
-                                const int c = min(c0 + u, q - 1);
-                                rb[u] = R[(std::int64_t)c * M + j];
-                                cb[u] = coef[c];
-                            }
-#pragma unroll
-                            for (int u = 0; u < kBatch; ++u)
-                                if (c0 + u < q)
-                                    value -= rb[u] * cb[u];
-                        }
-                        const T dg = R[(std::int64_t)j * M + j];
-                        if (!isfinite(dg)) {
-                            st = kSpaiNumericalFailure;
-                            break;
-                        }
-                        if (dg == 0) {
-                            st = kSpaiRankDeficient;
-                            break;
-                        }
-                        const T sj = value / dg;
-                        if (!isfinite(sj)) {
-                            st = kSpaiNumericalFailure;
-                            break;
-                        }
-                        coef[j] = sj;
-                    }
-                    s_fail = st;
-                }
-                __syncthreads();
-                fit_status = s_fail;
-            }
-            if (fit_status != kSpaiSuccess) {
-                status = fit_status;
-                break;
             }
             const T pivot_ratio = inf_norm == 0 ? T(0) : min_pivot / inf_norm;
             // ---- fitted residual s = B(I,J) m - e_k(I): per row, the stored
@@ -1995,9 +2365,17 @@ spai_block_kernel(int n, const int *__restrict__ aptr, const int *__restrict__ a
                 const double fm = m, fq = q;
                 qr_flops += 2 * fm * fq * fq - 2 * fq * fq * fq / 3 + 4 * fm * fq + fq * fq;
             }
-            for (int c = tid; c < q; c += kBlk) {
-                jout[c] = J[c];
-                cout_[c] = coef[c];
+            if constexpr (kExact) {
+                for (int c = tid; c < q; c += kBlk) {
+                    jout[c] = J[c];
+                    cout_[c] = coef[c];
+                }
+            } else { // (FP32) coef follows Jf; the pattern is written sorted
+                for (int c = tid; c < q; c += kBlk) {
+                    const int at = lower_bound(J, q, Jf[c]);
+                    jout[at] = Jf[c];
+                    cout_[at] = coef[c];
+                }
             }
             qdone = q;
             last_res = (double)rnorm;
@@ -2040,13 +2418,13 @@ spai_block_kernel(int n, const int *__restrict__ aptr, const int *__restrict__ a
             for (int jj = tid; jj < q; jj += kBlk)
                 atomicAnd(bm + (J[jj] >> 5), ~(1u << (J[jj] & 31)));
             __syncthreads();
-            const int c = bitmap_compact(bm, s_wlo, s_whi, cand, C, sh);
-            if (c > C) {
-                status = kSpaiOverflow;
-                break;
-            }
-            if (c == 0) {
-                status = kSpaiEmptyCandidates;
+            const int cw0 = s_wlo, cw1 = s_whi;
+            const int c = kExact ? bitmap_compact(bm, cw0, cw1, cand, C, sh)
+                                 : bitmap_rank_list(bm, cw0, cw1, cand, C, sh, wpre);
+            if (c > C || c == 0) {
+                if (!kExact)
+                    bitmap_clear(bm, cw0, cw1);
+                status = c > C ? kSpaiOverflow : kSpaiEmptyCandidates;
                 break;
             }
             // ---- restricted scores (src/spai.cpp:198-215), one thread per
@@ -2055,37 +2433,137 @@ spai_block_kernel(int n, const int *__restrict__ aptr, const int *__restrict__ a
             if (tid == 0)
                 s_fail = INT_MAX;
             __syncthreads();
-            for (int t = tid; t < c; t += kBlk) {
-                const int j = cand[t];
-                const T dj = dscale[j];
-                T dt = 0, den = 0;
-                const int e1 = aptr[j + 1];
-                for (int e0 = aptr[j]; e0 < e1; e0 += kRowB) {
-                    int pb[kRowB];
-                    T ab[kRowB], rb[kRowB];
-#pragma unroll
-                    for (int u = 0; u < kRowB; ++u)
-                        pb[u] = pos[acol[min(e0 + u, e1 - 1)]];
-#pragma unroll
-                    for (int u = 0; u < kRowB; ++u) {
-                        ab[u] = aval[min(e0 + u, e1 - 1)];
-                        rb[u] = res[max(pb[u], 0)];
-                    }
-#pragma unroll
-                    for (int u = 0; u < kRowB; ++u)
-                        if (e0 + u < e1 && pb[u] >= 0) {
-                            const T value = ab[u] * dj;
-                            dt += rb[u] * value;
-                            den += value * value;
+            if constexpr (kExact) {
+                for (int t = tid; t < c; t += kBlk) {
+                    const int j = cand[t];
+                    const T dj = dscale[j];
+                    T dt = 0, den = 0;
+                    const int e1 = aptr[j + 1];
+                    for (int e0 = aptr[j]; e0 < e1; e0 += kRowB) {
+                        int pb[kRowB];
+                        T ab[kRowB], rb[kRowB];
+    #pragma unroll
+                        for (int u = 0; u < kRowB; ++u)
+                            pb[u] = acol[min(e0 + u, e1 - 1)];
+    #pragma unroll
+                        for (int u = 0; u < kRowB; ++u)
+                            pb[u] = (inI[pb[u] >> 5] >> (pb[u] & 31)) & 1u ? pos[pb[u]] : -1;
+    #pragma unroll
+                        for (int u = 0; u < kRowB; ++u) {
+                            ab[u] = aval[min(e0 + u, e1 - 1)];
+                            rb[u] = res[max(pb[u], 0)];
                         }
+    #pragma unroll
+                        for (int u = 0; u < kRowB; ++u)
+                            if (e0 + u < e1 && pb[u] >= 0) {
+                                const T value = ab[u] * dj;
+                                dt += rb[u] * value;
+                                den += value * value;
+                            }
+                    }
+                    T sv = 0;
+                    bool cl = false;
+                    const int st = score_of(rr, dt, den, sv, cl);
+                    sc[t] = sv;
+                    clampf[t] = st == kSpaiSuccess ? (cl ? 1 : 0) : -st;
+                    if (st != kSpaiSuccess)
+                        atomicMin(&s_fail, t);
                 }
-                T sv = 0;
-                bool cl = false;
-                const int st = score_of(rr, dt, den, sv, cl);
-                sc[t] = sv;
-                clampf[t] = st == kSpaiSuccess ? (cl ? 1 : 0) : -st;
-                if (st != kSpaiSuccess)
-                    atomicMin(&s_fail, t);
+            } else {
+                // (FP32) Scores from the side of I.  A candidate's dot
+                // products run over its row of A restricted to I, and on the
+                // power-law systems that row is long (31 entries on average
+                // at n = 200,000) while only one or two of its entries lie in
+                // I.  So the pairs are found from I instead: the entries of
+                // B's rows i in I whose column j is a candidate (j's position
+                // in cand[] from the bitmap's rank dictionary), bucketed per
+                // candidate by a counting sort, each bucket ordered by slot,
+                // then summed in that order.
+                const std::int64_t kPairCap = caps.pcap;
+                auto in_cand = [&](int j) { return (bm[j >> 5] >> (j & 31)) & 1u; };
+                auto rank = [&](int j) {
+                    return wpre[(j >> 5) - cw0] + __popc(bm[j >> 5] & ((1u << (j & 31)) - 1u));
+                };
+                for (int t = tid; t <= c; t += kBlk)
+                    cnt[t] = 0;
+                __syncthreads();
+                for (int r = tid; r < m; r += kBlk) {
+                    const int i = I[r];
+                    for (int e = bptr[i]; e < bptr[i + 1]; ++e) {
+                        const int j = bcol[e];
+                        if (in_cand(j))
+                            atomicAdd(cnt + rank(j), 1);
+                    }
+                }
+                __syncthreads();
+                // exclusive scan of the counts (each thread a run of candidates)
+                {
+                    const int per = (c + kBlk - 1) / kBlk;
+                    const int a = tid * per, b = min(c, a + per);
+                    int sum = 0;
+                    for (int t = a; t < b; ++t)
+                        sum += cnt[t];
+                    int total;
+                    int at = block_scan_excl(sum, sh, total);
+                    __syncthreads();
+                    for (int t = a; t < b; ++t) {
+                        const int x = cnt[t];
+                        cnt[t] = at;
+                        clampf[t] = at; // fill cursor
+                        at += x;
+                    }
+                    if (tid == 0)
+                        cnt[c] = total;
+                    __syncthreads();
+                    if (total > kPairCap) {
+                        bitmap_clear(bm, cw0, cw1);
+                        status = kSpaiOverflow;
+                        break;
+                    }
+                }
+                for (int r = tid; r < m; r += kBlk) {
+                    const int i = I[r];
+                    for (int e = bptr[i]; e < bptr[i + 1]; ++e) {
+                        const int j = bcol[e];
+                        if (in_cand(j)) {
+                            const int at = atomicAdd(clampf + rank(j), 1);
+                            pr[at] = r;
+                            pv[at] = bval[e];
+                        }
+                    }
+                }
+                __syncthreads();
+                bitmap_clear(bm, cw0, cw1);
+                for (int t = tid; t < c; t += kBlk) {
+                    const int j = cand[t];
+                    const T dj = dscale[j];
+                    const int p0 = cnt[t], p1 = cnt[t + 1];
+                    for (int a = p0 + 1; a < p1; ++a) { // by slot (few pairs each)
+                        const int rr0 = pr[a];
+                        const T vv0 = pv[a];
+                        int b = a - 1;
+                        while (b >= p0 && pr[b] > rr0) {
+                            pr[b + 1] = pr[b];
+                            pv[b + 1] = pv[b];
+                            --b;
+                        }
+                        pr[b + 1] = rr0;
+                        pv[b + 1] = vv0;
+                    }
+                    T dt = 0, den = 0;
+                    for (int a = p0; a < p1; ++a) {
+                        const T value = pv[a] * dj;
+                        dt += res[pr[a]] * value;
+                        den += value * value;
+                    }
+                    T sv = 0;
+                    bool cl = false;
+                    const int st = score_of(rr, dt, den, sv, cl);
+                    sc[t] = sv;
+                    clampf[t] = st == kSpaiSuccess ? (cl ? 1 : 0) : -st;
+                    if (st != kSpaiSuccess)
+                        atomicMin(&s_fail, t);
+                }
             }
             __syncthreads();
             const int first_bad = s_fail;
@@ -2225,13 +2703,18 @@ spai_block_kernel(int n, const int *__restrict__ aptr, const int *__restrict__ a
                 }
                 for (int r = 0; r < t; ++r)
                     J[r] = Jt[r];
+                if (!kExact)
+                    for (int r = 0; r < nsel; ++r)
+                        Jf[q + r] = sel[r];
             }
             q += nsel;
             __syncthreads();
         }
         __syncthreads();
-        for (int r = tid; r < mapped; r += kBlk)
+        for (int r = tid; r < mapped; r += kBlk) {
             pos[I[r]] = -1;
+            atomicAnd(inI + (I[r] >> 5), ~(1u << (I[r] & 31)));
+        }
         mapped = 0;
         __syncthreads();
         if (tid == 0) {
@@ -2252,17 +2735,19 @@ spai_block_kernel(int n, const int *__restrict__ aptr, const int *__restrict__ a
 
 template <typename T>
 void launch_spai_block(hipStream_t s, int blocks, int n, const int *aptr, const int *acol,
-                       const T *aval, const T *dscale, const int *bptr, const int *bcol, T eps,
-                       long long alpha, int beta, const int *worklist, int nwork, int *cursor,
-                       int mcap, int qcap, int ccap, T *dwork, int *iwork, SpaiColumnOut out,
-                       int *out_j, T *out_c, std::int64_t out_stride, int occupancy) {
+                       const T *aval, const T *dscale, const int *bptr, const int *bcol,
+                       const T *bval, T eps, long long alpha, int beta, const int *worklist,
+                       int nwork, int *cursor, int mcap, int qcap, int ccap, T *dwork,
+                       int *iwork, SpaiColumnOut out, int *out_j, T *out_c,
+                       std::int64_t out_stride, int occupancy, std::int64_t pcap) {
     const int words = (n + 31) / 32;
-    BlockCaps caps{mcap, qcap, ccap, words};
+    BlockCaps caps{mcap, qcap, ccap, words, pcap};
     auto kernel = occupancy >= 4 ? spai_block_kernel<T, 4> : spai_block_kernel<T, 3>;
     hipLaunchKernelGGL(kernel, dim3(blocks), dim3(kBlk), 0, s, n, aptr, acol, aval,
-                       dscale, bptr, bcol, eps, alpha, beta, worklist, nwork, cursor, caps, dwork,
-                       iwork, spai_block_doubles(mcap, qcap, ccap),
-                       spai_block_ints(mcap, qcap, ccap, words, n), out, out_j, out_c,
+                       dscale, bptr, bcol, bval, eps, alpha, beta, worklist, nwork, cursor, caps,
+                       dwork,
+                       iwork, spai_block_doubles(mcap, qcap, ccap, pcap),
+                       spai_block_ints(mcap, qcap, ccap, pcap, words, n), out, out_j, out_c,
                        out_stride);
     HIP_LAUNCH_CHECK();
 }
@@ -2285,9 +2770,11 @@ void launch_fill_int(hipStream_t s, std::int64_t blocks, std::int64_t stride, st
 
 #define BSPAI_BLOCK_INST(T)                                                                       \
     template void launch_spai_block<T>(hipStream_t, int, int, const int *, const int *,          \
-                                       const T *, const T *, const int *, const int *, T,        \
+                                       const T *, const T *, const int *, const int *, const T *, \
+                                       T,                                                        \
                                        long long, int, const int *, int, int *, int, int, int,    \
-                                       T *, int *, SpaiColumnOut, int *, T *, std::int64_t, int);
+                                       T *, int *, SpaiColumnOut, int *, T *, std::int64_t, int,  \
+                                       std::int64_t);
 BSPAI_BLOCK_INST(double)
 BSPAI_BLOCK_INST(float)
 #undef BSPAI_BLOCK_INST
