@@ -462,11 +462,15 @@ void spai_build_typed(Context &c, const DevCsr &a, const T *aval, T *d, double e
                 qcap *= 4;
                 ccap *= 4;
             }
-            mcap = (int)std::min<std::int64_t>(mcap, n);
+            // slots in multiples of 16: 16-byte aligned columns for the FP32
+            // build's matrix-core sweeps (a cap above n is never reached)
+            mcap = (int)((std::min<std::int64_t>(mcap, n) + 15) / 16 * 16);
             qcap = (int)std::min<std::int64_t>(qcap, n);
             ccap = (int)std::min<std::int64_t>(ccap, n);
             const int words = (int)((n + 31) / 32);
-            const std::int64_t pcap = spai_pair_cap(ccap, n, a.nnz);
+            // (the pair arrays serve the FP32 scoring only)
+            const std::int64_t pcap =
+                sizeof(T) == sizeof(double) ? 1 : spai_pair_cap(ccap, n, a.nnz);
             const std::int64_t bd = spai_block_doubles(mcap, qcap, ccap, pcap);
             const std::int64_t bi = spai_block_ints(mcap, qcap, ccap, pcap, words, n);
             const std::int64_t per_block = bd * (std::int64_t)sizeof(T) + bi * 4;

@@ -1500,6 +1500,7 @@ int spai_wave_max_n() {
 // problem, and so the coefficients, are the reference's up to FP32 rounding;
 // the summation order differs, which the FP32 build allows.
 constexpr int kBlk = 256;
+constexpr int kWyG = 16; // (FP32) reflectors per group of the blocked application
 
 // pcap: (FP32) room for the (row, candidate) pairs of one fit
 struct BlockCaps {
@@ -1507,11 +1508,17 @@ struct BlockCaps {
     std::int64_t pcap;
 };
 
+// T area per workgroup, rounded to 64 entries so that every workgroup's area
+// (and, with M a multiple of 16, every column of R) starts 16-byte aligned;
+// the FP32 build's group T factors ((Q/16 + 1) * 256) and staged pair values
+// (P) follow the pair values.
 std::int64_t spai_block_doubles(int M, int Q, int C, std::int64_t P) {
-    return 2LL * Q * M + 3LL * M + C + 3LL * Q + 1 + P;
+    const std::int64_t t = 2LL * Q * M + 3LL * M + C + 3LL * Q + 1 + 2 * P +
+                           (std::int64_t)(Q / kWyG + 1) * kWyG * kWyG;
+    return (t + 63) / 64 * 64;
 }
 std::int64_t spai_block_ints(int M, int Q, int C, std::int64_t P, int words, int n) {
-    return (std::int64_t)M + 5LL * Q + 3LL * C + 1 + P + 3LL * words + n;
+    return (std::int64_t)M + 5LL * Q + 3LL * C + 1 + 3 * P + 3LL * words + n;
 }
 
 // Block-wide helpers (blockDim.x == kBlk).
@@ -1643,6 +1650,8 @@ __device__ inline double block_sum_fixed(double v, double *sh) {
 
 constexpr int kBatch = 16; // loads issued together in the sequential loops
 constexpr int kRowB = 4;   // rows (or entries) per lane loaded together
+constexpr int kPairB = 4;  // (FP32 scores) entries, pairs or candidates per thread at a time
+constexpr int kTop = 8;    // (FP32 selection) smallest scores each thread keeps
 
 // hypot_ref(...hypot_ref(hypot_ref(0, x0), x1)..., x_{len-1}): the reference's
 // norm2 chain, with the loads issued in batches ahead of the chain.
@@ -1674,6 +1683,44 @@ __device__ inline int lower_bound(const int *a, int m, int x) {
     return lo;
 }
 
+// ---- (FP32) blocked Householder application on the matrix cores.
+// Reflectors j of the incremental QR are grouped by 16 (group g holds
+// j = 16g .. 16g+15); group g's product H_16g ... H_16g+15 is I - V T V^T with
+// T upper triangular (16 x 16, kept in the work area), so applying the group
+// to columns X in order is X - V T^T (V^T X): two tall-skinny products on
+// v_mfma_f32_16x16x4f32 plus a 16 x 16 triangular one.  The instruction's
+// lane layout (one wave, l = lane): a = A[l % 16][l / 16],
+// b = B[l / 16][l % 16], d[i] = D[4 (l / 16) + i][l % 16].
+typedef float wy_f4 __attribute__((ext_vector_type(4)));
+
+__device__ inline wy_f4 wy_mfma(float a, float b, wy_f4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+
+// Rows r .. r+3 (r % 4 == 0, r + 3 < the column stride) of reflector j: zero
+// above its diagonal and from row hi on, its head on the diagonal, the stored
+// tail below; all zero when j >= jend (not a reflector yet).
+__device__ inline wy_f4 wy_refl4(const float *R, int M, const float *vhead, int j, int jend,
+                                 int r, int hi) {
+    wy_f4 x = {0.f, 0.f, 0.f, 0.f};
+    if (j >= jend || r >= hi || r + 3 < j)
+        return x;
+    x = *reinterpret_cast<const wy_f4 *>(R + (std::int64_t)j * M + r);
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        const int row = r + e;
+        x[e] = row < j || row >= hi ? 0.f : (row == j ? vhead[j] : x[e]);
+    }
+    return x;
+}
+// One entry of reflector j at row `row` (same conventions).
+__device__ inline float wy_refl1(const float *R, int M, const float *vhead, int j, int jend,
+                                 int row, int hi) {
+    if (j >= jend || row < j || row >= hi)
+        return 0.f;
+    return row == j ? vhead[j] : R[(std::int64_t)j * M + row];
+}
+
 // kOcc: waves per SIMD the register allocation is held to: 4 for the first
 // workgroup pass (moderate columns: more of them in flight pays), 3 beyond
 // (columns with work areas of megabytes).  Measured: a batch of 256
@@ -1693,7 +1740,7 @@ spai_block_kernel(int n, const int *__restrict__ aptr, const int *__restrict__ a
                   SpaiColumnOut out, int *__restrict__ out_j, T *__restrict__ out_c,
                   std::int64_t out_stride) {
     __shared__ int sh[kBlk + 1];
-    __shared__ int s_slot, s_pk, s_flag, s_wlo, s_whi, s_fail;
+    __shared__ int s_slot, s_pk, s_flag, s_wlo, s_whi, s_fail, s_np;
     __shared__ T s_red[kBlk];
     __shared__ int s_redi[kBlk];
     __shared__ double s_dred[kBlk];
@@ -1702,6 +1749,12 @@ spai_block_kernel(int n, const int *__restrict__ aptr, const int *__restrict__ a
     // (FP32) the back substitution's working vector, when q fits
     constexpr int kYLds = kExact ? 1 : 2048;
     __shared__ T s_y[kYLds];
+    // (FP32) blocked reflector application: the waves' partial 16 x 16 sums, a
+    // group's T factor and T^T W
+    constexpr int kNW = kBlk / kWave;
+    static_assert(kNW == 4, "the chunk dealing assumes four waves");
+    constexpr int kWyLds = kExact ? 1 : (kNW + 2) * 256;
+    __shared__ float s_wy[kWyLds];
     const int tid = threadIdx.x, wave = tid / kWave, lane = tid % kWave;
     const int M = caps.mcap, Q = caps.qcap, C = caps.ccap;
     T *D = dwork + (std::int64_t)blockIdx.x * blk_d;
@@ -1709,11 +1762,15 @@ spai_block_kernel(int n, const int *__restrict__ aptr, const int *__restrict__ a
     T *R = D, *G = D + (std::int64_t)Q * M, *rhs = G + (std::int64_t)Q * M, *v = rhs + M,
       *res = v + M, *sc = res + M, *coef = sc + C, *dot = coef + Q, *vhead = dot + Q + 1;
     T *pv = vhead + Q; // (FP32) values of the (row, candidate) pairs
+    T *Tg = pv + caps.pcap; // (FP32) T factors of the reflector groups, 16 x 16 each
+    T *spv = Tg + (Q / kWyG + 1) * kWyG * kWyG; // (FP32) staged pairs' scaled values
     int *I = Iw, *J = I + M, *Jt = J + Q, *cand = Jt + Q, *clampf = cand + C, *sel = clampf + C;
     int *Jf = sel + Q, *vlen = Jf + Q; // (FP32) factorization order, reflector supports
     // (FP32) pairs per candidate (counts, then offsets), their rows, and the
     // candidate bitmap's rank dictionary (set bits before each word)
-    int *cnt = vlen + Q, *pr = cnt + C + 1, *wpre = pr + caps.pcap;
+    // and, before bucketing, the staged pairs' candidate ranks and slots
+    int *cnt = vlen + Q, *pr = cnt + C + 1, *sk = pr + caps.pcap, *sr = sk + caps.pcap;
+    int *wpre = sr + caps.pcap;
     unsigned *bm = reinterpret_cast<unsigned *>(wpre + caps.words);
     unsigned *inI = bm + caps.words;                       // rows of I; clear between columns
     int *pos = reinterpret_cast<int *>(inI + caps.words); // all -1 between columns
@@ -2109,48 +2166,85 @@ spai_block_kernel(int n, const int *__restrict__ aptr, const int *__restrict__ a
                     __syncthreads();
                     inf_norm = s_scal[3];
                 }
-                // The old reflectors on the new columns: one wave per column,
-                // reflectors in order (v_j^T x in FP64, fixed butterfly).
+                // The old reflectors on the new columns, group after group in
+                // their order, in the blocked form above.  16-row chunks of the
+                // slots are dealt to the waves (chunk c to wave c % 4) alike for
+                // every group, so the sweep that applies group g to a wave's rows
+                // also forms that wave's share of group g+1's V^T X from the rows
+                // it has just rewritten.  No old reflector reaches a slot >= mold.
                 {
+                    float *wp = s_wy, *tt = s_wy + kNW * 256, *yy = tt + 256;
+                    const int cl = lane & 15, quad = lane >> 4;
+                    const int ngroups = (qf + kWyG - 1) / kWyG;
+                    const int nchunks = (mold + 15) / 16;
                     bool bad = false;
-                    for (int c = qf + wave; c < q; c += kBlk / kWave) {
-                        T *x = R + (std::int64_t)c * M;
-                        for (int j = 0; j < qf; ++j) {
-                            const int len = vlen[j];
-                            const T *vj = R + (std::int64_t)j * M;
-                            const T h0 = vhead[j];
-                            double d = 0;
-                            for (int r0 = j + lane; r0 < len; r0 += kWave * kRowB) {
-                                T vb[kRowB], xb[kRowB];
-    #pragma unroll
-                                for (int u = 0; u < kRowB; ++u) {
-                                    const int r = min(r0 + u * kWave, len - 1);
-                                    vb[u] = r == j ? h0 : vj[r];
-                                    xb[u] = x[r];
-                                }
-    #pragma unroll
-                                for (int u = 0; u < kRowB; ++u)
-                                    if (r0 + u * kWave < len)
-                                        d += (double)vb[u] * (double)xb[u];
-                            }
-                            const T dc = (T)wave_sum(d);
-                            for (int r0 = j + lane; r0 < len; r0 += kWave * kRowB) {
-                                T vb[kRowB], xb[kRowB];
-    #pragma unroll
-                                for (int u = 0; u < kRowB; ++u) {
-                                    const int r = min(r0 + u * kWave, len - 1);
-                                    vb[u] = r == j ? h0 : vj[r];
-                                    xb[u] = x[r];
-                                }
-    #pragma unroll
-                                for (int u = 0; u < kRowB; ++u)
-                                    if (r0 + u * kWave < len) {
-                                        const T y = xb[u] - T(2) * vb[u] * dc;
-                                        x[r0 + u * kWave] = y;
-                                        bad = bad || !isfinite(y);
+                    for (int c0 = qf; c0 < q; c0 += 16) {
+                        const int xc = c0 + cl;
+                        const bool xok = xc < q;
+                        float *X = R + (std::int64_t)(xok ? xc : 0) * M;
+                        wy_f4 acc = {0.f, 0.f, 0.f, 0.f};
+                        for (int ch = wave; ch < nchunks; ch += kNW) {
+                            const int r = 16 * ch + 4 * quad;
+                            const wy_f4 va = wy_refl4(R, M, vhead, cl, qf, r, mold);
+                            wy_f4 xb = {0.f, 0.f, 0.f, 0.f};
+                            if (xok && r < mold)
+                                xb = *reinterpret_cast<const wy_f4 *>(X + r);
+#pragma unroll
+                            for (int e = 0; e < 4; ++e)
+                                acc = wy_mfma(va[e], xb[e], acc);
+                        }
+                        for (int g = 0; g < ngroups; ++g) {
+                            const int g0 = g * kWyG;
+#pragma unroll
+                            for (int e = 0; e < 4; ++e)
+                                wp[wave * 256 + (4 * quad + e) * 16 + cl] = acc[e];
+                            tt[tid] = Tg[(std::int64_t)g * 256 + tid];
+                            __syncthreads();
+                            {
+                                // Y = T^T W (W = the waves' partial sums)
+                                const int i = tid >> 4, c = tid & 15;
+                                float y = 0.f;
+                                if (g0 + i < qf)
+                                    for (int l = 0; l <= i; ++l) {
+                                        const int z = l * 16 + c;
+                                        y += tt[l * 16 + i] * (wp[z] + wp[256 + z] + wp[512 + z] +
+                                                               wp[768 + z]);
                                     }
+                                yy[tid] = y;
                             }
-                            wave_sync(); // x as rewritten, before the next dot
+                            __syncthreads();
+                            float yb[4];
+#pragma unroll
+                            for (int kk = 0; kk < 4; ++kk)
+                                yb[kk] = yy[(4 * kk + quad) * 16 + cl];
+                            const bool next = g + 1 < ngroups;
+                            acc = wy_f4{0.f, 0.f, 0.f, 0.f};
+                            for (int ch = g + ((wave - g) & (kNW - 1)); ch < nchunks; ch += kNW) {
+                                const int r0 = 16 * ch;
+                                wy_f4 d = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                                for (int kk = 0; kk < 4; ++kk)
+                                    d = wy_mfma(wy_refl1(R, M, vhead, g0 + 4 * kk + quad, qf,
+                                                         r0 + cl, mold),
+                                                yb[kk], d);
+                                const int r = r0 + 4 * quad;
+                                wy_f4 xv = {0.f, 0.f, 0.f, 0.f};
+                                if (xok && r < mold) {
+                                    xv = *reinterpret_cast<const wy_f4 *>(X + r);
+                                    xv -= d; // zero at rows >= mold
+                                    *reinterpret_cast<wy_f4 *>(X + r) = xv;
+#pragma unroll
+                                    for (int e = 0; e < 4; ++e)
+                                        bad = bad || (r + e < mold && !isfinite(xv[e]));
+                                }
+                                if (next) {
+                                    const wy_f4 va =
+                                        wy_refl4(R, M, vhead, g0 + kWyG + cl, qf, r, mold);
+#pragma unroll
+                                    for (int e = 0; e < 4; ++e)
+                                        acc = wy_mfma(va[e], xv[e], acc);
+                                }
+                            }
                         }
                     }
                     if (tid == 0)
@@ -2260,6 +2354,53 @@ spai_block_kernel(int n, const int *__restrict__ aptr, const int *__restrict__ a
                     const T ad = fabs(diagonal);
                     piv_min = ad < piv_min ? ad : piv_min;
                     __syncthreads();
+                }
+                // T factors of the groups the new reflectors joined: the Gram
+                // matrix Z = V_g^T V_g on the matrix cores, then column by column
+                // T[0:jj, jj] = -2 T[0:jj, 0:jj] Z[0:jj, jj] and T[jj][jj] = 2
+                // (unit reflectors, H = I - 2 v v^T); zero below the diagonal.
+                if (fit_status == kSpaiSuccess) {
+                    float *wp = s_wy, *tt = s_wy + kNW * 256;
+                    const int cl = lane & 15, quad = lane >> 4;
+                    const int nchunks = (m + 15) / 16;
+                    for (int g = qf / kWyG; g * kWyG < q; ++g) {
+                        const int g0 = g * kWyG;
+                        wy_f4 acc = {0.f, 0.f, 0.f, 0.f};
+                        for (int ch = g + ((wave - g) & (kNW - 1)); ch < nchunks; ch += kNW) {
+                            const wy_f4 va = wy_refl4(R, M, vhead, g0 + cl, q, 16 * ch + 4 * quad, m);
+#pragma unroll
+                            for (int e = 0; e < 4; ++e)
+                                acc = wy_mfma(va[e], va[e], acc);
+                        }
+#pragma unroll
+                        for (int e = 0; e < 4; ++e)
+                            wp[wave * 256 + (4 * quad + e) * 16 + cl] = acc[e];
+                        tt[tid] = Tg[(std::int64_t)g * 256 + tid];
+                        __syncthreads();
+                        if (wave == 0) {
+                            const int j1 = min(q, g0 + kWyG) - g0;
+                            for (int jj = max(qf, g0) - g0; jj < j1; ++jj) {
+                                float t = 0.f;
+                                if (lane < jj) {
+                                    for (int l = lane; l < jj; ++l) {
+                                        const int z = l * 16 + jj;
+                                        t += tt[lane * 16 + l] *
+                                             (wp[z] + wp[256 + z] + wp[512 + z] + wp[768 + z]);
+                                    }
+                                    t *= -2.f;
+                                } else if (lane == jj) {
+                                    t = 2.f;
+                                }
+                                wave_sync();
+                                if (lane < 16)
+                                    tt[lane * 16 + jj] = t;
+                                wave_sync();
+                            }
+                            for (int e = lane; e < 256; e += kWave)
+                                Tg[(std::int64_t)g * 256 + e] = tt[e];
+                        }
+                        __syncthreads();
+                    }
                 }
                 if (fit_status == kSpaiSuccess)
                     qf = q;
@@ -2433,6 +2574,17 @@ spai_block_kernel(int n, const int *__restrict__ aptr, const int *__restrict__ a
             if (tid == 0)
                 s_fail = INT_MAX;
             __syncthreads();
+            // (FP32) the thread's share of the mean and of the clamps, and its
+            // kTop smallest (score, index) pairs, ascending
+            double msum = 0;
+            int mycl = 0;
+            T tops[kTop];
+            int topj[kTop];
+#pragma unroll
+            for (int w = 0; w < kTop; ++w) {
+                tops[w] = T(INFINITY);
+                topj[w] = INT_MAX;
+            }
             if constexpr (kExact) {
                 for (int t = tid; t < c; t += kBlk) {
                     const int j = cand[t];
@@ -2476,26 +2628,72 @@ spai_block_kernel(int n, const int *__restrict__ aptr, const int *__restrict__ a
                 // at n = 200,000) while only one or two of its entries lie in
                 // I.  So the pairs are found from I instead: the entries of
                 // B's rows i in I whose column j is a candidate (j's position
-                // in cand[] from the bitmap's rank dictionary), bucketed per
-                // candidate by a counting sort, each bucket ordered by slot,
-                // then summed in that order.
+                // in cand[] from the bitmap's rank dictionary).  Pass A stages
+                // them as (rank, slot, scaled value) in any order and counts
+                // each candidate's pairs; pass B buckets the staged pairs per
+                // candidate (counting sort); pass C orders each bucket by slot
+                // and sums it, and, over the candidates a thread scores (in a
+                // fixed order), accumulates the mean's partial sum, the clamp
+                // count and the thread's kTop best (score, index).  Every pass
+                // issues its independent loads together (the work area lives
+                // in HBM, so each dependent load costs microseconds).
                 const std::int64_t kPairCap = caps.pcap;
-                auto in_cand = [&](int j) { return (bm[j >> 5] >> (j & 31)) & 1u; };
-                auto rank = [&](int j) {
-                    return wpre[(j >> 5) - cw0] + __popc(bm[j >> 5] & ((1u << (j & 31)) - 1u));
-                };
                 for (int t = tid; t <= c; t += kBlk)
                     cnt[t] = 0;
+                if (tid == 0)
+                    s_np = 0;
                 __syncthreads();
                 for (int r = tid; r < m; r += kBlk) {
                     const int i = I[r];
-                    for (int e = bptr[i]; e < bptr[i + 1]; ++e) {
-                        const int j = bcol[e];
-                        if (in_cand(j))
-                            atomicAdd(cnt + rank(j), 1);
+                    const int e1 = bptr[i + 1];
+                    for (int e0 = bptr[i]; e0 < e1; e0 += kPairB) {
+                        int jb[kPairB], rk[kPairB];
+                        unsigned wb[kPairB];
+                        T vb[kPairB], db[kPairB];
+#pragma unroll
+                        for (int u = 0; u < kPairB; ++u)
+                            jb[u] = bcol[min(e0 + u, e1 - 1)];
+#pragma unroll
+                        for (int u = 0; u < kPairB; ++u) {
+                            wb[u] = bm[jb[u] >> 5];
+                            vb[u] = bval[min(e0 + u, e1 - 1)];
+                            db[u] = dscale[jb[u]];
+                        }
+                        int nk = 0;
+#pragma unroll
+                        for (int u = 0; u < kPairB; ++u) {
+                            const bool ok = e0 + u < e1 && ((wb[u] >> (jb[u] & 31)) & 1u);
+                            rk[u] = ok ? (jb[u] >> 5) - cw0 : -1;
+                            nk += ok ? 1 : 0;
+                        }
+#pragma unroll
+                        for (int u = 0; u < kPairB; ++u)
+                            if (rk[u] >= 0)
+                                rk[u] = wpre[rk[u]] +
+                                        __popc(wb[u] & ((1u << (jb[u] & 31)) - 1u));
+                        if (nk == 0)
+                            continue;
+                        int at = atomicAdd(&s_np, nk);
+#pragma unroll
+                        for (int u = 0; u < kPairB; ++u)
+                            if (rk[u] >= 0) {
+                                atomicAdd(cnt + rk[u], 1);
+                                if (at < kPairCap) {
+                                    sk[at] = rk[u];
+                                    sr[at] = r;
+                                    spv[at] = vb[u] * db[u];
+                                }
+                                ++at;
+                            }
                     }
                 }
                 __syncthreads();
+                const int np = s_np;
+                if (np > kPairCap) {
+                    bitmap_clear(bm, cw0, cw1);
+                    status = kSpaiOverflow;
+                    break;
+                }
                 // exclusive scan of the counts (each thread a run of candidates)
                 {
                     const int per = (c + kBlk - 1) / kBlk;
@@ -2515,59 +2713,114 @@ spai_block_kernel(int n, const int *__restrict__ aptr, const int *__restrict__ a
                     if (tid == 0)
                         cnt[c] = total;
                     __syncthreads();
-                    if (total > kPairCap) {
-                        bitmap_clear(bm, cw0, cw1);
-                        status = kSpaiOverflow;
-                        break;
-                    }
                 }
-                for (int r = tid; r < m; r += kBlk) {
-                    const int i = I[r];
-                    for (int e = bptr[i]; e < bptr[i + 1]; ++e) {
-                        const int j = bcol[e];
-                        if (in_cand(j)) {
-                            const int at = atomicAdd(clampf + rank(j), 1);
-                            pr[at] = r;
-                            pv[at] = bval[e];
-                        }
+                // pass B: the staged pairs into their candidates' buckets
+                for (int a0 = tid; a0 < np; a0 += kBlk * kPairB) {
+                    int kb[kPairB], rb[kPairB], pb[kPairB];
+                    T vb[kPairB];
+#pragma unroll
+                    for (int u = 0; u < kPairB; ++u) {
+                        const int x = min(a0 + u * kBlk, np - 1);
+                        kb[u] = sk[x];
+                        rb[u] = sr[x];
+                        vb[u] = spv[x];
                     }
+#pragma unroll
+                    for (int u = 0; u < kPairB; ++u)
+                        pb[u] = a0 + u * kBlk < np ? atomicAdd(clampf + kb[u], 1) : -1;
+#pragma unroll
+                    for (int u = 0; u < kPairB; ++u)
+                        if (pb[u] >= 0) {
+                            pr[pb[u]] = rb[u];
+                            pv[pb[u]] = vb[u];
+                        }
                 }
                 __syncthreads();
                 bitmap_clear(bm, cw0, cw1);
-                for (int t = tid; t < c; t += kBlk) {
-                    const int j = cand[t];
-                    const T dj = dscale[j];
-                    const int p0 = cnt[t], p1 = cnt[t + 1];
-                    for (int a = p0 + 1; a < p1; ++a) { // by slot (few pairs each)
-                        const int rr0 = pr[a];
-                        const T vv0 = pv[a];
-                        int b = a - 1;
-                        while (b >= p0 && pr[b] > rr0) {
-                            pr[b + 1] = pr[b];
-                            pv[b + 1] = pv[b];
-                            --b;
+                // pass C, kPairB candidates of the thread at a time
+                for (int t0 = tid; t0 < c; t0 += kBlk * kPairB) {
+                    int p0[kPairB], p1[kPairB], jc[kPairB], r0[kPairB];
+                    T v0[kPairB], x0[kPairB];
+#pragma unroll
+                    for (int u = 0; u < kPairB; ++u) {
+                        const int t = min(t0 + u * kBlk, c - 1);
+                        p0[u] = cnt[t];
+                        p1[u] = cnt[t + 1];
+                        jc[u] = cand[t];
+                    }
+#pragma unroll
+                    for (int u = 0; u < kPairB; ++u) {
+                        const bool any = p1[u] > p0[u];
+                        r0[u] = any ? pr[p0[u]] : 0;
+                        v0[u] = any ? pv[p0[u]] : T(0);
+                    }
+#pragma unroll
+                    for (int u = 0; u < kPairB; ++u)
+                        x0[u] = res[r0[u]];
+#pragma unroll
+                    for (int u = 0; u < kPairB; ++u) {
+                        const int t = t0 + u * kBlk;
+                        if (t >= c)
+                            continue;
+                        T dt = 0, den = 0;
+                        if (p1[u] - p0[u] == 1) {
+                            dt = x0[u] * v0[u];
+                            den = v0[u] * v0[u];
+                        } else {
+                            const int q0 = p0[u], q1 = p1[u];
+                            for (int a = q0 + 1; a < q1; ++a) { // by slot (few pairs each)
+                                const int rr0 = pr[a];
+                                const T vv0 = pv[a];
+                                int b = a - 1;
+                                while (b >= q0 && pr[b] > rr0) {
+                                    pr[b + 1] = pr[b];
+                                    pv[b + 1] = pv[b];
+                                    --b;
+                                }
+                                pr[b + 1] = rr0;
+                                pv[b + 1] = vv0;
+                            }
+                            for (int a = q0; a < q1; ++a) {
+                                const T value = pv[a];
+                                dt += res[pr[a]] * value;
+                                den += value * value;
+                            }
                         }
-                        pr[b + 1] = rr0;
-                        pv[b + 1] = vv0;
+                        T sv_ = 0;
+                        bool cl = false;
+                        const int st = score_of(rr, dt, den, sv_, cl);
+                        sc[t] = sv_;
+                        clampf[t] = st == kSpaiSuccess ? (cl ? 1 : 0) : -st;
+                        if (st != kSpaiSuccess) {
+                            atomicMin(&s_fail, t);
+                            continue;
+                        }
+                        msum += (double)sv_;
+                        mycl += cl ? 1 : 0;
+                        // keep the thread's kTop smallest (score, index)
+                        const int j = jc[u];
+                        if (sv_ < tops[kTop - 1] || (sv_ == tops[kTop - 1] && j < topj[kTop - 1])) {
+                            T ins = sv_;
+                            int insj = j;
+#pragma unroll
+                            for (int w = 0; w < kTop; ++w) {
+                                const bool lt = ins < tops[w] || (ins == tops[w] && insj < topj[w]);
+                                const T os = tops[w];
+                                const int oj = topj[w];
+                                tops[w] = lt ? ins : os;
+                                topj[w] = lt ? insj : oj;
+                                ins = lt ? os : ins;
+                                insj = lt ? oj : insj;
+                            }
+                        }
                     }
-                    T dt = 0, den = 0;
-                    for (int a = p0; a < p1; ++a) {
-                        const T value = pv[a] * dj;
-                        dt += res[pr[a]] * value;
-                        den += value * value;
-                    }
-                    T sv = 0;
-                    bool cl = false;
-                    const int st = score_of(rr, dt, den, sv, cl);
-                    sc[t] = sv;
-                    clampf[t] = st == kSpaiSuccess ? (cl ? 1 : 0) : -st;
-                    if (st != kSpaiSuccess)
-                        atomicMin(&s_fail, t);
                 }
             }
             __syncthreads();
             const int first_bad = s_fail;
-            {
+            if (!kExact && first_bad == INT_MAX) {
+                clamps += (int)block_sum_fixed((double)mycl, s_dred);
+            } else {
                 // clamps are counted up to the first failing candidate, as the
                 // reference's sequential loop does
                 const int lim = first_bad != INT_MAX ? first_bad : c;
@@ -2605,10 +2858,7 @@ spai_block_kernel(int n, const int *__restrict__ aptr, const int *__restrict__ a
                 __syncthreads();
                 mean = s_scal[0];
             } else {
-                double acc = 0;
-                for (int t = tid; t < c; t += kBlk)
-                    acc += (double)sc[t];
-                mean = (T)(block_sum_fixed(acc, s_dred) / (double)c);
+                mean = (T)(block_sum_fixed(msum, s_dred) / (double)c);
             }
             if (!isfinite(mean)) {
                 status = kSpaiNumericalFailure;
@@ -2618,6 +2868,58 @@ spai_block_kernel(int n, const int *__restrict__ aptr, const int *__restrict__ a
             int nsel = 0;
             T prev_s = T(-INFINITY);
             int prev_j = -1;
+            if (!kExact && beta <= kTop) {
+                // (FP32) merge of the threads' sorted lists: each round takes
+                // the smallest head (a wave minimum, then the four waves'), and
+                // its thread drops it.  Rounds alternate between two halves of
+                // s_red / s_redi, so one barrier per round suffices.
+                for (; nsel < beta; ++nsel) {
+                    T bs = tops[0] <= mean ? tops[0] : T(INFINITY);
+                    int bj = tops[0] <= mean ? topj[0] : INT_MAX;
+#pragma unroll
+                    for (int off = kWave / 2; off > 0; off >>= 1) {
+                        const T os = __shfl_xor(bs, off, kWave);
+                        const int oj = __shfl_xor(bj, off, kWave);
+                        const bool lt = os < bs || (os == bs && oj < bj);
+                        bs = lt ? os : bs;
+                        bj = lt ? oj : bj;
+                    }
+                    const int half = (nsel & 1) * kNW;
+                    if (lane == 0) {
+                        s_red[half + wave] = bs;
+                        s_redi[half + wave] = bj;
+                    }
+                    __syncthreads();
+                    bs = s_red[half];
+                    bj = s_redi[half];
+#pragma unroll
+                    for (int w = 1; w < kNW; ++w) {
+                        const T os = s_red[half + w];
+                        const int oj = s_redi[half + w];
+                        const bool lt = os < bs || (os == bs && oj < bj);
+                        bs = lt ? os : bs;
+                        bj = lt ? oj : bj;
+                    }
+                    if (bj == INT_MAX)
+                        break;
+                    if (q + nsel + 1 > Q) { // the grown pattern would not fit
+                        nsel = -1;
+                        break;
+                    }
+                    if (topj[0] == bj) {
+#pragma unroll
+                        for (int w = 0; w + 1 < kTop; ++w) {
+                            tops[w] = tops[w + 1];
+                            topj[w] = topj[w + 1];
+                        }
+                        tops[kTop - 1] = T(INFINITY);
+                        topj[kTop - 1] = INT_MAX;
+                    }
+                    if (tid == 0)
+                        sel[nsel] = bj;
+                }
+                __syncthreads();
+            } else
             for (; nsel < beta; ++nsel) {
                 T best_s = T(INFINITY);
                 int best_j = -1;
