@@ -40,6 +40,9 @@ def parse():
                          "'canon' (canonical CSR from 29M coordinate entries)")
     ap.add_argument("--powerlaw-n", type=int, default=50000,
                     help="rows of the power-law matrix timed by --extra powerlaw")
+    ap.add_argument("--only-extra", action="store_true",
+                    help="time only the --extra workloads (no headline, no CPU baseline) and "
+                         "print their JSON line")
     ap.add_argument("--pmc-workload", action="store_true",
                     help="run the short workload profiled for HBM traffic counters instead of "
                          "the benchmark: a 1 GiB device copy (counter calibration), one "
@@ -119,6 +122,17 @@ def main():
     stream = torch.cuda.Stream(dev)
     torch.cuda.set_stream(stream)
     ctx = bspai_hip.Context(0, stream=stream.cuda_stream)
+    if args.only_extra:
+        extras = [e for e in args.extra.split(",") if e]
+        out = {}
+        if "many" in extras:
+            out["many_systems"] = bench_many(ctx)
+        if "powerlaw" in extras:
+            out["power_law"] = bench_power_law(ctx, args.powerlaw_n)
+        if "canon" in extras:
+            out["canonical_csr"] = bench_canon(ctx, cpu=False)
+        print(json.dumps({"extra": out}), flush=True)
+        return
     a = P.convection_diffusion_3d(side=args.side)
     n, nnz = a.n, a.nnz
     b_host = P.equal_rhs(n)
@@ -240,6 +254,13 @@ def main():
         "kernels": {name: [{k: r[k] for k in ("kernel", "launches", "avg_us", "share", "bound",
                                               "achieved", "unit", "frac")}
                            for r in rows[:8]] for name, rows in roof.items()},
+        # the SpMV rows on their own (>= 70% of HBM peak),
+        # against the algorithmic bytes of an int32-column CSR product
+        # (value + 4-byte column per entry, row offsets, x and y once)
+        "spmv": {name: {r["kernel"]: {"avg_us": r["avg_us"], "achieved_GBs": r["achieved"],
+                                      "frac": r["frac"]}
+                        for r in rows if r["kernel"].startswith("spmv_")}
+                 for name, rows in roof.items()},
         "cpu_baseline": cpu,
         "cpu_baseline_c1": cpu_c1,
         "precision_report": precision_report,
@@ -302,15 +323,29 @@ def bench_power_law(ctx, n):
     a = P.power_law(n=n)
     b = P.equal_rhs(n)
     m = bspai_hip.Matrix(a, ctx)
+    log(f"power law n={n}: SPAI build (mixed, eps 0.2)")
+    ctx.set_timing("spai")
     t0 = time.perf_counter()
     s = bspai_hip.construct_spai(m, 0.2, -(-n // 8), 8, precision="mixed")
+    t1 = time.perf_counter()
+    kt = ctx.kernel_times()
+    ctx.set_timing(False)
+    log(f"power law n={n}: SPAI {t1 - t0:.1f} s; mixed IR")
     r = bspai_hip.refine_with_preconditioner(m, s, b, precision="mixed", tolerance=1e-4,
                                              relres_target=1e-10)
     el = time.perf_counter() - t0
     x = np.asarray(r.solution)
     relres = float(np.linalg.norm(b - a.to_scipy() @ x) / np.linalg.norm(b))
+    # per pass: kernel time and the QR rate of its accepted fits (FP32 peak
+    # 157.3 TFLOP/s)
+    passes = {k: {"ms": round(v["total_ms"], 1),
+                  "qr_tflops": round(v["flops"] / (v["total_ms"] * 1e-3) / 1e12, 4)
+                  if v["total_ms"] > 0 else 0.0}
+              for k, v in kt.items()}
     return {"n": n, "nnz": int(a.nnz), "status": r.status, "relres": relres,
             "spai_ms": round(s.build_ms, 1), "passes": s.passes, "fits": s.fits,
+            "pass_kernels": passes, "outer_steps": r.outer_steps,
+            "inner_iterations": r.inner_iterations,
             "solve_ms": round(r.solve_ms, 1), "wall_s": round(el, 3)}
 
 

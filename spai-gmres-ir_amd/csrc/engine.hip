@@ -445,6 +445,12 @@ void spai_build_typed(Context &c, const DevCsr &a, const T *aval, T *d, double e
     const std::int64_t area_cap =
         std::min<std::int64_t>(8LL << 30, std::max<std::int64_t>(1LL << 28, free_bytes / 4));
     DeviceBuffer &dwork = w.dwork, &iwork = w.iwork;
+    // Waves per SIMD of the workgroup kernel: 4 for the first workgroup pass,
+    // 3 beyond; BSPAI_SPAI_BLOCK_OCC=3 or 4 sets every pass (tuning).
+    int occ_env = 0;
+    if (const char *env = std::getenv("BSPAI_SPAI_BLOCK_OCC"))
+        occ_env = std::atoi(env);
+    auto block_occ = [&](int pass) { return occ_env == 3 || occ_env == 4 ? occ_env : (pass == 1 ? 4 : 3); };
     std::vector<std::string> pass_labels; // timer label of each capacity pass
     while (nwork > 0) {
         if (out.passes >= 8)
@@ -503,7 +509,7 @@ void spai_build_typed(Context &c, const DevCsr &a, const T *aval, T *d, double e
                                      (T)eps, alpha, beta, work.data(),
                                      nwork, counter.data(), mcap, qcap, ccap, dwork.as<T>(),
                                      iwork.as<int>(), o, out.pool_j.back().data(),
-                                     out.pool_c.back().as<T>(), qcap, pass == 1 ? 4 : 3,
+                                     out.pool_c.back().as<T>(), qcap, block_occ(pass),
                                      pcap);
             }, true);
             hipLaunchKernelGGL(collect_overflow_kernel, dim3(ceil_div(nwork, 256)), dim3(256), 0,
@@ -888,10 +894,11 @@ void GmresEngine<T>::solve(const DevCsr &m1, const DevCsr &m2, GmresMode mode, c
     const std::string lupd = std::string("gram_update_kernel<") + tname + ">";
     const std::string lnrm = std::string("normalize_givens_kernel<") + tname + ">";
     const double sz = sizeof(T), nn = (double)L_.n;
-    // (the column index takes 2 bytes on 16-bit offsets, else 4)
-    auto spmv_bytes = [&](const DevCsr &m, const SpmvChoice &k) {
-        return (double)m.nnz * (sz + (k.c16 ? 2 : 4)) + 4.0 * (double)(m.n + 1) +
-               2.0 * (double)m.n * sz;
+    // (the algorithmic bytes count a 4-byte column index per entry, also
+    // where the kernel reads 16-bit offsets: value + int32 column per entry,
+    // int32 row offsets, x once, y once)
+    auto spmv_bytes = [&](const DevCsr &m, const SpmvChoice &) {
+        return (double)m.nnz * (sz + 4) + 4.0 * (double)(m.n + 1) + 2.0 * (double)m.n * sz;
     };
     const short *c16a = k1.c16 ? m1.ensure_col16(c_) : nullptr;
     const short *c16b = k2.c16 ? m2.ensure_col16(c_) : nullptr;

@@ -208,7 +208,7 @@ struct Layout {
     int uniform = 0; // every tile full and 16-byte aligned for FP32 and FP64
     Tiles tiles() const {
         return Tiles{ntiles,         nsys,           t_sys.data(), t_row0.data(),
-                     t_row1.data(),  s_tile0.data(), s_tile1.data(), uniform};
+                     t_row1.data(),  s_tile0.data(), s_tile1.data(), uniform, (int)n};
     }
     void build(const Context &c, std::int64_t n_, const std::vector<std::int64_t> &offs);
 };

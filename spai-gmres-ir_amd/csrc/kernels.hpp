@@ -101,6 +101,8 @@ struct Tiles {
     const int *sys_tile0;
     const int *sys_tile1;
     int uniform; // every tile full (kTileRows rows) and starting at a multiple of 4 rows
+    int nrows;   // rows of all systems; with one system, tile t is rows
+                 // [t kTileRows, min((t+1) kTileRows, nrows)) of system 0
 };
 // A tile is one block of the vector kernels: 256 threads x 8 rows, one
 // partial per tile and reduction.

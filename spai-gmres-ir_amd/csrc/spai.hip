@@ -1635,15 +1635,14 @@ __device__ inline void publish_range(int lo, int hi, int *wlo, int *whi) {
 
 // Block sum in a fixed tree order (every thread gets the same bits).
 __device__ inline double block_sum_fixed(double v, double *sh) {
+    // wave sums by a fixed butterfly, then the four waves' in a fixed order
     const int tid = threadIdx.x;
-    sh[tid] = v;
+    v = wave_sum_d(v);
+    if ((tid & (kWave - 1)) == 0)
+        sh[tid / kWave] = v;
     __syncthreads();
-    for (int off = kBlk / 2; off > 0; off >>= 1) {
-        if (tid < off)
-            sh[tid] += sh[tid + off];
-        __syncthreads();
-    }
-    const double r = sh[0];
+    static_assert(kBlk == 4 * kWave, "four waves per workgroup");
+    const double r = (sh[0] + sh[1]) + (sh[2] + sh[3]);
     __syncthreads();
     return r;
 }
@@ -1652,6 +1651,8 @@ constexpr int kBatch = 16; // loads issued together in the sequential loops
 constexpr int kRowB = 4;   // rows (or entries) per lane loaded together
 constexpr int kPairB = 4;  // (FP32 scores) entries, pairs or candidates per thread at a time
 constexpr int kTop = 8;    // (FP32 selection) smallest scores each thread keeps
+constexpr int kPairS = 4;  // (FP32 scores) pairs of a candidate ordered in registers
+constexpr int kCandB = 2;  // (FP32 scores) candidates per thread at a time in the sums
 
 // hypot_ref(...hypot_ref(hypot_ref(0, x0), x1)..., x_{len-1}): the reference's
 // norm2 chain, with the loads issued in batches ahead of the chain.
@@ -2219,30 +2220,54 @@ spai_block_kernel(int n, const int *__restrict__ aptr, const int *__restrict__ a
                                 yb[kk] = yy[(4 * kk + quad) * 16 + cl];
                             const bool next = g + 1 < ngroups;
                             acc = wy_f4{0.f, 0.f, 0.f, 0.f};
-                            for (int ch = g + ((wave - g) & (kNW - 1)); ch < nchunks; ch += kNW) {
-                                const int r0 = 16 * ch;
-                                wy_f4 d = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-                                for (int kk = 0; kk < 4; ++kk)
-                                    d = wy_mfma(wy_refl1(R, M, vhead, g0 + 4 * kk + quad, qf,
-                                                         r0 + cl, mold),
-                                                yb[kk], d);
-                                const int r = r0 + 4 * quad;
-                                wy_f4 xv = {0.f, 0.f, 0.f, 0.f};
-                                if (xok && r < mold) {
+                            // the next chunk's loads are issued before this
+                            // chunk's stores (other rows: no overlap)
+                            int ch = g + ((wave - g) & (kNW - 1));
+                            float a0 = 0.f, a1 = 0.f, a2 = 0.f, a3 = 0.f;
+                            wy_f4 xv = {0.f, 0.f, 0.f, 0.f}, vn = {0.f, 0.f, 0.f, 0.f};
+                            if (ch < nchunks) {
+                                const int r0 = 16 * ch, r = r0 + 4 * quad;
+                                a0 = wy_refl1(R, M, vhead, g0 + quad, qf, r0 + cl, mold);
+                                a1 = wy_refl1(R, M, vhead, g0 + 4 + quad, qf, r0 + cl, mold);
+                                a2 = wy_refl1(R, M, vhead, g0 + 8 + quad, qf, r0 + cl, mold);
+                                a3 = wy_refl1(R, M, vhead, g0 + 12 + quad, qf, r0 + cl, mold);
+                                if (xok && r < mold)
                                     xv = *reinterpret_cast<const wy_f4 *>(X + r);
-                                    xv -= d; // zero at rows >= mold
-                                    *reinterpret_cast<wy_f4 *>(X + r) = xv;
+                                if (next)
+                                    vn = wy_refl4(R, M, vhead, g0 + kWyG + cl, qf, r, mold);
+                            }
+                            for (; ch < nchunks; ch += kNW) {
+                                wy_f4 d = {0.f, 0.f, 0.f, 0.f};
+                                d = wy_mfma(a0, yb[0], d);
+                                d = wy_mfma(a1, yb[1], d);
+                                d = wy_mfma(a2, yb[2], d);
+                                d = wy_mfma(a3, yb[3], d);
+                                wy_f4 xc = xv;
+                                const wy_f4 vc = vn;
+                                const int r = 16 * ch + 4 * quad;
+                                if (ch + kNW < nchunks) {
+                                    const int r0n = 16 * (ch + kNW), rn = r0n + 4 * quad;
+                                    a0 = wy_refl1(R, M, vhead, g0 + quad, qf, r0n + cl, mold);
+                                    a1 = wy_refl1(R, M, vhead, g0 + 4 + quad, qf, r0n + cl, mold);
+                                    a2 = wy_refl1(R, M, vhead, g0 + 8 + quad, qf, r0n + cl, mold);
+                                    a3 = wy_refl1(R, M, vhead, g0 + 12 + quad, qf, r0n + cl, mold);
+                                    xv = wy_f4{0.f, 0.f, 0.f, 0.f};
+                                    if (xok && rn < mold)
+                                        xv = *reinterpret_cast<const wy_f4 *>(X + rn);
+                                    if (next)
+                                        vn = wy_refl4(R, M, vhead, g0 + kWyG + cl, qf, rn, mold);
+                                }
+                                if (xok && r < mold) {
+                                    xc -= d; // zero at rows >= mold
+                                    *reinterpret_cast<wy_f4 *>(X + r) = xc;
 #pragma unroll
                                     for (int e = 0; e < 4; ++e)
-                                        bad = bad || (r + e < mold && !isfinite(xv[e]));
+                                        bad = bad || (r + e < mold && !isfinite(xc[e]));
                                 }
                                 if (next) {
-                                    const wy_f4 va =
-                                        wy_refl4(R, M, vhead, g0 + kWyG + cl, qf, r, mold);
 #pragma unroll
                                     for (int e = 0; e < 4; ++e)
-                                        acc = wy_mfma(va[e], xv[e], acc);
+                                        acc = wy_mfma(vc[e], xc[e], acc);
                                 }
                             }
                         }
@@ -2471,6 +2496,8 @@ spai_block_kernel(int n, const int *__restrict__ aptr, const int *__restrict__ a
                 if (r == pk)
                     acc -= T(1);
                 res[r] = acc;
+                if (!kExact && m <= kYLds)
+                    s_y[r] = acc; // (FP32) the scores' gathers read it from LDS
             }
             __syncthreads();
             T rnorm;
@@ -2737,38 +2764,68 @@ spai_block_kernel(int n, const int *__restrict__ aptr, const int *__restrict__ a
                 }
                 __syncthreads();
                 bitmap_clear(bm, cw0, cw1);
-                // pass C, kPairB candidates of the thread at a time
-                for (int t0 = tid; t0 < c; t0 += kBlk * kPairB) {
-                    int p0[kPairB], p1[kPairB], jc[kPairB], r0[kPairB];
-                    T v0[kPairB], x0[kPairB];
+                // pass C, kCandB candidates of the thread at a time; buckets of
+                // up to kPairS pairs are put in slot order in registers
+                const T *rs = m <= kYLds ? s_y : res;
+                for (int t0 = tid; t0 < c; t0 += kBlk * kCandB) {
+                    int p0[kCandB], pn[kCandB], jc[kCandB];
 #pragma unroll
-                    for (int u = 0; u < kPairB; ++u) {
+                    for (int u = 0; u < kCandB; ++u) {
                         const int t = min(t0 + u * kBlk, c - 1);
                         p0[u] = cnt[t];
-                        p1[u] = cnt[t + 1];
+                        pn[u] = cnt[t + 1];
                         jc[u] = cand[t];
                     }
+                    int rw[kCandB][kPairS];
+                    T vw[kCandB][kPairS], xw[kCandB][kPairS];
 #pragma unroll
-                    for (int u = 0; u < kPairB; ++u) {
-                        const bool any = p1[u] > p0[u];
-                        r0[u] = any ? pr[p0[u]] : 0;
-                        v0[u] = any ? pv[p0[u]] : T(0);
+                    for (int u = 0; u < kCandB; ++u) {
+                        pn[u] -= p0[u];
+#pragma unroll
+                        for (int w = 0; w < kPairS; ++w) {
+                            const bool ok = w < pn[u];
+                            rw[u][w] = ok ? pr[p0[u] + w] : INT_MAX;
+                            vw[u][w] = ok ? pv[p0[u] + w] : T(0);
+                        }
                     }
 #pragma unroll
-                    for (int u = 0; u < kPairB; ++u)
-                        x0[u] = res[r0[u]];
+                    for (int u = 0; u < kCandB; ++u) {
+                        // sorting network on four (slot, value) pairs
+                        auto cswap = [&](int x, int y) {
+                            const bool sw = rw[u][x] > rw[u][y];
+                            const int ra = rw[u][x], rb2 = rw[u][y];
+                            const T va = vw[u][x], vb2 = vw[u][y];
+                            rw[u][x] = sw ? rb2 : ra;
+                            rw[u][y] = sw ? ra : rb2;
+                            vw[u][x] = sw ? vb2 : va;
+                            vw[u][y] = sw ? va : vb2;
+                        };
+                        static_assert(kPairS == 4, "four-input sorting network");
+                        cswap(0, 1);
+                        cswap(2, 3);
+                        cswap(0, 2);
+                        cswap(1, 3);
+                        cswap(1, 2);
 #pragma unroll
-                    for (int u = 0; u < kPairB; ++u) {
+                        for (int w = 0; w < kPairS; ++w)
+                            xw[u][w] = rw[u][w] != INT_MAX ? rs[rw[u][w]] : T(0);
+                    }
+#pragma unroll
+                    for (int u = 0; u < kCandB; ++u) {
                         const int t = t0 + u * kBlk;
                         if (t >= c)
                             continue;
                         T dt = 0, den = 0;
-                        if (p1[u] - p0[u] == 1) {
-                            dt = x0[u] * v0[u];
-                            den = v0[u] * v0[u];
+                        if (pn[u] <= kPairS) {
+#pragma unroll
+                            for (int w = 0; w < kPairS; ++w)
+                                if (w < pn[u]) {
+                                    dt += xw[u][w] * vw[u][w];
+                                    den += vw[u][w] * vw[u][w];
+                                }
                         } else {
-                            const int q0 = p0[u], q1 = p1[u];
-                            for (int a = q0 + 1; a < q1; ++a) { // by slot (few pairs each)
+                            const int q0 = p0[u], q1 = p0[u] + pn[u];
+                            for (int a = q0 + 1; a < q1; ++a) { // by slot
                                 const int rr0 = pr[a];
                                 const T vv0 = pv[a];
                                 int b = a - 1;
@@ -2782,7 +2839,7 @@ spai_block_kernel(int n, const int *__restrict__ aptr, const int *__restrict__ a
                             }
                             for (int a = q0; a < q1; ++a) {
                                 const T value = pv[a];
-                                dt += res[pr[a]] * value;
+                                dt += rs[pr[a]] * value;
                                 den += value * value;
                             }
                         }

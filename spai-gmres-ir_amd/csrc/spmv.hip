@@ -60,6 +60,10 @@ spmv_group_kernel(Tiles tiles, const int *__restrict__ flag, int flag_stride,
 // With C16 the columns come from 16-bit offsets to the block's first row
 // (col16[e] = col[e] - rbeg, see encode_col16_kernel): 2 bytes per entry
 // instead of 4, the same column and the same sums.
+// A block's path is a chain of dependent loads (entry range, entries, x
+// gathers), so nothing else is put in front of it: one system's block
+// geometry is arithmetic, the run flag loads beside the entry range, and a
+// replayed graph's x is looked up behind the entry loads.
 template <typename TV, typename TX, typename TA, typename TY, int PER, bool C16>
 __global__ void __launch_bounds__(kSpmvRows)
 spmv_stream_kernel(Tiles tiles, const int *__restrict__ flag, int flag_stride,
@@ -67,25 +71,36 @@ spmv_stream_kernel(Tiles tiles, const int *__restrict__ flag, int flag_stride,
                    const short *__restrict__ col16, const TV *__restrict__ val,
                    const TX *__restrict__ x, TY *__restrict__ y, XSel xs) {
     constexpr int kStreamChunk = kSpmvRows * PER;
-    __shared__ int rp[kSpmvRows + 1];
     __shared__ TA prod[kStreamChunk];
     const int b = xcd_block(blockIdx.x, gridDim.x);
-    const int t = b / kSpmvSub, sb = b % kSpmvSub;
-    if (flag && !flag[(std::int64_t)tiles.sys[t] * flag_stride])
-        return;
-    if (xs.tab)
-        x = static_cast<const TX *>(xs.tab[min(*xs.idx + xs.off, xs.last)]);
-    const int rbeg = tiles.row0[t] + sb * kSpmvRows;
-    const int rend = min(tiles.row1[t], rbeg + kSpmvRows);
-    if (rbeg >= rend)
-        return;
+    int rbeg, rend, e0, e1;
+    if (tiles.nsys == 1) {
+        // one system: tile t is rows [t kTileRows, ...), so block b starts at
+        // row 256 b; the entry range and the run flag load together (both
+        // in bounds either way)
+        rbeg = b * kSpmvRows;
+        rend = min(tiles.nrows, rbeg + kSpmvRows);
+        if (rbeg >= rend)
+            return;
+        e0 = ptr[rbeg];
+        e1 = ptr[rend];
+        if (flag && !flag[0])
+            return;
+    } else {
+        const int t = b / kSpmvSub, sb = b % kSpmvSub;
+        if (flag && !flag[(std::int64_t)tiles.sys[t] * flag_stride])
+            return;
+        rbeg = tiles.row0[t] + sb * kSpmvRows;
+        rend = min(tiles.row1[t], rbeg + kSpmvRows);
+        if (rbeg >= rend)
+            return;
+        e0 = ptr[rbeg];
+        e1 = ptr[rend];
+    }
     const int nr = rend - rbeg;
-    for (int i = threadIdx.x; i <= nr; i += kSpmvRows)
-        rp[i] = ptr[rbeg + i];
-    __syncthreads();
-    const int e0 = rp[0], e1 = rp[nr];
     const int tid = threadIdx.x;
-    const int mb = tid < nr ? rp[tid] : 0, me = tid < nr ? rp[tid + 1] : 0;
+    // (each thread's row bounds load alongside; the sums need them)
+    const int mb = tid < nr ? ptr[rbeg + tid] : 0, me = tid < nr ? ptr[rbeg + tid + 1] : 0;
     TA acc = 0;
     constexpr int per = PER;
     for (int c0 = e0; c0 < e1; c0 += kStreamChunk) {
@@ -103,6 +118,8 @@ spmv_stream_kernel(Tiles tiles, const int *__restrict__ flag, int flag_stride,
                 cj[u] = col[e];
             vv[u] = val[e];
         }
+        if (xs.tab && c0 == e0)
+            x = static_cast<const TX *>(xs.tab[min(*xs.idx + xs.off, xs.last)]);
         TX xv[per];
 #pragma unroll
         for (int u = 0; u < per; ++u)
