@@ -445,12 +445,33 @@ void spai_build_typed(Context &c, const DevCsr &a, const T *aval, T *d, double e
     const std::int64_t area_cap =
         std::min<std::int64_t>(8LL << 30, std::max<std::int64_t>(1LL << 28, free_bytes / 4));
     DeviceBuffer &dwork = w.dwork, &iwork = w.iwork;
-    // Waves per SIMD of the workgroup kernel: 4 for the first workgroup pass,
-    // 3 beyond; BSPAI_SPAI_BLOCK_OCC=3 or 4 sets every pass (tuning).
+    // Launch shape of the workgroup kernel per capacity bucket: the waves per
+    // SIMD its registers are held to (3 or 4) and the workgroups per CU
+    // launched (persistent: each takes columns from the bucket's worklist
+    // until it is empty).  The rule: 4 waves in the first workgroup bucket,
+    // 3 beyond, and 2 workgroups per CU (measured on the power-law buckets,
+    // whose work areas of tens of MB thrash the caches at 4 per CU: the
+    // tuning below chose 2 per CU in every large bucket).  A bucket of at
+    // least tune_min columns is tuned instead: each candidate shape fits the
+    // same slice of the worklist (one round of the largest grid) between HIP
+    // events, the
+    // fastest is kept per (n, nnz, value type, bucket) in the context, and
+    // the bucket goes on from the slice's end.  A column's fit does not
+    // depend on the workgroup that runs it, so the shape changes timing
+    // only, never a result bit.  BSPAI_SPAI_BLOCK_OCC=3 or 4 sets the waves
+    // of every bucket (no tuning); BSPAI_SPAI_AUTOTUNE=0 keeps the rule.
     int occ_env = 0;
     if (const char *env = std::getenv("BSPAI_SPAI_BLOCK_OCC"))
         occ_env = std::atoi(env);
-    auto block_occ = [&](int pass) { return occ_env == 3 || occ_env == 4 ? occ_env : (pass == 1 ? 4 : 3); };
+    if (occ_env != 3 && occ_env != 4)
+        occ_env = 0;
+    bool tune = occ_env == 0;
+    if (const char *env = std::getenv("BSPAI_SPAI_AUTOTUNE"))
+        tune = tune && std::string(env) != "0";
+    int tune_min = 16 * cus * 4;
+    if (const char *env = std::getenv("BSPAI_SPAI_TUNE_MIN"))
+        tune_min = std::max(1, std::atoi(env));
+    const std::pair<int, int> shapes[4] = {{4, 4}, {3, 3}, {4, 2}, {3, 2}};
     std::vector<std::string> pass_labels; // timer label of each capacity pass
     while (nwork > 0) {
         if (out.passes >= 8)
@@ -480,7 +501,15 @@ void spai_build_typed(Context &c, const DevCsr &a, const T *aval, T *d, double e
             const std::int64_t bd = spai_block_doubles(mcap, qcap, ccap, pcap);
             const std::int64_t bi = spai_block_ints(mcap, qcap, ccap, pcap, words, n);
             const std::int64_t per_block = bd * (std::int64_t)sizeof(T) + bi * 4;
-            std::int64_t blocks = std::min<std::int64_t>(nwork, cus * 4LL);
+            // the bucket's launch shape: forced, cached, tuned on a slice, or the rule
+            std::pair<int, int> shape = std::make_pair(occ_env ? occ_env : (pass == 1 ? 4 : 3), 2);
+            const auto key = std::make_tuple((std::int64_t)n, a.nnz, (int)sizeof(T), pass);
+            const bool cached = tune && c.spai_tuned.count(key) > 0;
+            if (cached)
+                shape = c.spai_tuned[key];
+            const bool tune_now = tune && !cached && nwork >= tune_min;
+            // work areas for the largest grid the bucket launches
+            std::int64_t blocks = std::min<std::int64_t>(nwork, (std::int64_t)cus * (tune_now ? 4 : shape.second));
             // Large buckets hold tens of MB per workgroup: allow up to a quarter
             // of free memory (48 GiB at most) so that a full wave of
             // workgroups stays resident.
@@ -499,19 +528,57 @@ void spai_build_typed(Context &c, const DevCsr &a, const T *aval, T *d, double e
                             out.c_clamps.data(), out.c_q.data(),     out.c_res.data(),
                             out.c_piv.data(),    out.c_flops.data(), out.c_off.data(),
                             out.c_pass.data(),   pass};
+            auto run = [&](int occ, std::int64_t nb, int count) {
+                launch_spai_block<T>(s, (int)nb, n, a.ptr.data(), a.col.data(), aval, d,
+                                     bptr.data(), bcol.data(), need_bval ? bval : nullptr,
+                                     (T)eps, alpha, beta, work.data(), count, counter.data(),
+                                     mcap, qcap, ccap, dwork.as<T>(), iwork.as<int>(), o,
+                                     out.pool_j.back().data(), out.pool_c.back().as<T>(), qcap,
+                                     occ, pcap);
+            };
+            int start = 0; // worklist slots already fitted (by the tuning)
+            if (tune_now) {
+                const int slice = (int)std::min<std::int64_t>(nwork, blocks);
+                hipEvent_t t0, t1;
+                HIP_CHECK(hipEventCreate(&t0));
+                HIP_CHECK(hipEventCreate(&t1));
+                float best_ms = 0;
+                bool first = true;
+                for (const auto &cand : shapes) {
+                    const std::int64_t nb =
+                        std::min<std::int64_t>(blocks, (std::int64_t)cus * cand.second);
+                    HIP_CHECK(hipMemsetAsync(counter.data(), 0, sizeof(int), s));
+                    HIP_CHECK(hipEventRecord(t0, s));
+                    run(cand.first, nb, slice);
+                    HIP_CHECK(hipEventRecord(t1, s));
+                    HIP_CHECK(hipEventSynchronize(t1));
+                    float ms = 0;
+                    HIP_CHECK(hipEventElapsedTime(&ms, t0, t1));
+                    if (first || ms < best_ms) {
+                        best_ms = ms;
+                        shape = cand;
+                        first = false;
+                    }
+                }
+                HIP_CHECK(hipEventDestroy(t0));
+                HIP_CHECK(hipEventDestroy(t1));
+                c.spai_tuned[key] = shape;
+                start = slice;
+            }
+            const std::int64_t nb = std::min<std::int64_t>(blocks, (std::int64_t)cus * shape.second);
+            if (start > 0) {
+                static thread_local int start_host = 0;
+                start_host = start;
+                upload(c, counter.data(), &start_host, 1);
+                sync(c);
+            }
             const std::string label = std::string(kSpaiBlockKernelName<T>) + " pass " +
                                       std::to_string(pass) + " (" + std::to_string(nwork) +
-                                      " columns)";
+                                      " columns, occ " + std::to_string(shape.first) + ", " +
+                                      std::to_string(shape.second) + "/CU" +
+                                      (start > 0 ? ", tuned" : "") + ")";
             pass_labels.push_back(label);
-            timed(c, label.c_str(), 0.0, 0.0, [&] {
-                launch_spai_block<T>(s, (int)blocks, n, a.ptr.data(), a.col.data(), aval, d,
-                                     bptr.data(), bcol.data(), need_bval ? bval : nullptr,
-                                     (T)eps, alpha, beta, work.data(),
-                                     nwork, counter.data(), mcap, qcap, ccap, dwork.as<T>(),
-                                     iwork.as<int>(), o, out.pool_j.back().data(),
-                                     out.pool_c.back().as<T>(), qcap, block_occ(pass),
-                                     pcap);
-            }, true);
+            timed(c, label.c_str(), 0.0, 0.0, [&] { run(shape.first, nb, nwork); }, true);
             hipLaunchKernelGGL(collect_overflow_kernel, dim3(ceil_div(nwork, 256)), dim3(256), 0,
                                s, nwork, work.data(), out.c_status.data(), next.data(),
                                counter.data() + 1);

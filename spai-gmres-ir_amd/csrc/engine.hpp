@@ -175,6 +175,9 @@ struct Context {
     KernelTimer timer;
     // SpMV autotuning results by matrix shape (n, nnz, value bytes)
     std::map<std::tuple<std::int64_t, std::int64_t, int>, SpmvChoice> spmv_tuned;
+    // SPAI capacity-bucket launch shapes by (n, nnz, value bytes, bucket):
+    // waves per SIMD the workgroup kernel is compiled for, workgroups per CU
+    std::map<std::tuple<std::int64_t, std::int64_t, int, int>, std::pair<int, int>> spai_tuned;
     CanonWork canon;
     SpaiWork spai;
     EngineSlot<double> engine64;

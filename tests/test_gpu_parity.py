@@ -563,3 +563,42 @@ def test_graph_replay_same_bits(monkeypatch, case):
     assert np.array_equal(out["0"][1], out["1"][1])
     assert out["0"][2] == (0, 0)
     assert out["1"][2][1] > 0  # graphs were replayed
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_spai_bucket_autotune_keeps_bits(monkeypatch, precision):
+    """The capacity-bucket autotuner times launch shapes of the workgroup SPAI
+    kernel on a slice of a bucket's worklist and fits the rest with the
+    fastest; a column's fit does not depend on the shape.  Power law
+    n = 5,000 with the tuning threshold lowered so that it tunes: patterns,
+    coefficients and per-column records equal the untuned build bit for bit,
+    and a second build of the same shape reuses the cached shape."""
+    import bspai_hip
+
+    P = _problems()
+    a = P.power_law(n=5000)
+    out = {}
+    for mode in ("0", "1"):
+        monkeypatch.setenv("BSPAI_SPAI_AUTOTUNE", mode)
+        monkeypatch.setenv("BSPAI_SPAI_TUNE_MIN", "64")
+        c = bspai_hip.Context(0)
+        m = bspai_hip.Matrix(a, c)
+        c.set_timing("spai")
+        s = bspai_hip.construct_spai(m, 0.2, -(-a.n // 8), 8, precision=precision)
+        labels = list(c.kernel_times())
+        s2 = bspai_hip.construct_spai(m, 0.2, -(-a.n // 8), 8, precision=precision)
+        labels2 = list(c.kernel_times())
+        c.set_timing(False)
+        assert s.status == s2.status == "success" and s.passes >= 2
+        if mode == "1":
+            assert any(k.endswith(", tuned)") for k in labels), labels
+            assert not any(k.endswith(", tuned)") for k in labels2[len(labels):]), labels2
+        out[mode] = (s, s2)
+    ref = out["0"][0]
+    for s in (out["1"][0], out["1"][1], out["0"][1]):
+        ptr, col, val = s.preconditioner()
+        rp, rc, rv = ref.preconditioner()
+        assert np.array_equal(ptr, rp) and np.array_equal(col, rc)
+        assert np.array_equal(np.asarray(val).view(np.uint64), np.asarray(rv).view(np.uint64))
+        for k, v in s.columns().items():
+            assert np.array_equal(np.asarray(v), np.asarray(ref.columns()[k])), k
