@@ -841,6 +841,36 @@ __device__ inline float wave_max_f(float v) {
     swap32(x, y);
     return __uint_as_float(x) > __uint_as_float(y) ? __uint_as_float(x) : __uint_as_float(y);
 }
+// Minimum over the wave of 64-bit keys (every lane ends with the same key):
+// the stages of wave_sum_f on both halves, the smaller key kept at each.
+template <int kCtrl> __device__ inline unsigned long long dpp_u64(unsigned long long v) {
+    return ((unsigned long long)dpp_u<kCtrl>((unsigned)(v >> 32)) << 32) | dpp_u<kCtrl>((unsigned)v);
+}
+__device__ inline unsigned long long wave_min_u64(unsigned long long v) {
+    unsigned long long o = dpp_u64<kDppXor1>(v);
+    v = o < v ? o : v;
+    o = dpp_u64<kDppXor2>(v);
+    v = o < v ? o : v;
+    o = dpp_u64<kDppXor7>(v);
+    v = o < v ? o : v;
+    o = dpp_u64<kDppXor8>(v);
+    v = o < v ? o : v;
+#pragma unroll
+    for (int st = 0; st < 2; ++st) {
+        unsigned xl = (unsigned)v, yl = xl, xh = (unsigned)(v >> 32), yh = xh;
+        if (st == 0) {
+            swap16(xl, yl);
+            swap16(xh, yh);
+        } else {
+            swap32(xl, yl);
+            swap32(xh, yh);
+        }
+        const unsigned long long a = ((unsigned long long)xh << 32) | xl,
+                                 b = ((unsigned long long)yh << 32) | yl;
+        v = a < b ? a : b;
+    }
+    return v;
+}
 __device__ inline float lane_value(float v, int src) {
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), src));
 }
@@ -1175,11 +1205,15 @@ spai_wave_kernel(const int *__restrict__ aptr, const int *__restrict__ acol,
                 }
                 const float xj = lane_value(x, j);
                 const float diagonal = -copysignf(length, xj);
-                const float vnorm = sqrtf(2.f + 2.f * fabsf(xj) / length);
-                float v = act ? x / length : 0.f;
+                // reciprocals of the step's two (wave-uniform) scales by the
+                // hardware rcp (1 ulp; the FP32 build's order and rounding are
+                // its own) in place of per-lane divisions
+                const float inv_len = __builtin_amdgcn_rcpf(length);
+                const float vnorm = sqrtf(2.f + 2.f * fabsf(xj) * inv_len);
+                float v = act ? x * inv_len : 0.f;
                 if (lane == j)
                     v += copysignf(1.f, v);
-                v = v / vnorm;
+                v = v * __builtin_amdgcn_rcpf(vnorm);
                 // v is 0 outside rows j..m-1, so the products need no mask;
                 // the sums of columns c <= j are formed but never read.
                 float p[kWQ];
@@ -1216,12 +1250,16 @@ spai_wave_kernel(const int *__restrict__ aptr, const int *__restrict__ acol,
                     fit_status = kSpaiNumericalFailure;
             }
             // back substitution, one coefficient per lane (lane j: coef_j)
+            // The status is that of the first failing step (from j = q-1
+            // down), as a check per step would give: each lane j records its
+            // own, and the largest failing j decides after the loop (a failed
+            // step's non-finite values reach only later, smaller j).
             float coef = 0.f;
             if (fit_status == kSpaiSuccess) {
                 float acc = rh;
+                int st = kSpaiSuccess;
 #pragma unroll 1
                 for (int j = q - 1; j >= 0; --j) {
-                    int st = kSpaiSuccess;
                     float cj = 0.f;
                     if (lane == j) {
                         if (!isfinite(dg))
@@ -1235,11 +1273,6 @@ spai_wave_kernel(const int *__restrict__ aptr, const int *__restrict__ acol,
                         }
                         coef = cj;
                     }
-                    st = __builtin_amdgcn_readlane(st, j);
-                    if (st != kSpaiSuccess) {
-                        fit_status = st;
-                        break;
-                    }
                     cj = lane_value(cj, j);
                     float rj = 0.f;
 #pragma unroll
@@ -1248,6 +1281,9 @@ spai_wave_kernel(const int *__restrict__ aptr, const int *__restrict__ acol,
                     if (lane < j)
                         acc -= rj * cj;
                 }
+                const unsigned long long badm = __ballot(st != kSpaiSuccess);
+                if (badm != 0ull)
+                    fit_status = __builtin_amdgcn_readlane(st, 63 - __clzll((long long)badm));
             }
             if (fit_status != kSpaiSuccess) {
                 status = fit_status;
@@ -1314,6 +1350,7 @@ spai_wave_kernel(const int *__restrict__ aptr, const int *__restrict__ acol,
             // ---- restricted scores, from the sorted terms
             const float rr = rnorm * rnorm;
             int first_bad = INT_MAX, bad_status = kSpaiSuccess, cl_before[kWC / kWave];
+            float msum_l = 0.f; // the lane's scores, in its order
 #pragma unroll
             for (int u = 0; u < kWC / kWave; ++u) {
                 const int t = lane + u * kWave;
@@ -1338,6 +1375,7 @@ spai_wave_kernel(const int *__restrict__ aptr, const int *__restrict__ acol,
                         }
                     } else {
                         cl_before[u] = cl ? 1 : 0;
+                        msum_l += sv;
                     }
                 }
             }
@@ -1356,10 +1394,8 @@ spai_wave_kernel(const int *__restrict__ aptr, const int *__restrict__ acol,
                 break;
             }
             wave_sync();
-            float msum = 0.f;
-            for (int t = 0; t < nc; ++t) // sequential, as the lane kernel
-                msum += W.score[t];
-            const float mean = msum / (float)nc;
+            // the mean: each lane's partial sum, then a fixed butterfly
+            const float mean = wave_sum_f(msum_l) / (float)nc;
             if (!isfinite(mean)) {
                 status = kSpaiNumericalFailure;
                 break;
@@ -1397,11 +1433,7 @@ spai_wave_kernel(const int *__restrict__ aptr, const int *__restrict__ acol,
 #pragma unroll
                 for (int u = 1; u < kWC / kWave; ++u)
                     lo = skey[u] < lo ? skey[u] : lo;
-#pragma unroll
-                for (int off = kWave / 2; off > 0; off >>= 1) {
-                    const unsigned long long o = __shfl_xor(lo, off, kWave);
-                    lo = o < lo ? o : lo;
-                }
+                lo = wave_min_u64(lo);
 #pragma unroll
                 for (int u = 0; u < kWC / kWave; ++u)
                     if (skey[u] == lo)
