@@ -101,7 +101,9 @@ typedef struct {
     double spai_epsilon;
     int64_t spai_alpha;      /* 0 = ceil(n/8), as the solver CLI */
     int64_t spai_beta;
-    double gmres_tolerance;  /* inner tau */
+    double gmres_tolerance;  /* inner tau; BSPAI_HIP_MIXED: the floor of an
+                                adaptive tau, max(gmres_tolerance, 0.3 x the
+                                contraction the outer stop still needs) */
     int64_t gmres_max_iterations; /* 0 = n */
     int64_t max_outer_steps; /* <= 20 */
     int precision;           /* BSPAI_HIP_FP64 or BSPAI_HIP_MIXED */

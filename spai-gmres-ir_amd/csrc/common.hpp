@@ -56,6 +56,63 @@ template <int G, typename T> __device__ inline T group_sum(T v) {
     return v;
 }
 
+// ---------------------------------------------------------------- row sums
+// Accumulators of the sparse products.  Acc<TA> forms each product and the
+// running sum in TA.  Acc<Comp32> is compensated FP32 (Ogita, Rump and Oishi,
+// "Accurate sum and dot product", SIAM J. Sci. Comput. 26(6), 2005,
+// Algorithm Dot2): every operation is an FP32 operation; a product is split
+// exactly into p + e by one fused multiply-add, the sum by TwoSum, and the
+// errors gather in lo.  The row sum comes out as if accumulated in twice the
+// FP32 precision and rounded once, which keeps the digits that cancelling
+// rows (a stencil's interior rows sum to zero) lose in a plain FP32 sum.
+struct Comp32 {
+    float hi, lo;
+};
+
+template <typename TA> struct Acc {
+    using Prod = TA;
+    template <typename A, typename B> __device__ static Prod mul(A a, B b) {
+        return (TA)a * (TA)b;
+    }
+    __device__ static void add(TA &s, Prod p) { s += p; }
+    __device__ static TA zero() { return (TA)0; }
+    template <int G> __device__ static TA group(TA v) { return group_sum<G>(v); }
+    template <typename TY> __device__ static TY out(TA v) { return (TY)v; }
+};
+
+__device__ inline void two_sum(float a, float b, float &s, float &t) {
+    s = a + b;
+    const float bb = s - a;
+    t = (a - (s - bb)) + (b - bb);
+}
+
+template <> struct Acc<Comp32> {
+    using Prod = Comp32; // p = fl(a b), e = a b - p exactly
+    __device__ static Prod mul(float a, float b) {
+        const float p = a * b;
+        return Comp32{p, __builtin_fmaf(a, b, -p)};
+    }
+    __device__ static void add(Comp32 &s, Prod q) {
+        float h, t;
+        two_sum(s.hi, q.hi, h, t);
+        s.hi = h;
+        s.lo = s.lo + (t + q.lo);
+    }
+    __device__ static Comp32 zero() { return Comp32{0.f, 0.f}; }
+    template <int G> __device__ static Comp32 group(Comp32 v) {
+#pragma unroll
+        for (int off = G / 2; off > 0; off >>= 1) {
+            const Comp32 o{__shfl_xor(v.hi, off, G), __shfl_xor(v.lo, off, G)};
+            float h, t;
+            two_sum(v.hi, o.hi, h, t);
+            v.lo = (v.lo + o.lo) + t;
+            v.hi = h;
+        }
+        return v;
+    }
+    template <typename TY> __device__ static TY out(Comp32 v) { return (TY)(v.hi + v.lo); }
+};
+
 // Block-wide sum (blockDim.x a multiple of 64, <= 1024) in a fixed order.
 template <typename T> __device__ inline T block_sum(T v, T *lds) {
     v = wave_sum(v);

@@ -899,14 +899,15 @@ template <typename T> void GmresEngine<T>::ensure_basis(int count) {
 
 template <typename T>
 void GmresEngine<T>::solve(const DevCsr &m1, const DevCsr &m2, GmresMode mode, const T *rhs,
-                           const double *x0, double tol, double *x, const IrSysState *ir) {
+                           const double *x0, double tol, double *x, const IrSysState *ir,
+                           const InnerTolRule &rule) {
     hipStream_t s = c_.stream;
     const Tiles tl = L_.tiles();
     const bool left = mode == kLeft;
     args_.x = left ? x : xz_.data();
     args_.x0 = left ? x0 : nullptr;
     ensure_basis(std::min(cap_ + 1, 2));
-    gmres_reset<T>(s, L_.nsys, args_, ir, L_.s_rows.data(), tol);
+    gmres_reset<T>(s, L_.nsys, args_, ir, L_.s_rows.data(), tol, rule);
     SysState *st = st_.data();
     double *part = part_.data();
     const int *active = reinterpret_cast<const int *>(reinterpret_cast<const char *>(st) +
@@ -916,6 +917,12 @@ void GmresEngine<T>::solve(const DevCsr &m1, const DevCsr &m2, GmresMode mode, c
     const int stride = (int)(sizeof(SysState) / sizeof(int));
     const T *m1v = m1.values<T>(), *m2v = m2.values<T>();
     const SpmvChoice k1 = spmv_choice<T>(c_, m1), k2 = spmv_choice<T>(c_, m2);
+    // The FP32 solve forms its products with A (left) or A^T (adjoint) as
+    // compensated FP32 row sums (Acc<Comp32>): rows of A cancel (a
+    // stencil's interior rows sum to zero), and plain FP32 sums there lose
+    // digits in proportion to the cancellation.  The compensated sums keep
+    // each product at one FP32 rounding, which lowers the inner iteration
+    // counts at no extra bytes.
     const T *none = nullptr;
     // ---- start (src/gmres.cpp:134-167)
     gmres_diff_partial<T, T, T>(s, tl, st, part, rhs, none, (T *)nullptr, kSlotRhs);
@@ -1012,14 +1019,23 @@ void GmresEngine<T>::solve(const DevCsr &m1, const DevCsr &m2, GmresMode mode, c
     auto step = [&](const KrylovArgs<T> &a, int k, int kmax, bool verify_now, XSel xs) {
         const int *run = run_flag(k);
         const T *vk = a.kdev ? nullptr : vptrs_[k];
-        timed(c_, l1.c_str(), spmv_bytes(m1, k1), 2.0 * (double)m1.nnz, [&] {
-            launch_spmv<T, T, T, T>(s, k1.lanes, tl, run, stride, m1.ptr.data(), m1.col.data(),
-                                    m1v, vk, t_.data(), k1.per, c16a, xs);
-        });
-        timed(c_, l2.c_str(), spmv_bytes(m2, k2), 2.0 * (double)m2.nnz, [&] {
-            launch_spmv<T, T, T, T>(s, k2.lanes, tl, run, stride, m2.ptr.data(), m2.col.data(),
-                                    m2v, t_.data(), w_.data(), k2.per, c16b);
-        });
+        // (TA: the row-sum accumulator; Comp32 for the product with A)
+        auto product = [&](const DevCsr &mm, const SpmvChoice &kc, const T *mv, const T *in,
+                           T *out, const short *c16, XSel sel, bool with_a) {
+            if constexpr (sizeof(T) == 4) {
+                if (with_a) {
+                    launch_spmv<T, T, Comp32, T>(s, kc.lanes, tl, run, stride, mm.ptr.data(),
+                                                 mm.col.data(), mv, in, out, kc.per, c16, sel);
+                    return;
+                }
+            }
+            launch_spmv<T, T, T, T>(s, kc.lanes, tl, run, stride, mm.ptr.data(), mm.col.data(),
+                                    mv, in, out, kc.per, c16, sel);
+        };
+        timed(c_, l1.c_str(), spmv_bytes(m1, k1), 2.0 * (double)m1.nnz,
+              [&] { product(m1, k1, m1v, vk, t_.data(), c16a, xs, left); });
+        timed(c_, l2.c_str(), spmv_bytes(m2, k2), 2.0 * (double)m2.nnz,
+              [&] { product(m2, k2, m2v, t_.data(), w_.data(), c16b, XSel{}, !left); });
         timed(c_, lproj.c_str(), (k + 3) * nn * sz, 2.0 * (2 * k + 1) * nn,
               [&] { gmres_project<T>(s, tl, a, k, kmax); });
         timed(c_, lred.c_str(), 2.0 * (2 * k + 1) * tl.ntiles * 8.0, 0.0,
@@ -1174,6 +1190,11 @@ GmresEngine<T> &cached_engine(Context &c, EngineSlot<T> &slot,
     return *slot.engine;
 }
 
+// Safety factor of the mixed path's adaptive inner tolerance (InnerTolRule):
+// the inner solve of a refinement step targets 0.3 x the contraction the
+// outer stop still needs, or the caller's tolerance if that is looser.
+constexpr double kMixedInnerTheta = 0.3;
+
 std::vector<IrResultHost> ir_run(Context &c, const DevCsr &a64, const DevCsr &m1, const DevCsr &m2,
                                  GmresMode mode, const double *b, const IrOptionsC &o, double *x) {
     hipStream_t s = c.stream;
@@ -1243,7 +1264,13 @@ std::vector<IrResultHost> ir_run(Context &c, const DevCsr &a64, const DevCsr &m1
             ir_absorb_inner<double>(s, nsys, g64->args(), a, 0);
         } else {
             ir_scale_rhs(s, tl, r.data(), rf.data(), a);
-            g32->solve(m1, m2, mode, rf.data(), nullptr, o.tol, d.data(), st);
+            // The caller's tolerance is the FP32 solve's floor; a step that
+            // needs less contraction to meet the outer stop asks for less.
+            InnerTolRule rule;
+            rule.theta = kMixedInnerTheta;
+            rule.be_tol = a.be_tol;
+            rule.relres_target = a.relres_target;
+            g32->solve(m1, m2, mode, rf.data(), nullptr, o.tol, d.data(), st, rule);
             ir_absorb_inner<float>(s, nsys, g32->args(), a, 1);
         }
         ir_update(s, tl, x, d.data(), a);

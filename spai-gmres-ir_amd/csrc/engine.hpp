@@ -279,8 +279,9 @@ template <typename T> class GmresEngine {
     GmresEngine &operator=(const GmresEngine &) = delete;
     // m1 then m2 applied per Arnoldi step: left (A, P), right (P^T, A^T).
     // rhs in T; x (FP64) receives the solution; ir (optional) masks systems.
+    // rule (with ir): per-system adaptive tolerance, see InnerTolRule.
     void solve(const DevCsr &m1, const DevCsr &m2, GmresMode mode, const T *rhs, const double *x0,
-               double tol, double *x, const IrSysState *ir);
+               double tol, double *x, const IrSysState *ir, const InnerTolRule &rule = {});
     std::vector<GmresOutcome> outcomes();
     KrylovArgs<T> args() const { return args_; }
     int cap() const { return cap_; }

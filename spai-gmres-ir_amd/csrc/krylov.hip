@@ -1073,12 +1073,25 @@ finish_kernel(Tiles tiles, KrylovArgs<T> a) {
 
 template <typename T>
 __global__ void gmres_reset_kernel(int nsys, KrylovArgs<T> k, const IrSysState *ir,
-                                   const int *nrows, double tol) {
+                                   const int *nrows, double tol, InnerTolRule rule) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= nsys)
         return;
     SysState st = {};
     st.tol = tol;
+    if (ir && rule.theta > 0) {
+        // Adaptive inner tolerance: the contraction this refinement step
+        // still needs to meet the outer stop (backward error, and the
+        // relative residual when asked for), times a safety factor; never
+        // tighter than the caller's tolerance, never looser than 1/2.
+        const IrSysState &q = ir[s];
+        double need = q.nbe > 0 ? rule.be_tol / q.nbe : 1.0;
+        if (rule.relres_target > 0 && q.r2 > 0 && q.bnorm > 0)
+            need = fmin(need, rule.relres_target / (q.r2 / q.bnorm));
+        const double t = rule.theta * need;
+        if (t == t)
+            st.tol = fmin(0.5, fmax(tol, t));
+    }
     st.nrows = nrows[s];
     // Each system stops at its own cap: the workspace's cap (the caller's, or
     // the largest system) bounded by its own dimension, as a single solve of
@@ -1191,9 +1204,9 @@ void gmres_decide(hipStream_t s, const Tiles &tl, const KrylovArgs<T> &a, int ph
 }
 template <typename T>
 void gmres_reset(hipStream_t s, int nsys, const KrylovArgs<T> &k, const IrSysState *ir,
-                 const int *nrows, double tol) {
+                 const int *nrows, double tol, const InnerTolRule &rule) {
     hipLaunchKernelGGL(gmres_reset_kernel<T>, dim3(ceil_div(nsys, 64)), dim3(64), 0, s, nsys, k,
-                       ir, nrows, tol);
+                       ir, nrows, tol, rule);
     HIP_LAUNCH_CHECK();
 }
 template <typename TV, typename TU>
@@ -1227,7 +1240,7 @@ void gmres_diff_partial(hipStream_t s, const Tiles &tl, SysState *st, double *pa
     template void gmres_finish<T>(hipStream_t, const Tiles &, const KrylovArgs<T> &);             \
     template void gmres_decide<T>(hipStream_t, const Tiles &, const KrylovArgs<T> &, int, int);   \
     template void gmres_reset<T>(hipStream_t, int, const KrylovArgs<T> &, const IrSysState *,     \
-                                 const int *, double);                                            \
+                                 const int *, double, const InnerTolRule &);                      \
     template void gmres_spmv_resid<T, T>(hipStream_t, const Tiles &, SysState *, double *,        \
                                          const int *, const int *, const T *, const double *,     \
                                          const T *, double *, int, int);                          \

@@ -122,6 +122,13 @@ struct IrSysState {
     int history_len;
 };
 
+// Inner tolerance of a refinement step (theta > 0): the larger of the
+// caller's tolerance and theta times the contraction the step still needs
+// (be_tol / backward error, and relres_target / relative residual when set).
+struct InnerTolRule {
+    double theta = 0, be_tol = 0, relres_target = 0;
+};
+
 struct IrArgs {
     IrSysState *st;
     double *part;    // kIrNumSlots slots x 2 doubles per tile (norm partials)
@@ -162,7 +169,7 @@ template <typename T>
 void gmres_decide(hipStream_t s, const Tiles &tl, const KrylovArgs<T> &a, int phase, int right);
 template <typename T>
 void gmres_reset(hipStream_t s, int nsys, const KrylovArgs<T> &k, const IrSysState *ir,
-                 const int *nrows, double tol);
+                 const int *nrows, double tol, const InnerTolRule &rule);
 template <typename TV, typename TU>
 void gmres_spmv_resid(hipStream_t s, const Tiles &tl, SysState *st, double *part, const int *ptr,
                       const int *col, const TV *val, const double *x, const TU *u, double *y,

@@ -41,13 +41,13 @@ spmv_group_kernel(Tiles tiles, const int *__restrict__ flag, int flag_stride,
     const int groups = kSpmvRows / LANES;
     const int g = threadIdx.x / LANES, lane = threadIdx.x % LANES;
     for (int row = rbeg + g; row < rend; row += groups) {
-        TA acc = 0;
+        TA acc = Acc<TA>::zero();
         const int e1 = ptr[row + 1];
         for (int e = ptr[row] + lane; e < e1; e += LANES)
-            acc += (TA)val[e] * (TA)x[col[e]];
-        acc = group_sum<LANES>(acc);
+            Acc<TA>::add(acc, Acc<TA>::mul(val[e], x[col[e]]));
+        acc = Acc<TA>::template group<LANES>(acc);
         if (lane == 0)
-            y[row] = (TY)acc;
+            y[row] = Acc<TA>::template out<TY>(acc);
     }
 }
 
@@ -71,7 +71,7 @@ spmv_stream_kernel(Tiles tiles, const int *__restrict__ flag, int flag_stride,
                    const short *__restrict__ col16, const TV *__restrict__ val,
                    const TX *__restrict__ x, TY *__restrict__ y, XSel xs) {
     constexpr int kStreamChunk = kSpmvRows * PER;
-    __shared__ TA prod[kStreamChunk];
+    __shared__ typename Acc<TA>::Prod prod[kStreamChunk];
     const int b = xcd_block(blockIdx.x, gridDim.x);
     int rbeg, rend, e0, e1;
     if (tiles.nsys == 1) {
@@ -101,7 +101,7 @@ spmv_stream_kernel(Tiles tiles, const int *__restrict__ flag, int flag_stride,
     const int tid = threadIdx.x;
     // (each thread's row bounds load alongside; the sums need them)
     const int mb = tid < nr ? ptr[rbeg + tid] : 0, me = tid < nr ? ptr[rbeg + tid + 1] : 0;
-    TA acc = 0;
+    TA acc = Acc<TA>::zero();
     constexpr int per = PER;
     for (int c0 = e0; c0 < e1; c0 += kStreamChunk) {
         const int c1 = min(e1, c0 + kStreamChunk);
@@ -128,16 +128,16 @@ spmv_stream_kernel(Tiles tiles, const int *__restrict__ flag, int flag_stride,
         for (int u = 0; u < per; ++u) {
             const int e = c0 + tid + u * kSpmvRows;
             if (e < c1)
-                prod[e - c0] = (TA)vv[u] * (TA)xv[u];
+                prod[e - c0] = Acc<TA>::mul(vv[u], xv[u]);
         }
         __syncthreads();
         const int lo = max(mb, c0), hi = min(me, c1);
         for (int e = lo; e < hi; ++e)
-            acc += prod[e - c0];
+            Acc<TA>::add(acc, prod[e - c0]);
         __syncthreads();
     }
     if (tid < nr)
-        y[rbeg + tid] = (TY)acc;
+        y[rbeg + tid] = Acc<TA>::template out<TY>(acc);
 }
 
 // Row blocks of the SpMV kernels: block b of tile t covers rows
@@ -223,6 +223,7 @@ BSPAI_SPMV_INST(double, double, double, double)
 BSPAI_SPMV_INST(float, float, float, float)
 BSPAI_SPMV_INST(float, double, double, double)
 BSPAI_SPMV_INST(double, float, double, double)
+BSPAI_SPMV_INST(float, float, Comp32, float)
 #undef BSPAI_SPMV_INST
 
 template <typename TS, typename TD>
