@@ -230,8 +230,16 @@ void bspai_hip_mm_destroy(bspai_hip_mm *mm);
 
 /* y = A x in FP64.  lanes: 0 = the matrix's default; 1 = streaming kernel
  * (the block's entries staged in LDS); -1 = one thread per row; 2..64 (a
- * power of two) = that many lanes per row.  1 and -1 add each row in CSR
- * order, bit-identical to the reference's CsrMatrix::multiply. */
+ * power of two) = that many lanes per row; BSPAI_HIP_SPMV_SLICED = one
+ * wavefront per 64-row slice of a sliced copy (entries column-major per
+ * slice, int32 columns), BSPAI_HIP_SPMV_SLICED16 = the same on int16 column
+ * offsets (both EINVAL when a row has more than 255 entries or the slices
+ * would hold more than 3x the entries; SLICED16 also when an offset
+ * col - row does not fit 16 bits).  1, -1 and the sliced
+ * kernels add each row in CSR order, bit-identical to the reference's
+ * CsrMatrix::multiply. */
+#define BSPAI_HIP_SPMV_SLICED 256
+#define BSPAI_HIP_SPMV_SLICED16 257
 int bspai_hip_spmv(bspai_hip_ctx *ctx, const bspai_hip_matrix *a, const double *x, double *y,
                    int where, int lanes);
 /* y = A x with FP32 values and vectors (the mixed path's inner product). */

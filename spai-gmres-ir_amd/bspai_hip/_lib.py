@@ -13,6 +13,7 @@ LIB_PATH = os.path.join(PKG_ROOT, "libbspai_hip.so")
 OK, EINVAL, EDEVICE, ENOMEM, EFORMAT = 0, 1, 2, 3, 4
 HOST, DEVICE = 0, 1
 FP64, MIXED = 0, 1
+SPMV_SLICED, SPMV_SLICED16 = 256, 257  # lanes values of bspai_hip_spmv
 MAX_OUTER = 24
 
 SPAI_STATUS = ["success", "not_attempted", "invalid_input", "rank_deficient",

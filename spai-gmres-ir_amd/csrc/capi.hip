@@ -444,7 +444,7 @@ int bspai_hip_matrix_set_values(bspai_hip_ctx *ctx, bspai_hip_matrix *m, const d
             where == BSPAI_HIP_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
         HIP_CHECK(hipMemcpyAsync(m->a.val.data(), val, m->a.nnz * sizeof(double), kind,
                                  ctx->c.stream));
-        m->a.has32 = false;
+        m->a.values_changed();
         HIP_CHECK(hipStreamSynchronize(ctx->c.stream));
     });
 }
@@ -559,16 +559,43 @@ int bspai_hip_matrix_transpose(bspai_hip_ctx *ctx, const bspai_hip_matrix *m,
     });
 }
 
+namespace {
+void check_spmv_lanes(int lanes) {
+    require(lanes == 0 || lanes == -1 || lanes == BSPAI_HIP_SPMV_SLICED ||
+                lanes == BSPAI_HIP_SPMV_SLICED16 ||
+                (lanes > 0 && lanes <= 64 && (lanes & (lanes - 1)) == 0),
+            "lanes: 0, -1, 2^k <= 64, BSPAI_HIP_SPMV_SLICED or BSPAI_HIP_SPMV_SLICED16");
+}
+// The sliced layout a lanes argument asks for (0: none), built on first use.
+int sliced_kind(const Context &c, const DevCsr &a, int lanes) {
+    if (lanes != BSPAI_HIP_SPMV_SLICED && lanes != BSPAI_HIP_SPMV_SLICED16)
+        return 0;
+    require(a.ensure_sliced(c),
+            "sliced SpMV: a row has more than 255 entries, or the slices would hold more than "
+            "3x the entries");
+    if (lanes == BSPAI_HIP_SPMV_SLICED16) {
+        require(a.s16 > 0, "sliced SpMV: a column offset col - row does not fit 16 bits");
+        return 2;
+    }
+    return 1;
+}
+} // namespace
+
 int bspai_hip_spmv(bspai_hip_ctx *ctx, const bspai_hip_matrix *a, const double *x, double *y,
                    int where, int lanes) {
     return guard(ctx, [&] {
         require(a && x && y, "null argument");
-        require(lanes == 0 || lanes == -1 || (lanes > 0 && lanes <= 64 && (lanes & (lanes - 1)) == 0),
-                "lanes: 0, -1 or 2^k <= 64");
+        check_spmv_lanes(lanes);
         Context &c = ctx->c;
+        const int kind = sliced_kind(c, a->a, lanes);
         In<double> xi(c, x, a->a.n, where);
         Out<double> yo(c, y, a->a.n, where);
-        spmv_any(c, a->a, xi.p, yo.p, lanes);
+        if (kind)
+            launch_spmv_sliced<double, double, double, double>(
+                c.stream, a->a.layout->tiles(), nullptr, 0, a->a.slice_view(kind),
+                a->a.sliced_values<double>(c), xi.p, yo.p);
+        else
+            spmv_any(c, a->a, xi.p, yo.p, lanes);
         yo.finish();
     });
 }
@@ -577,16 +604,21 @@ int bspai_hip_spmv_f32(bspai_hip_ctx *ctx, const bspai_hip_matrix *a, const floa
                        int where, int lanes) {
     return guard(ctx, [&] {
         require(a && x && y, "null argument");
-        require(lanes == 0 || lanes == -1 || (lanes > 0 && lanes <= 64 && (lanes & (lanes - 1)) == 0),
-                "lanes: 0, -1 or 2^k <= 64");
+        check_spmv_lanes(lanes);
         Context &c = ctx->c;
         DevCsr &A = const_cast<DevCsr &>(a->a);
         A.ensure32(c);
+        const int kind = sliced_kind(c, A, lanes);
         In<float> xi(c, x, A.n, where);
         Out<float> yo(c, y, A.n, where);
-        launch_spmv<float, float, float, float>(c.stream, lanes ? lanes : A.lanes,
-                                                A.layout->tiles(), nullptr, 0, A.ptr.data(),
-                                                A.col.data(), A.val32.data(), xi.p, yo.p);
+        if (kind)
+            launch_spmv_sliced<float, float, float, float>(c.stream, A.layout->tiles(), nullptr, 0,
+                                                           A.slice_view(kind),
+                                                           A.sliced_values<float>(c), xi.p, yo.p);
+        else
+            launch_spmv<float, float, float, float>(c.stream, lanes ? lanes : A.lanes,
+                                                    A.layout->tiles(), nullptr, 0, A.ptr.data(),
+                                                    A.col.data(), A.val32.data(), xi.p, yo.p);
         yo.finish();
     });
 }

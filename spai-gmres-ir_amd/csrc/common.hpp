@@ -30,6 +30,41 @@ inline void hip_check(hipError_t e, const char *what, const char *file, int line
 
 constexpr int kWave = 64; // CDNA wavefront width
 
+// ---------------------------------------------------------------- global memory
+// Loads and stores through pointers known to address device memory (HBM):
+// a pointer that reaches a kernel through memory (a table of basis-vector
+// pointers, a struct) is a generic pointer to the compiler, which then emits
+// flat instructions; flat loads count against both the vector-memory and
+// the LDS counters, so every wait for an LDS value also waits for them.
+// Through the global address space they are global_load/global_store, waited
+// for on the vector-memory counter alone.
+#define BSPAI_GLOBAL_AS __attribute__((address_space(1)))
+template <typename T> __device__ inline T gld(const T *p) {
+    return *reinterpret_cast<const BSPAI_GLOBAL_AS T *>(reinterpret_cast<std::uintptr_t>(p));
+}
+template <typename T> __device__ inline void gst(T *p, const T &v) {
+    *reinterpret_cast<BSPAI_GLOBAL_AS T *>(reinterpret_cast<std::uintptr_t>(p)) = v;
+}
+// The same for HIP's vector types (float4, double2), through a native
+// vector of the same elements.
+template <typename V> struct NativeOf {
+    using E = decltype(V{}.x);
+    typedef E type __attribute__((ext_vector_type(sizeof(V) / sizeof(E))));
+};
+template <typename V> __device__ inline V gld_vec(const V *p) {
+    using N = typename NativeOf<V>::type;
+    const N n = *reinterpret_cast<const BSPAI_GLOBAL_AS N *>(reinterpret_cast<std::uintptr_t>(p));
+    V v;
+    __builtin_memcpy(&v, &n, sizeof(V));
+    return v;
+}
+template <typename V> __device__ inline void gst_vec(V *p, const V &v) {
+    using N = typename NativeOf<V>::type;
+    N n;
+    __builtin_memcpy(&n, &v, sizeof(V));
+    *reinterpret_cast<BSPAI_GLOBAL_AS N *>(reinterpret_cast<std::uintptr_t>(p)) = n;
+}
+
 // ---------------------------------------------------------------- reductions
 template <typename T> __device__ inline T wave_sum(T v) {
 #pragma unroll

@@ -1,6 +1,8 @@
 // Host orchestration of the SPAI / GMRES / IR kernels.
 #include "engine.hpp"
 
+#include <atomic>
+
 #include <algorithm>
 #include <climits>
 #include <cmath>
@@ -82,9 +84,10 @@ const std::vector<KernelTimer::Total> &KernelTimer::collect() {
     return out;
 }
 
-std::string spmv_label(int lanes, const char *tname, const char *role) {
-    std::string k = lanes == 1 ? std::string("spmv_stream_kernel")
-                               : "spmv_group_kernel<" + std::to_string(lanes < 0 ? 1 : lanes) + ">";
+std::string spmv_label(int lanes, const char *tname, const char *role, int sliced = 0) {
+    std::string k = sliced      ? std::string("spmv_slice_kernel")
+                    : lanes == 1 ? std::string("spmv_stream_kernel")
+                                 : "spmv_group_kernel<" + std::to_string(lanes < 0 ? 1 : lanes) + ">";
     return k + "<" + tname + "> " + role;
 }
 
@@ -170,7 +173,108 @@ const short *DevCsr::ensure_col16(const Context &c) const {
     return c16 > 0 ? col16.data() : nullptr;
 }
 
-template <typename T> SpmvChoice spmv_choice(Context &c, const DevCsr &m) {
+std::uint64_t next_matrix_generation() {
+    static std::atomic<std::uint64_t> counter{0};
+    return ++counter;
+}
+
+bool DevCsr::ensure_sliced(const Context &c) const {
+    if (sliced == 0) {
+        sliced = -1;
+        s16 = -1;
+        if (nnz > 0 && layout && layout->ntiles > 0) {
+            const Tiles tl = layout->tiles();
+            const std::size_t ns = (std::size_t)tl.ntiles * (kTileRows / kWave);
+            slen.resize((std::size_t)n);
+            DeviceArray<int> width(ns), bad(1);
+            HIP_CHECK(hipMemsetAsync(bad.data(), 0, sizeof(int), c.stream));
+            launch_slice_width(c.stream, tl, ptr.data(), col.data(), slen.data(), width.data(),
+                               bad.data());
+            std::vector<int> w(ns);
+            int hb = 0;
+            download(c, w.data(), width.data(), ns);
+            download(c, &hb, bad.data(), 1);
+            sync(c);
+            std::vector<int> off(ns + 1);
+            std::int64_t acc = 0;
+            for (std::size_t i = 0; i < ns; ++i) {
+                off[i] = (int)std::min<std::int64_t>(acc, INT_MAX);
+                acc += (std::int64_t)w[i] * kWave;
+            }
+            off[ns] = (int)std::min<std::int64_t>(acc, INT_MAX);
+            // (no layout for rows longer than 255, or when the slices would
+            // hold more than 3x the entries: ragged rows gain nothing)
+            if (!(hb & 1) && acc < INT_MAX && acc <= 3 * nnz + (std::int64_t)ns * kWave) {
+                spadded = acc;
+                soff.resize(ns + 1);
+                upload(c, soff.data(), off.data(), ns + 1);
+                scol.resize((std::size_t)acc);
+                const bool fits16 = !(hb & 2);
+                if (fits16)
+                    scol16.resize((std::size_t)acc);
+                launch_slice_fill<double>(c.stream, tl, ptr.data(), col.data(), nullptr,
+                                          slice_view(1), scol.data(),
+                                          fits16 ? scol16.data() : nullptr, nullptr);
+                sync(c);
+                sliced = 1;
+                s16 = fits16 ? 1 : -1;
+            }
+        }
+    }
+    return sliced > 0;
+}
+
+SliceView DevCsr::slice_view(int kind) const {
+    SliceView v;
+    v.soff = soff.data();
+    v.slen = slen.data();
+    if (kind == 2)
+        v.scol16 = scol16.data();
+    else
+        v.scol = scol.data();
+    return v;
+}
+
+template <> const double *DevCsr::sliced_values<double>(const Context &c) const {
+    if (sliced <= 0)
+        throw std::logic_error("sliced SpMV values of a matrix without the sliced layout");
+    if (!sval_ok) {
+        sval.resize((std::size_t)spadded);
+        launch_slice_fill<double>(c.stream, layout->tiles(), ptr.data(), col.data(), val.data(),
+                                  slice_view(1), nullptr, nullptr, sval.data());
+        sval_ok = true;
+    }
+    return sval.data();
+}
+template <> const float *DevCsr::sliced_values<float>(const Context &c) const {
+    if (sliced <= 0)
+        throw std::logic_error("sliced SpMV values of a matrix without the sliced layout");
+    if (!sval32_ok) {
+        if (!has32)
+            throw std::logic_error("sliced FP32 values before the FP32 copy");
+        sval32.resize((std::size_t)spadded);
+        launch_slice_fill<float>(c.stream, layout->tiles(), ptr.data(), col.data(), val32.data(),
+                                 slice_view(1), nullptr, nullptr, sval32.data());
+        sval32_ok = true;
+    }
+    return sval32.data();
+}
+
+// A choice is tuned per matrix shape, and another matrix of that shape (A and
+// its transpose, say) can lack what it asks for: its sliced layout is built
+// here, and a form it cannot take falls back (int16 -> int32 offsets -> the
+// streaming kernel).
+SpmvChoice usable_choice(const Context &c, const DevCsr &m, SpmvChoice k) {
+    if (k.sliced && !m.ensure_sliced(c))
+        k.sliced = 0;
+    if (k.sliced == 2 && m.s16 <= 0)
+        k.sliced = 1;
+    return k;
+}
+
+// comp: the product is formed with compensated FP32 sums (the FP32 solve's
+// product with A), which the candidates are timed with.
+template <typename T> SpmvChoice spmv_choice(Context &c, const DevCsr &m, bool comp = false) {
     SpmvChoice rule{m.lanes, stream_per(m.n, m.nnz)};
     if (m.lanes != 1 || m.n == 0)
         return rule;
@@ -185,14 +289,33 @@ template <typename T> SpmvChoice spmv_choice(Context &c, const DevCsr &m) {
             return SpmvChoice{1, rule.per, true};
         return rule;
     }
-    const auto key = std::make_tuple(m.n, m.nnz, (int)sizeof(T));
+    // BSPAI_SPMV_SLICED=1 takes the sliced kernel whenever the matrix allows
+    // (int16 offsets when they fit), =0 leaves it out of the tuning.
+    const char *senv = std::getenv("BSPAI_SPMV_SLICED");
+    const bool no_sliced = senv && std::string(senv) == "0";
+    if (senv && std::string(senv) == "1") {
+        if (m.ensure_sliced(c))
+            return SpmvChoice{1, rule.per, false, m.s16 > 0 ? 2 : 1};
+        return rule;
+    }
+    comp = comp && sizeof(T) == 4;
+    const auto key =
+        std::make_tuple(m.n, m.nnz, (int)sizeof(T) + (no_sliced ? 16 : 0) + (comp ? 32 : 0));
     const auto it = c.spmv_tuned.find(key);
     if (it != c.spmv_tuned.end())
-        return it->second;
+        return usable_choice(c, m, it->second);
     const short *c16 = m.ensure_col16(c);
-    const SpmvChoice all[5] = {{1, 8}, {1, 16}, {-1, 8}, {1, 8, true}, {1, 16, true}};
-    const int ncand = c16 ? 5 : 3;
-    const SpmvChoice *cand = all;
+    std::vector<SpmvChoice> cand = {{1, 8}, {1, 16}, {-1, 8}};
+    if (c16) {
+        cand.push_back({1, 8, true});
+        cand.push_back({1, 16, true});
+    }
+    if (!no_sliced && m.ensure_sliced(c)) {
+        cand.push_back({1, 8, false, 1});
+        if (m.s16 > 0)
+            cand.push_back({1, 8, false, 2});
+    }
+    const int ncand = (int)cand.size();
     DeviceArray<T> x((std::size_t)m.n), y((std::size_t)m.n);
     HIP_CHECK(hipMemsetAsync(x.data(), 0, (std::size_t)m.n * sizeof(T), c.stream));
     hipEvent_t e0, e1;
@@ -203,10 +326,29 @@ template <typename T> SpmvChoice spmv_choice(Context &c, const DevCsr &m) {
     float best_ms = 0;
     for (int ci = 0; ci < ncand; ++ci) {
         const SpmvChoice &k = cand[ci];
+        const T *sv = k.sliced ? m.sliced_values<T>(c) : nullptr;
         auto run = [&] {
-            launch_spmv<T, T, T, T>(c.stream, k.lanes, tl, nullptr, 0, m.ptr.data(),
-                                    m.col.data(), m.values<T>(), x.data(), y.data(), k.per,
-                                    k.c16 ? c16 : nullptr);
+            if constexpr (sizeof(T) == 4) {
+                if (comp) {
+                    if (k.sliced)
+                        launch_spmv_sliced<T, T, Comp32, T>(c.stream, tl, nullptr, 0,
+                                                            m.slice_view(k.sliced), sv, x.data(),
+                                                            y.data());
+                    else
+                        launch_spmv<T, T, Comp32, T>(c.stream, k.lanes, tl, nullptr, 0,
+                                                     m.ptr.data(), m.col.data(), m.values<T>(),
+                                                     x.data(), y.data(), k.per,
+                                                     k.c16 ? c16 : nullptr);
+                    return;
+                }
+            }
+            if (k.sliced)
+                launch_spmv_sliced<T, T, T, T>(c.stream, tl, nullptr, 0, m.slice_view(k.sliced),
+                                               sv, x.data(), y.data());
+            else
+                launch_spmv<T, T, T, T>(c.stream, k.lanes, tl, nullptr, 0, m.ptr.data(),
+                                        m.col.data(), m.values<T>(), x.data(), y.data(), k.per,
+                                        k.c16 ? c16 : nullptr);
         };
         run();
         HIP_CHECK(hipEventRecord(e0, c.stream));
@@ -216,7 +358,7 @@ template <typename T> SpmvChoice spmv_choice(Context &c, const DevCsr &m) {
         HIP_CHECK(hipEventSynchronize(e1));
         float ms = 0;
         HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
-        if (&k == cand || ms < best_ms) {
+        if (ci == 0 || ms < best_ms) {
             best_ms = ms;
             best = k;
         }
@@ -884,9 +1026,15 @@ template <typename T> void GmresEngine<T>::ensure_basis(int count) {
     const int chunk = 16;
     if (count <= vcount_)
         return;
-    // Vector stride padded to 64 elements: every basis vector starts 256-byte
-    // aligned, which the 16-byte accesses of the Gram-Schmidt passes rely on.
-    const std::size_t ld = (std::size_t)((L_.n + 63) / 64 * 64);
+    // Vector stride: n rounded up to 64 elements (every basis vector starts
+    // 256-byte aligned, which the 16-byte accesses of the Gram-Schmidt passes
+    // rely on), plus a skew of 576 elements.  The passes read several vectors
+    // at the same rows at once; at a stride that is a multiple of a large
+    // power of two (n = 2^21 at 128^3) those reads fall on the same HBM
+    // channels.  Measured at 128^3 (mixed): projection 127 -> 117 us,
+    // update 116 -> 109 us per launch; FP64 projection 280 -> 247 us.
+    constexpr std::size_t kBasisSkew = 576;
+    const std::size_t ld = (std::size_t)((L_.n + 63) / 64 * 64) + kBasisSkew;
     while ((int)vptrs_.size() < count) {
         const int take = std::min(chunk, cap_ + 1 - (int)vptrs_.size());
         chunks_.emplace_back((std::size_t)take * ld);
@@ -916,7 +1064,7 @@ void GmresEngine<T>::solve(const DevCsr &m1, const DevCsr &m2, GmresMode mode, c
                                                     offsetof(SysState, triggered));
     const int stride = (int)(sizeof(SysState) / sizeof(int));
     const T *m1v = m1.values<T>(), *m2v = m2.values<T>();
-    const SpmvChoice k1 = spmv_choice<T>(c_, m1), k2 = spmv_choice<T>(c_, m2);
+    const SpmvChoice k1 = spmv_choice<T>(c_, m1, left), k2 = spmv_choice<T>(c_, m2, !left);
     // The FP32 solve forms its products with A (left) or A^T (adjoint) as
     // compensated FP32 row sums (Acc<Comp32>): rows of A cancel (a
     // stencil's interior rows sum to zero), and plain FP32 sums there lose
@@ -960,7 +1108,8 @@ void GmresEngine<T>::solve(const DevCsr &m1, const DevCsr &m2, GmresMode mode, c
     // and out (update).
     const char *tname = sizeof(T) == 8 ? "double" : "float";
     const char *role1 = left ? "A" : "P^T", *role2 = left ? "P" : "A^T";
-    const std::string l1 = spmv_label(k1.lanes, tname, role1), l2 = spmv_label(k2.lanes, tname, role2);
+    const std::string l1 = spmv_label(k1.lanes, tname, role1, k1.sliced),
+                      l2 = spmv_label(k2.lanes, tname, role2, k2.sliced);
     const std::string lproj = std::string("project_kernel<") + tname + ">";
     const std::string lred = std::string("gram_reduce_kernel<") + tname + ">";
     const std::string lsol = std::string("gram_solve_kernel<") + tname + ">";
@@ -976,6 +1125,11 @@ void GmresEngine<T>::solve(const DevCsr &m1, const DevCsr &m2, GmresMode mode, c
     };
     const short *c16a = k1.c16 ? m1.ensure_col16(c_) : nullptr;
     const short *c16b = k2.c16 ? m2.ensure_col16(c_) : nullptr;
+    const T *m1sv = k1.sliced ? m1.sliced_values<T>(c_) : nullptr;
+    const T *m2sv = k2.sliced ? m2.sliced_values<T>(c_) : nullptr;
+    // (every pointer a replayed graph bakes in is part of its key below)
+    const SliceView sv1 = k1.sliced ? m1.slice_view(k1.sliced) : SliceView{};
+    const SliceView sv2 = k2.sliced ? m2.slice_view(k2.sliced) : SliceView{};
     // Step k's kernels are gated by run[k & 1] (active and not held).
     auto run_flag = [&](int k) {
         return reinterpret_cast<const int *>(reinterpret_cast<const char *>(st) +
@@ -1020,8 +1174,20 @@ void GmresEngine<T>::solve(const DevCsr &m1, const DevCsr &m2, GmresMode mode, c
         const int *run = run_flag(k);
         const T *vk = a.kdev ? nullptr : vptrs_[k];
         // (TA: the row-sum accumulator; Comp32 for the product with A)
-        auto product = [&](const DevCsr &mm, const SpmvChoice &kc, const T *mv, const T *in,
-                           T *out, const short *c16, XSel sel, bool with_a) {
+        auto product = [&](const DevCsr &mm, const SpmvChoice &kc, const T *mv, const T *msv,
+                           const T *in, T *out, const short *c16, XSel sel, bool with_a) {
+            if (kc.sliced) {
+                const SliceView &view = &mm == &m1 ? sv1 : sv2;
+                if constexpr (sizeof(T) == 4) {
+                    if (with_a) {
+                        launch_spmv_sliced<T, T, Comp32, T>(s, tl, run, stride, view, msv, in,
+                                                            out, sel);
+                        return;
+                    }
+                }
+                launch_spmv_sliced<T, T, T, T>(s, tl, run, stride, view, msv, in, out, sel);
+                return;
+            }
             if constexpr (sizeof(T) == 4) {
                 if (with_a) {
                     launch_spmv<T, T, Comp32, T>(s, kc.lanes, tl, run, stride, mm.ptr.data(),
@@ -1033,9 +1199,9 @@ void GmresEngine<T>::solve(const DevCsr &m1, const DevCsr &m2, GmresMode mode, c
                                     mv, in, out, kc.per, c16, sel);
         };
         timed(c_, l1.c_str(), spmv_bytes(m1, k1), 2.0 * (double)m1.nnz,
-              [&] { product(m1, k1, m1v, vk, t_.data(), c16a, xs, left); });
+              [&] { product(m1, k1, m1v, m1sv, vk, t_.data(), c16a, xs, left); });
         timed(c_, l2.c_str(), spmv_bytes(m2, k2), 2.0 * (double)m2.nnz,
-              [&] { product(m2, k2, m2v, t_.data(), w_.data(), c16b, XSel{}, !left); });
+              [&] { product(m2, k2, m2v, m2sv, t_.data(), w_.data(), c16b, XSel{}, !left); });
         timed(c_, lproj.c_str(), (k + 3) * nn * sz, 2.0 * (2 * k + 1) * nn,
               [&] { gmres_project<T>(s, tl, a, k, kmax); });
         timed(c_, lred.c_str(), 2.0 * (2 * k + 1) * tl.ntiles * 8.0, 0.0,
@@ -1071,6 +1237,10 @@ void GmresEngine<T>::solve(const DevCsr &m1, const DevCsr &m2, GmresMode mode, c
             t_.data(), ax_.data(), pax_.data(), xz_.data(),
             reinterpret_cast<const void *>((std::intptr_t)(k1.lanes * 64 + k1.per)),
             reinterpret_cast<const void *>((std::intptr_t)(k2.lanes * 64 + k2.per)),
+            reinterpret_cast<const void *>((std::intptr_t)(k1.sliced * 4 + k2.sliced)), m1sv,
+            m2sv, sv1.soff, sv1.slen, sv1.scol, sv1.scol16, sv2.soff, sv2.slen, sv2.scol,
+            sv2.scol16, reinterpret_cast<const void *>((std::uintptr_t)m1.gen),
+            reinterpret_cast<const void *>((std::uintptr_t)m2.gen),
             reinterpret_cast<const void *>((std::intptr_t)mode),
             reinterpret_cast<const void *>((std::intptr_t)kmax)};
         StepGraph &g = graphs_[verify_each ? 1 : 0];

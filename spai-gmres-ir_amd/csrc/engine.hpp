@@ -120,6 +120,7 @@ class KernelTimer {
 struct SpmvChoice {
     int lanes = 1, per = 8;
     bool c16 = false; // streaming kernel on 16-bit column offsets
+    int sliced = 0;   // sliced kernel: 1 int32 columns, 2 int16 offsets
 };
 
 // Scratch of the coordinate-entry canonicalization, kept by the context and
@@ -217,8 +218,14 @@ struct Layout {
 };
 
 // Device CSR with FP64 values (and an FP32 copy made on demand).
+// Generation numbers of matrices: a new matrix (or a matrix given a new
+// structure) never shares one with an older one, even where the allocator
+// hands it the older one's addresses.
+std::uint64_t next_matrix_generation();
+
 struct DevCsr {
     std::int64_t n = 0, nnz = 0;
+    std::uint64_t gen = next_matrix_generation();
     DeviceArray<int> ptr, col;
     DeviceArray<double> val;
     DeviceArray<float> val32;
@@ -229,8 +236,29 @@ struct DevCsr {
     // c16 0 = not tried, 1 = available, -1 = an offset does not fit.
     mutable DeviceArray<short> col16;
     mutable int c16 = 0;
+    // Sliced layout for the wavefront-per-slice SpMV (structure made on
+    // first use; values filled per type on demand and again after new
+    // values).  sliced: 0 = not tried, 1 = available, -1 = a row is longer
+    // than 255 entries or the slices would hold more than 3x the entries.
+    // s16: the int16 column offsets fit (1) or not (-1).
+    mutable DeviceArray<int> soff, scol;
+    mutable DeviceArray<short> scol16;
+    mutable DeviceArray<unsigned char> slen;
+    mutable DeviceArray<double> sval;
+    mutable DeviceArray<float> sval32;
+    mutable std::int64_t spadded = 0;
+    mutable int sliced = 0, s16 = 0;
+    mutable bool sval_ok = false, sval32_ok = false;
     void ensure32(const Context &c);
     const short *ensure_col16(const Context &c) const;
+    // kind 1: int32 columns, 2: int16 offsets; false if not available.
+    bool ensure_sliced(const Context &c) const;
+    SliceView slice_view(int kind) const;
+    template <typename T> const T *sliced_values(const Context &c) const;
+    void values_changed() {
+        has32 = false;
+        sval_ok = sval32_ok = false;
+    }
     template <typename T> const T *values() const;
 };
 template <> inline const double *DevCsr::values<double>() const { return val.data(); }

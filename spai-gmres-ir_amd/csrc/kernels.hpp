@@ -133,6 +133,29 @@ template <typename TV, typename TX, typename TA, typename TY>
 void launch_spmv(hipStream_t s, int lanes, const Tiles &tiles, const int *flag, int flag_stride,
                  const int *ptr, const int *col, const TV *val, const TX *x, TY *y,
                  int stream_per = 8, const short *col16 = nullptr, XSel xs = XSel{});
+// Sliced layout of a matrix for the wavefront-per-slice SpMV (64 rows per
+// slice, entries column-major within a slice; see spmv.hip).  soff: entry
+// offset of each slice (ntiles * 32 + 1); slen: each row's length (<= 255);
+// scol (int32 columns) or scol16 (int16 offsets col - row, when set).
+struct SliceView {
+    const int *soff = nullptr;
+    const unsigned char *slen = nullptr;
+    const int *scol = nullptr;
+    const short *scol16 = nullptr;
+};
+template <typename TV, typename TX, typename TA, typename TY>
+void launch_spmv_sliced(hipStream_t s, const Tiles &tiles, const int *flag, int flag_stride,
+                        const SliceView &sv, const TV *sval, const TX *x, TY *y,
+                        XSel xs = XSel{});
+// Slice geometry: slen per row, width per slice (longest row); *bad (zeroed
+// by the caller) gets bit 0 for a row longer than 255, bit 1 for a column
+// offset outside int16.
+void launch_slice_width(hipStream_t s, const Tiles &tiles, const int *ptr, const int *col,
+                        unsigned char *slen, int *width, int *bad);
+// Fills the sliced columns (scol and/or scol16) and/or values (sval).
+template <typename TV>
+void launch_slice_fill(hipStream_t s, const Tiles &tiles, const int *ptr, const int *col,
+                       const TV *val, const SliceView &sv, int *scol, short *scol16, TV *sval);
 // col16[e] = col[e] - (first row of the SpMV row block holding entry e);
 // *bad (zeroed by the caller) is set when an offset does not fit 16 bits.
 void launch_encode_col16(hipStream_t s, const Tiles &tiles, const int *ptr, const int *col,

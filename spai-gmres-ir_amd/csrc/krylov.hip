@@ -220,13 +220,13 @@ template <typename T, bool A, bool C = false>
 __device__ inline typename Vec16<T>::type ld16(const T *p, int lim = 16) {
     using V16 = typename Vec16<T>::type;
     if constexpr (A && !C) {
-        return *reinterpret_cast<const V16 *>(p);
+        return gld_vec(reinterpret_cast<const V16 *>(p));
     } else {
         V16 v;
         T *q = reinterpret_cast<T *>(&v);
 #pragma unroll
         for (int e = 0; e < (int)(16 / sizeof(T)); ++e)
-            q[e] = !C || e < lim ? p[e] : T(0);
+            q[e] = !C || e < lim ? gld(p + e) : T(0);
         return v;
     }
 }
@@ -234,13 +234,13 @@ template <typename T, bool A, bool C = false>
 __device__ inline void st16(T *p, const typename Vec16<T>::type &v, int lim = 16) {
     using V16 = typename Vec16<T>::type;
     if constexpr (A && !C) {
-        *reinterpret_cast<V16 *>(p) = v;
+        gst_vec(reinterpret_cast<V16 *>(p), v);
     } else {
         const T *q = reinterpret_cast<const T *>(&v);
 #pragma unroll
         for (int e = 0; e < (int)(16 / sizeof(T)); ++e)
             if (!C || e < lim)
-                p[e] = q[e];
+                gst(p + e, q[e]);
     }
 }
 
@@ -272,8 +272,8 @@ project_kernel(Tiles tiles, KrylovArgs<T> a, int k) {
     const T *__restrict__ w = a.w + r0;
     const T *__restrict__ vk = a.V[k] + r0;
     for (int r = threadIdx.x; r < kTileRows; r += kProjWaves * kWave) {
-        ws[r] = r < nr ? w[r] : T(0);
-        vs[r] = r < nr ? vk[r] : T(0);
+        ws[r] = r < nr ? gld(w + r) : T(0);
+        vs[r] = r < nr ? gld(vk + r) : T(0);
     }
     __syncthreads();
     const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;

@@ -44,7 +44,7 @@ spmv_group_kernel(Tiles tiles, const int *__restrict__ flag, int flag_stride,
         TA acc = Acc<TA>::zero();
         const int e1 = ptr[row + 1];
         for (int e = ptr[row] + lane; e < e1; e += LANES)
-            Acc<TA>::add(acc, Acc<TA>::mul(val[e], x[col[e]]));
+            Acc<TA>::add(acc, Acc<TA>::mul(val[e], gld(x + col[e])));
         acc = Acc<TA>::template group<LANES>(acc);
         if (lane == 0)
             y[row] = Acc<TA>::template out<TY>(acc);
@@ -123,7 +123,7 @@ spmv_stream_kernel(Tiles tiles, const int *__restrict__ flag, int flag_stride,
         TX xv[per];
 #pragma unroll
         for (int u = 0; u < per; ++u)
-            xv[u] = x[cj[u]];
+            xv[u] = gld(x + cj[u]);
 #pragma unroll
         for (int u = 0; u < per; ++u) {
             const int e = c0 + tid + u * kSpmvRows;
@@ -170,6 +170,196 @@ void launch_encode_col16(hipStream_t s, const Tiles &tiles, const int *ptr, cons
                        dim3(kSpmvRows), 0, s, tiles, ptr, col, col16, bad);
     HIP_LAUNCH_CHECK();
 }
+
+// ------------------------------------------------------------ sliced SpMV
+// Sliced layout (rows in slices of 64, one wavefront per slice): slice s of
+// tile t holds rows row0(t) + 64 (s - 32 t) ..., never crossing a tile, so
+// never a system.  A slice's entries are stored column-major -- slot
+// (j, lane) at soff[s] + 64 j + lane holds entry j of the lane's row -- up to
+// the slice's longest row (padded slots: column = the row, value 0);
+// slen[row] is the row's length.  Lane l adds its
+// row's products for j = 0, 1, ... in CSR order, the reference's sequential
+// row sum (src/cpu.cpp:88-89), bit for bit.  Every load is coalesced: the
+// entries of 64 consecutive rows are contiguous per j, and for a stencil the
+// x gathers of one j are consecutive rows too.  Columns are int32, or int16
+// offsets to the row (scol16[q] = col - row) when every entry fits.
+constexpr int kSliceRows = kWave;
+constexpr int kSlicesPerTile = kTileRows / kSliceRows;
+constexpr int kSlicesPerBlock = kSpmvRows / kSliceRows;
+
+template <typename TV, typename TX, typename TA, typename TY, bool C16>
+__global__ void __launch_bounds__(kSpmvRows)
+spmv_slice_kernel(Tiles tiles, const int *__restrict__ flag, int flag_stride, SliceView sv,
+                  const TV *__restrict__ sval, const TX *__restrict__ x, TY *__restrict__ y,
+                  XSel xs) {
+    const int b = xcd_block(blockIdx.x, gridDim.x);
+    const int sl = b * kSlicesPerBlock + (int)(threadIdx.x / kSliceRows);
+    const int lane = threadIdx.x % kSliceRows;
+    int row0, row1;
+    if (tiles.nsys == 1) {
+        row0 = sl * kSliceRows;
+        row1 = min(tiles.nrows, row0 + kSliceRows);
+        if (flag && !flag[0])
+            return;
+    } else {
+        const int t = b / kSpmvSub;
+        if (flag && !flag[(std::int64_t)tiles.sys[t] * flag_stride])
+            return;
+        row0 = tiles.row0[t] + (sl - t * kSlicesPerTile) * kSliceRows;
+        row1 = min(tiles.row1[t], row0 + kSliceRows);
+    }
+    const int row = row0 + lane;
+    if (row >= row1)
+        return;
+    const int s0 = sv.soff[sl], s1 = sv.soff[sl + 1];
+    const int len = sv.slen[row];
+    const int width = (s1 - s0) / kSliceRows; // the slice's longest row
+    const int base = s0 + lane;
+    if (xs.tab)
+        x = static_cast<const TX *>(xs.tab[min(*xs.idx + xs.off, xs.last)]);
+    TA acc = Acc<TA>::zero();
+    // U slots in flight per lane: every slot below the slice's width is
+    // loaded (a padded slot holds column = the row and value 0), and only the
+    // row's own entries are added, in order.
+    constexpr int U = 8;
+    for (int j0 = 0; j0 < width; j0 += U) {
+        int cj[U];
+        TV vv[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int q = base + min(j0 + u, width - 1) * kSliceRows;
+            if constexpr (C16)
+                cj[u] = row + (int)sv.scol16[q];
+            else
+                cj[u] = sv.scol[q];
+            vv[u] = sval[q];
+        }
+        TX xv[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            xv[u] = gld(x + cj[u]);
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (j0 + u < len)
+                Acc<TA>::add(acc, Acc<TA>::mul(vv[u], xv[u]));
+    }
+    y[row] = Acc<TA>::template out<TY>(acc);
+}
+
+template <typename TV, typename TX, typename TA, typename TY>
+void launch_spmv_sliced(hipStream_t s, const Tiles &tiles, const int *flag, int flag_stride,
+                        const SliceView &sv, const TV *sval, const TX *x, TY *y, XSel xs) {
+    if (tiles.ntiles == 0)
+        return;
+    const dim3 grid((unsigned)tiles.ntiles * kSpmvSub), block(kSpmvRows);
+    if (sv.scol16)
+        hipLaunchKernelGGL((spmv_slice_kernel<TV, TX, TA, TY, true>), grid, block, 0, s, tiles,
+                           flag, flag_stride, sv, sval, x, y, xs);
+    else
+        hipLaunchKernelGGL((spmv_slice_kernel<TV, TX, TA, TY, false>), grid, block, 0, s, tiles,
+                           flag, flag_stride, sv, sval, x, y, xs);
+    HIP_LAUNCH_CHECK();
+}
+#define BSPAI_SLICED_INST(TV, TX, TA, TY)                                                         \
+    template void launch_spmv_sliced<TV, TX, TA, TY>(hipStream_t, const Tiles &, const int *, int, \
+                                                     const SliceView &, const TV *, const TX *,   \
+                                                     TY *, XSel);
+BSPAI_SLICED_INST(double, double, double, double)
+BSPAI_SLICED_INST(float, float, float, float)
+BSPAI_SLICED_INST(float, float, Comp32, float)
+#undef BSPAI_SLICED_INST
+
+// Slice geometry: one wavefront per slice writes its rows' lengths (0 past
+// the tile) and the slice's width, the longest row; *bad is set when a row is
+// longer than 255 entries or an entry's column offset does not fit 16 bits
+// (bit 0: length, bit 1: offset).
+__global__ void __launch_bounds__(kSpmvRows)
+slice_width_kernel(Tiles tiles, const int *__restrict__ ptr, const int *__restrict__ col,
+                   unsigned char *__restrict__ slen, int *__restrict__ width, int *__restrict__ bad) {
+    const int b = blockIdx.x;
+    const int t = b / kSpmvSub;
+    const int sl = b * kSlicesPerBlock + (int)(threadIdx.x / kSliceRows);
+    const int lane = threadIdx.x % kSliceRows;
+    const int row0 = tiles.row0[t] + (sl - t * kSlicesPerTile) * kSliceRows;
+    const int row1 = min(tiles.row1[t], row0 + kSliceRows);
+    const int row = row0 + lane;
+    int len = 0, flags = 0;
+    if (row < row1) {
+        const int e0 = ptr[row], e1 = ptr[row + 1];
+        len = e1 - e0;
+        if (len > 255)
+            flags |= 1;
+        for (int e = e0; e < e1; ++e) {
+            const int d = col[e] - row;
+            if (d < -32768 || d > 32767)
+                flags |= 2;
+        }
+        slen[row] = (unsigned char)min(len, 255);
+    }
+    len = wave_max(len);
+    flags = (__any(flags & 1) ? 1 : 0) | (__any(flags & 2) ? 2 : 0);
+    if (lane == 0) {
+        width[sl] = len;
+        if (flags)
+            atomicOr(bad, flags);
+    }
+}
+
+// Entries into the sliced layout (structure: columns, both forms; values:
+// a separate pass, so new values refill only those).
+template <typename TV>
+__global__ void __launch_bounds__(kSpmvRows)
+slice_fill_kernel(Tiles tiles, const int *__restrict__ ptr, const int *__restrict__ col,
+                  const TV *__restrict__ val, SliceView sv, int *__restrict__ scol,
+                  short *__restrict__ scol16, TV *__restrict__ sval) {
+    const int b = blockIdx.x;
+    const int t = b / kSpmvSub;
+    const int sl = b * kSlicesPerBlock + (int)(threadIdx.x / kSliceRows);
+    const int lane = threadIdx.x % kSliceRows;
+    const int row0 = tiles.row0[t] + (sl - t * kSlicesPerTile) * kSliceRows;
+    const int row1 = min(tiles.row1[t], row0 + kSliceRows);
+    const int row = row0 + lane;
+    if (row >= row1)
+        return;
+    const int e0 = ptr[row], len = ptr[row + 1] - e0;
+    const int base = sv.soff[sl] + lane;
+    const int width = (sv.soff[sl + 1] - sv.soff[sl]) / kSliceRows;
+    // slots past the row's end: column = the row (offset 0), value 0, so a
+    // load of any slot below the width is a valid gather
+    for (int j = 0; j < width; ++j) {
+        const int q = base + j * kSliceRows;
+        const bool in = j < len;
+        if (scol)
+            scol[q] = in ? col[e0 + j] : row;
+        if (scol16)
+            scol16[q] = in ? (short)(col[e0 + j] - row) : (short)0;
+        if (sval)
+            sval[q] = in ? val[e0 + j] : TV(0);
+    }
+}
+
+void launch_slice_width(hipStream_t s, const Tiles &tiles, const int *ptr, const int *col,
+                        unsigned char *slen, int *width, int *bad) {
+    if (tiles.ntiles == 0)
+        return;
+    hipLaunchKernelGGL(slice_width_kernel, dim3((unsigned)tiles.ntiles * kSpmvSub),
+                       dim3(kSpmvRows), 0, s, tiles, ptr, col, slen, width, bad);
+    HIP_LAUNCH_CHECK();
+}
+template <typename TV>
+void launch_slice_fill(hipStream_t s, const Tiles &tiles, const int *ptr, const int *col,
+                       const TV *val, const SliceView &sv, int *scol, short *scol16, TV *sval) {
+    if (tiles.ntiles == 0)
+        return;
+    hipLaunchKernelGGL((slice_fill_kernel<TV>), dim3((unsigned)tiles.ntiles * kSpmvSub),
+                       dim3(kSpmvRows), 0, s, tiles, ptr, col, val, sv, scol, scol16, sval);
+    HIP_LAUNCH_CHECK();
+}
+template void launch_slice_fill<double>(hipStream_t, const Tiles &, const int *, const int *,
+                                        const double *, const SliceView &, int *, short *,
+                                        double *);
+template void launch_slice_fill<float>(hipStream_t, const Tiles &, const int *, const int *,
+                                       const float *, const SliceView &, int *, short *, float *);
 
 template <typename TV, typename TX, typename TA, typename TY>
 void launch_spmv(hipStream_t s, int lanes, const Tiles &tiles, const int *flag, int flag_stride,
