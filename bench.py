@@ -65,7 +65,7 @@ def measured_traffic(kernel):
         text = open(TRAFFIC_PROFILE).read()
     except OSError:
         return None
-    for name, val in re.findall(r"^traffic_bytes_per_launch (\S+) (\S+)$", text, re.M):
+    for name, val in re.findall(r"^traffic_bytes_per_launch (.+) (\S+)$", text, re.M):
         if name == kernel:
             return float(val)
     return None
