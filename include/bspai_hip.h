@@ -16,6 +16,7 @@
  *   bspai_hip_matrix_create / bspai_hip_spmv    CudaCsrSpmv      include/bspai/cuda_spmv.hpp:7-19
  *   bspai_hip_csr_from_entries                  CsrMatrix::from_entries include/bspai/cpu.hpp:19
  *   bspai_hip_mm_parse (+ _info/_entries)       read_matrix_market include/bspai/cpu.hpp:47
+ *   bspai_hip_mm_parse_gpu (+ _gpu_info/_entries/_copy)   the same, parsed on the GPU
  *                                               (the parse; canonicalization is
  *                                               bspai_hip_csr_from_entries)
  *   bspai_hip_spai_build (+ _info/_columns/...) cuda_construct_spai include/bspai/cuda_spai.hpp:47
@@ -227,6 +228,28 @@ int bspai_hip_mm_info(const bspai_hip_mm *mm, int64_t *rows, int64_t *cols, int6
 /* row, col, val: room for `count` entries each (host arrays). */
 int bspai_hip_mm_entries(const bspai_hip_mm *mm, int64_t *row, int64_t *col, double *val);
 void bspai_hip_mm_destroy(bspai_hip_mm *mm);
+
+/* The same parse on the GPU (replaces read_matrix_market, src/cpu.cpp:105-161,
+ * like bspai_hip_mm_parse; the reference reader is host code): the banner and
+ * the dimension line are read on the host, the body is copied to the device
+ * once and its lines are found, parsed and put in file order there, with the
+ * same rules, the same values bit for bit and the same messages (BSPAI_HIP_EFORMAT).
+ * The entries stay on the device until bspai_hip_mm_gpu_destroy.  text: host
+ * bytes; bodies of up to 2^31 - 3 lines. */
+typedef struct bspai_hip_mm_gpu bspai_hip_mm_gpu;
+int bspai_hip_mm_parse_gpu(bspai_hip_ctx *ctx, const char *text, int64_t len,
+                           bspai_hip_mm_gpu **out, char *message, int message_len);
+int bspai_hip_mm_gpu_info(const bspai_hip_mm_gpu *mm, int64_t *rows, int64_t *cols,
+                          int64_t *count, int *integer_field, int *symmetric);
+/* Device pointers to the `count` entries (zero-based int64 indices, FP64
+ * values), valid until bspai_hip_mm_gpu_destroy: pass them to
+ * bspai_hip_csr_from_entries with BSPAI_HIP_DEVICE. */
+int bspai_hip_mm_gpu_entries(const bspai_hip_mm_gpu *mm, const int64_t **row, const int64_t **col,
+                             const double **val);
+/* The entries copied into caller arrays in `where` memory. */
+int bspai_hip_mm_gpu_copy(bspai_hip_ctx *ctx, const bspai_hip_mm_gpu *mm, int64_t *row,
+                          int64_t *col, double *val, int where);
+void bspai_hip_mm_gpu_destroy(bspai_hip_mm_gpu *mm);
 
 /* y = A x in FP64.  lanes: 0 = the matrix's default; 1 = streaming kernel
  * (the block's entries staged in LDS); -1 = one thread per row; 2..64 (a

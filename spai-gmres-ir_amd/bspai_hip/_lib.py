@@ -105,6 +105,12 @@ SIGNATURES = {
                                     C.POINTER(C.c_int)]),
     "bspai_hip_mm_entries": (C.c_int, [_VP, _VP, _VP, _VP]),
     "bspai_hip_mm_destroy": (None, [_VP]),
+    "bspai_hip_mm_parse_gpu": (C.c_int, [_VP, C.c_char_p, C.c_int64, _PP, C.c_char_p, C.c_int]),
+    "bspai_hip_mm_gpu_info": (C.c_int, [_VP, _I64P, _I64P, _I64P, C.POINTER(C.c_int),
+                                        C.POINTER(C.c_int)]),
+    "bspai_hip_mm_gpu_entries": (C.c_int, [_VP, _PP, _PP, _PP]),
+    "bspai_hip_mm_gpu_copy": (C.c_int, [_VP, _VP, _VP, _VP, _VP, C.c_int]),
+    "bspai_hip_mm_gpu_destroy": (None, [_VP]),
     "bspai_hip_spmv": (C.c_int, [_VP, _VP, _VP, _VP, C.c_int, C.c_int]),
     "bspai_hip_spmv_f32": (C.c_int, [_VP, _VP, _VP, _VP, C.c_int, C.c_int]),
     "bspai_hip_spai_build": (C.c_int, [_VP, _VP, C.POINTER(SpaiOptions), _PP]),

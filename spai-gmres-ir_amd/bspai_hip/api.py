@@ -182,10 +182,11 @@ class MatrixMarketError(RuntimeError):
     """A malformed Matrix Market text (the reference reader's std::runtime_error)."""
 
 
-def parse_matrix_market(text, threads: int = 0):
-    """Coordinate entries of a Matrix Market text, parsed on the host by the
-    library's multi-threaded reader (bspai_hip_mm_parse) with the rules and
-    messages of the reference's read_matrix_market (src/cpu.cpp:105-161).
+def parse_matrix_market(text, threads: int = 0, gpu: bool = False, ctx: Context | None = None):
+    """Coordinate entries of a Matrix Market text, parsed with the rules and
+    messages of the reference's read_matrix_market (src/cpu.cpp:105-161): on
+    the host by the library's multi-threaded reader (bspai_hip_mm_parse), or
+    with gpu=True on the GPU (bspai_hip_mm_parse_gpu, same rules and bits).
 
     Returns (rows, cols, row, col, val): zero-based int64 / float64 numpy
     arrays in file order (a symmetric file's off-diagonal entries followed by
@@ -193,6 +194,25 @@ def parse_matrix_market(text, threads: int = 0):
     reference's message for a malformed file."""
     data = text.encode() if isinstance(text, str) else bytes(text)
     L = _lib.load()
+    if gpu:
+        ctx = ctx or default_context()
+        h = C.c_void_p()
+        msg = C.create_string_buffer(256)
+        rc = L.bspai_hip_mm_parse_gpu(ctx.h, data, len(data), C.byref(h), msg, len(msg))
+        if rc == _lib.EFORMAT:
+            raise MatrixMarketError(msg.value.decode())
+        check(rc, ctx.h)
+        try:
+            rows, cols, count = C.c_int64(), C.c_int64(), C.c_int64()
+            L.bspai_hip_mm_gpu_info(h, C.byref(rows), C.byref(cols), C.byref(count), None, None)
+            r = np.empty(count.value, np.int64)
+            c = np.empty(count.value, np.int64)
+            v = np.empty(count.value, np.float64)
+            check(L.bspai_hip_mm_gpu_copy(ctx.h, h, r.ctypes.data, c.ctypes.data, v.ctypes.data,
+                                          HOST), ctx.h)
+        finally:
+            L.bspai_hip_mm_gpu_destroy(h)
+        return rows.value, cols.value, r, c, v
     h = C.c_void_p()
     msg = C.create_string_buffer(256)
     rc = L.bspai_hip_mm_parse(data, len(data), int(threads), C.byref(h), msg, len(msg))
@@ -212,18 +232,19 @@ def parse_matrix_market(text, threads: int = 0):
     return rows.value, cols.value, r, c, v
 
 
-def read_matrix_market(source, ctx: Context | None = None):
+def read_matrix_market(source, ctx: Context | None = None, gpu: bool = True):
     """Canonical CSR of a Matrix Market file (a path) or text (str/bytes with
-    a newline): parsed on the host (parse_matrix_market), sorted and summed on
-    the GPU (csr_from_entries) -- reference read_matrix_market
-    (include/bspai/cpu.hpp:47).  Returns (rows, cols, ptr, col, val); raises
-    MatrixMarketError or ValueError with the reference's messages."""
+    a newline): parsed on the GPU (gpu=False: on the host,
+    parse_matrix_market), sorted and summed on the GPU (csr_from_entries) --
+    reference read_matrix_market (include/bspai/cpu.hpp:47).  Returns (rows,
+    cols, ptr, col, val); raises MatrixMarketError or ValueError with the
+    reference's messages."""
     if isinstance(source, (bytes, bytearray)) or (isinstance(source, str) and "\n" in source):
         text = source
     else:
         with open(source, "rb") as f:
             text = f.read()
-    rows, cols, r, c, v = parse_matrix_market(text)
+    rows, cols, r, c, v = parse_matrix_market(text, gpu=gpu, ctx=ctx)
     ptr, col, val = csr_from_entries(rows, cols, r, c, v, ctx)
     return rows, cols, ptr, col, val
 

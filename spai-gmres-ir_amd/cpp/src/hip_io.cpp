@@ -1,6 +1,6 @@
 // Input helpers of the HIP back end (no counterpart in the reference's
 // headers): coordinate entries canonicalized on the GPU, and the Matrix
-// Market reader built on the library's multi-threaded parser.  Only the
+// Market reader built on the library's GPU parser.  Only the
 // reference's public CsrMatrix API is used, so this file compiles against the
 // reference's include/bspai too.
 #include "bspai/hip_io.hpp"
@@ -61,24 +61,47 @@ CsrMatrix hip_csr_from_entries(std::size_t rows, std::size_t cols,
 }
 
 CsrMatrix hip_read_matrix_market(std::istream &input, int device) {
+    // The text parsed on the GPU (bspai_hip_mm_parse_gpu: the reference
+    // reader's rules and messages), the entries canonicalized there too; only
+    // the canonical CSR comes back.
     const std::string text = read_all(input);
-    bspai_hip_mm *mm = nullptr;
+    bspai_hip_ctx *ctx = nullptr;
+    const int rcc = bspai_hip_ctx_create(device, nullptr, &ctx);
+    if (rcc != BSPAI_HIP_OK)
+        throw std::runtime_error("bspai_hip_ctx_create failed (code " + std::to_string(rcc) + ")");
+    std::unique_ptr<bspai_hip_ctx, void (*)(bspai_hip_ctx *)> hold_ctx(ctx, bspai_hip_ctx_destroy);
+    bspai_hip_mm_gpu *mm = nullptr;
     char msg[256];
-    const int rc = bspai_hip_mm_parse(text.data(), (std::int64_t)text.size(), 0, &mm, msg,
-                                      (int)sizeof msg);
-    if (rc != BSPAI_HIP_OK)
+    const int rc = bspai_hip_mm_parse_gpu(ctx, text.data(), (std::int64_t)text.size(), &mm, msg,
+                                          (int)sizeof msg);
+    if (rc == BSPAI_HIP_EFORMAT)
         throw std::runtime_error(msg);
-    std::unique_ptr<bspai_hip_mm, void (*)(bspai_hip_mm *)> hold(mm, bspai_hip_mm_destroy);
+    if (rc != BSPAI_HIP_OK)
+        throw std::runtime_error(std::string("HIP backend: ") + msg);
+    std::unique_ptr<bspai_hip_mm_gpu, void (*)(bspai_hip_mm_gpu *)> hold(mm,
+                                                                         bspai_hip_mm_gpu_destroy);
     std::int64_t rows = 0, cols = 0, count = 0;
-    bspai_hip_mm_info(mm, &rows, &cols, &count, nullptr, nullptr);
-    std::vector<std::int64_t> r((std::size_t)count), c((std::size_t)count);
-    std::vector<double> v((std::size_t)count);
-    bspai_hip_mm_entries(mm, r.data(), c.data(), v.data());
-    hold.reset();
-    std::vector<Entry> entries((std::size_t)count);
-    for (std::size_t i = 0; i < entries.size(); ++i)
-        entries[i] = Entry{(std::size_t)r[i], (std::size_t)c[i], v[i]};
-    return hip_csr_from_entries((std::size_t)rows, (std::size_t)cols, entries, device);
+    bspai_hip_mm_gpu_info(mm, &rows, &cols, &count, nullptr, nullptr);
+    if (rows >= INT_MAX || cols >= INT_MAX || count >= INT_MAX)
+        throw std::runtime_error("HIP backend: matrix exceeds int32 indexing");
+    const std::int64_t *r = nullptr, *c = nullptr;
+    const double *v = nullptr;
+    bspai_hip_mm_gpu_entries(mm, &r, &c, &v);
+    std::vector<std::int32_t> ptr((std::size_t)rows + 1), col(count ? (std::size_t)count : 1);
+    std::vector<double> val(count ? (std::size_t)count : 1);
+    std::int64_t nnz = 0;
+    const int rc2 = bspai_hip_csr_from_entries(ctx, rows, cols, count, r, c, v, BSPAI_HIP_DEVICE,
+                                               ptr.data(), col.data(), val.data(), BSPAI_HIP_HOST,
+                                               &nnz);
+    if (rc2 == BSPAI_HIP_EINVAL)
+        throw std::invalid_argument(bspai_hip_last_error(ctx));
+    if (rc2 != BSPAI_HIP_OK)
+        throw std::runtime_error(std::string("HIP backend: ") + bspai_hip_last_error(ctx));
+    std::vector<Entry> canon((std::size_t)nnz);
+    for (std::size_t k = 0; k < (std::size_t)rows; ++k)
+        for (std::int32_t q = ptr[k]; q < ptr[k + 1]; ++q)
+            canon[(std::size_t)q] = Entry{k, (std::size_t)col[(std::size_t)q], val[(std::size_t)q]};
+    return CsrMatrix::from_entries((std::size_t)rows, (std::size_t)cols, std::move(canon));
 }
 
 } // namespace bspai

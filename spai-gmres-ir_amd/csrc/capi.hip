@@ -3,6 +3,8 @@
 // the message for bspai_hip_last_error().
 #include "../../include/bspai_hip.h"
 #include "engine.hpp"
+#include "mmgpu.hpp"
+#include "mmio_internal.hpp"
 
 #include <algorithm>
 #include <climits>
@@ -16,6 +18,9 @@ struct bspai_hip_ctx {
 };
 struct bspai_hip_matrix {
     DevCsr a;
+};
+struct bspai_hip_mm_gpu {
+    MmGpu m;
 };
 struct bspai_hip_spai {
     std::unique_ptr<SpaiBuild> b;
@@ -919,6 +924,92 @@ int bspai_hip_score(bspai_hip_ctx *ctx, double residual_squared, double dot,
         spai_score_on_device(ctx->c.stream, residual_squared, dot, denominator_squared, status,
                              clamped, score);
     });
+}
+
+
+int bspai_hip_mm_parse_gpu(bspai_hip_ctx *ctx, const char *text, int64_t len,
+                           bspai_hip_mm_gpu **out, char *message, int message_len) {
+    auto put = [&](const char *m) {
+        if (message && message_len > 0) {
+            std::strncpy(message, m, (std::size_t)message_len - 1);
+            message[message_len - 1] = 0;
+        }
+    };
+    put("");
+    std::string format_error;
+    const int rc = guard(ctx, [&] {
+        require(ctx && out && len >= 0 && (len == 0 || text), "null argument");
+        *out = nullptr;
+        auto m = std::make_unique<bspai_hip_mm_gpu>();
+        try {
+            mm_parse_gpu(ctx->c, text, (std::size_t)len, m->m);
+        } catch (const bspai_mmio::FormatError &e) {
+            format_error = e.what();
+            return;
+        }
+        *out = m.release();
+    });
+    if (!format_error.empty()) {
+        ctx->c.last_error = format_error;
+        put(format_error.c_str());
+        return BSPAI_HIP_EFORMAT;
+    }
+    if (rc != BSPAI_HIP_OK && ctx)
+        put(ctx->c.last_error.c_str());
+    return rc;
+}
+
+int bspai_hip_mm_gpu_info(const bspai_hip_mm_gpu *mm, int64_t *rows, int64_t *cols,
+                          int64_t *count, int *integer_field, int *symmetric) {
+    if (!mm)
+        return BSPAI_HIP_EINVAL;
+    if (rows)
+        *rows = mm->m.rows;
+    if (cols)
+        *cols = mm->m.cols;
+    if (count)
+        *count = mm->m.count;
+    if (integer_field)
+        *integer_field = mm->m.integer;
+    if (symmetric)
+        *symmetric = mm->m.symmetric;
+    return BSPAI_HIP_OK;
+}
+
+int bspai_hip_mm_gpu_entries(const bspai_hip_mm_gpu *mm, const int64_t **row, const int64_t **col,
+                             const double **val) {
+    if (!mm)
+        return BSPAI_HIP_EINVAL;
+    if (row)
+        *row = mm->m.row.data();
+    if (col)
+        *col = mm->m.col.data();
+    if (val)
+        *val = mm->m.val.data();
+    return BSPAI_HIP_OK;
+}
+
+int bspai_hip_mm_gpu_copy(bspai_hip_ctx *ctx, const bspai_hip_mm_gpu *mm, int64_t *row,
+                          int64_t *col, double *val, int where) {
+    return guard(ctx, [&] {
+        require(ctx && mm, "null argument");
+        require(where == BSPAI_HIP_HOST || where == BSPAI_HIP_DEVICE, "where: HOST or DEVICE");
+        const std::size_t n = (std::size_t)mm->m.count;
+        if (n == 0)
+            return;
+        require(row && col && val, "null argument");
+        const hipMemcpyKind kind =
+            where == BSPAI_HIP_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+        hipStream_t s = ctx->c.stream;
+        HIP_CHECK(hipMemcpyAsync(row, mm->m.row.data(), n * sizeof(int64_t), kind, s));
+        HIP_CHECK(hipMemcpyAsync(col, mm->m.col.data(), n * sizeof(int64_t), kind, s));
+        HIP_CHECK(hipMemcpyAsync(val, mm->m.val.data(), n * sizeof(double), kind, s));
+        HIP_CHECK(hipStreamSynchronize(s));
+    });
+}
+
+void bspai_hip_mm_gpu_destroy(bspai_hip_mm_gpu *mm) {
+    delete mm;
 }
 
 } // extern "C"

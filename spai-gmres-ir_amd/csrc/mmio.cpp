@@ -20,10 +20,14 @@
 //     std::istringstream): the banner has five fields, the dimension line and
 //     the entries three, and a further field is an error;
 //   * indices and dimensions are decimal digit strings that must fit size_t;
-//   * a value is converted by strtod, which must consume the whole field, not
-//     report ERANGE and give a finite number; an integer field also allows
-//     only the characters "+-0123456789".
+//   * a value is what strtod converts, in full, without a range error and
+//     finite; an integer field also allows only the characters
+//     "+-0123456789".
+// The body-line rules and the value conversion are mmtext.hpp's, which the
+// GPU parser (mmgpu.hip) shares; mm_read_header is shared with it too.
 #include "../../include/bspai_hip.h"
+#include "mmio_internal.hpp"
+#include "mmtext.hpp"
 
 #include <algorithm>
 #include <cerrno>
@@ -45,18 +49,8 @@ struct Entry64 {
     double val;
 };
 
-struct FormatError : std::runtime_error {
-    using std::runtime_error::runtime_error;
-};
-
-// The delimiters of operator>> on a string in the classic locale.
-inline bool field_space(char c) {
-    return c == ' ' || c == '\t' || c == '\n' || c == '\v' || c == '\f' || c == '\r';
-}
-// The characters a line may start with and still be blank.
-inline bool blank_lead(char c) {
-    return c == ' ' || c == '\t' || c == '\r';
-}
+using bspai_mmio::FormatError;
+using bspai_mmio::MmHeader;
 
 struct Span {
     const char *b = nullptr, *e = nullptr;
@@ -71,27 +65,11 @@ struct Span {
 // Splits [b, e) into at most `want` fields; returns the number found, up to
 // want + 1 (one more means "extra field").
 int fields(const char *b, const char *e, Span *out, int want) {
-    int n = 0;
-    const char *p = b;
-    while (n <= want) {
-        while (p < e && field_space(*p))
-            ++p;
-        if (p == e)
-            break;
-        const char *s = p;
-        while (p < e && !field_space(*p))
-            ++p;
-        if (n < want)
-            out[n] = Span{s, p};
-        ++n;
-    }
+    const char *fb[5], *fe[5];
+    const int n = bspai_mm::split_fields(b, e, fb, fe, want);
+    for (int i = 0; i < n && i < want; ++i)
+        out[i] = Span{fb[i], fe[i]};
     return n;
-}
-
-bool is_data_line(const char *b, const char *e) {
-    while (b < e && blank_lead(*b))
-        ++b;
-    return b < e && *b != '%';
 }
 
 // End of the line that starts at p (the position of its '\n', or e).
@@ -102,57 +80,18 @@ inline const char *line_end(const char *p, const char *e) {
 
 // A decimal digit string within size_t, else the reference's messages.
 std::size_t to_count(const Span &t) {
-    std::size_t v = 0;
-    for (const char *p = t.b; p < t.e; ++p)
-        if (*p < '0' || *p > '9')
-            throw FormatError("Expected a nonnegative integer dimension/index");
-    const std::size_t top = std::numeric_limits<std::size_t>::max();
-    for (const char *p = t.b; p < t.e; ++p) {
-        const std::size_t d = (std::size_t)(*p - '0');
-        if (v > (top - d) / 10)
-            throw FormatError("Dimension/index exceeds size_t");
-        v = v * 10 + d;
-    }
-    return v;
+    std::uint64_t v = 0;
+    const int code = bspai_mm::parse_index(t.b, t.e, v);
+    if (code)
+        throw FormatError(bspai_mm::line_message(code));
+    return (std::size_t)v;
 }
 
-double to_value(const Span &t, bool integer_field) {
-    // strtod needs a terminated copy; fields are short.
-    char small[64];
-    std::string big;
-    const std::size_t n = t.size();
-    const char *s;
-    if (n < sizeof small) {
-        std::memcpy(small, t.b, n);
-        small[n] = 0;
-        s = small;
-    } else {
-        big.assign(t.b, n);
-        s = big.c_str();
-    }
-    char *end = nullptr;
-    errno = 0;
-    const double v = std::strtod(s, &end);
-    if (end == s || errno == ERANGE)
-        throw FormatError("Invalid coordinate value");
-    if ((std::size_t)(end - s) != n || !std::isfinite(v))
-        throw FormatError("Coordinate value must be finite and numeric");
-    if (integer_field)
-        for (std::size_t i = 0; i < n; ++i)
-            if (!std::strchr("+-0123456789", s[i]) || s[i] == 0)
-                throw FormatError("Integer field requires integer value tokens");
-    return v;
-}
+} // namespace
 
-struct Header {
-    std::size_t rows = 0, cols = 0, count = 0;
-    bool integer = false, symmetric = false;
-    std::size_t body = 0; // offset of the first line after the dimension line
-};
-
-Header read_header(const char *text, std::size_t len) {
+MmHeader bspai_mmio::mm_read_header(const char *text, std::size_t len) {
     const char *e = text + len;
-    Header h;
+    MmHeader h;
     if (len == 0)
         throw FormatError("Missing Matrix Market banner");
     const char *le = line_end(text, e);
@@ -173,7 +112,7 @@ Header read_header(const char *text, std::size_t len) {
         if (p >= e)
             throw FormatError("Missing matrix dimensions");
         const char *q = line_end(p, e);
-        if (is_data_line(p, q)) {
+        if (bspai_mm::is_data_line(p, q)) {
             Span d[3];
             const int nd = fields(p, q, d, 3);
             if (nd < 3)
@@ -195,6 +134,8 @@ Header read_header(const char *text, std::size_t len) {
     return h;
 }
 
+namespace {
+
 // One byte range of the body, parsed by one thread.
 struct Piece {
     const char *b = nullptr, *e = nullptr;
@@ -205,32 +146,24 @@ struct Piece {
     std::string message;
 };
 
-void parse_piece(Piece &pc, const Header &h) {
+void parse_piece(Piece &pc, const MmHeader &h) {
     const char *p = pc.b;
     while (p < pc.e) {
         const char *q = line_end(p, pc.e);
-        if (is_data_line(p, q)) {
+        if (bspai_mm::is_data_line(p, q)) {
             if (!pc.failed) {
-                try {
-                    Span t[3];
-                    const int nf = fields(p, q, t, 3);
-                    if (nf < 3)
-                        throw FormatError("Malformed coordinate entry");
-                    if (nf > 3)
-                        throw FormatError("Extra field in Matrix Market line");
-                    const std::size_t r = to_count(t[0]);
-                    const std::size_t c = to_count(t[1]);
-                    if (r == 0 || c == 0 || r > h.rows || c > h.cols)
-                        throw FormatError("Matrix Market index outside one-based dimensions");
-                    const double v = to_value(t[2], h.integer);
-                    pc.entries.push_back(Entry64{(std::int64_t)(r - 1), (std::int64_t)(c - 1), v});
-                    if (h.symmetric && r != c)
-                        pc.entries.push_back(
-                            Entry64{(std::int64_t)(c - 1), (std::int64_t)(r - 1), v});
-                } catch (const FormatError &err) {
+                std::uint64_t r = 0, c = 0;
+                double v = 0;
+                const int code =
+                    bspai_mm::parse_entry_line<true>(p, q, h.rows, h.cols, h.integer, r, c, v);
+                if (code) {
                     pc.failed = true;
                     pc.fail_line = pc.data_lines;
-                    pc.message = err.what();
+                    pc.message = bspai_mm::line_message(code);
+                } else {
+                    pc.entries.push_back(Entry64{(std::int64_t)r, (std::int64_t)c, v});
+                    if (h.symmetric && r != c)
+                        pc.entries.push_back(Entry64{(std::int64_t)c, (std::int64_t)r, v});
                 }
             }
             ++pc.data_lines;
@@ -250,7 +183,7 @@ struct bspai_hip_mm {
 namespace {
 
 void parse_all(const char *text, std::size_t len, int threads, bspai_hip_mm &out) {
-    const Header h = read_header(text, len);
+    const MmHeader h = bspai_mmio::mm_read_header(text, len);
     const char *b = text + h.body, *e = text + len;
     // Byte ranges of about 4 MiB or more, at most `threads` (16 by default),
     // each ending just after a line break.

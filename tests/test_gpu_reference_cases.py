@@ -231,20 +231,22 @@ def test_cli_rejects_like_reference(case):
     assert r.stdout == ""
 
 
+@pytest.mark.parametrize("gpu", [True, False], ids=["gpu-parse", "host-parse"])
 @pytest.mark.parametrize("case", MM_CASES, ids=[c["name"] for c in MM_CASES])
-def test_read_matrix_market_on_gpu(ctx, case):
-    """read_matrix_market: host parse + GPU canonicalization gives the
-    reference reader's CSR bits, or its message."""
+def test_read_matrix_market_on_gpu(ctx, case, gpu):
+    """read_matrix_market: the text parsed on the GPU (or on the host) and
+    canonicalized on the GPU gives the reference reader's CSR bits, or its
+    message."""
     import bspai_hip
 
     ref = case["reference"]
     if "error" in ref:
         kind = bspai_hip.MatrixMarketError if ref["error"] == "runtime_error" else ValueError
         with pytest.raises(kind) as e:
-            bspai_hip.read_matrix_market(case["text"].encode(), ctx)
+            bspai_hip.read_matrix_market(case["text"].encode(), ctx, gpu=gpu)
         assert str(e.value) == ref["message"]
         return
-    rows, cols, ptr, col, val = bspai_hip.read_matrix_market(case["text"].encode(), ctx)
+    rows, cols, ptr, col, val = bspai_hip.read_matrix_market(case["text"].encode(), ctx, gpu=gpu)
     assert (rows, cols) == (ref["rows"], ref["cols"])
     assert list(ptr) == ref["ptr"] and list(col) == ref["col"]
     assert [format(int(x), "016x") for x in np.asarray(val).view(np.uint64)] == ref["val_bits"]
