@@ -19,7 +19,6 @@
 #include "engine.hpp"
 #include "kernels.hpp"
 
-#include <hipcub/hipcub.hpp>
 
 #include <climits>
 #include <map>
@@ -128,14 +127,8 @@ struct CanonBuffers {
     std::size_t sort_tmp_bytes;
 };
 
-// Radix-sort temporary storage for `count` pairs on the key bits [0, end_bit).
-std::size_t sort_tmp_bytes(std::int64_t count, int end_bit, hipStream_t s) {
-    std::size_t bytes = 0;
-    HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(
-        nullptr, bytes, (const unsigned long long *)nullptr, (unsigned long long *)nullptr,
-        (const double *)nullptr, (double *)nullptr, (int)count, 0, end_bit, s));
-    return bytes;
-}
+// Radix-sort temporary storage for `count` pairs (radix.hip).
+std::size_t sort_tmp_bytes(std::int64_t count) { return radix_sort_tmp_bytes(count); }
 
 int key_bits(std::int64_t rows, std::int64_t cols) {
     const unsigned long long max_key =
@@ -161,11 +154,9 @@ std::int64_t canon_run(Context &c, std::int64_t rows, std::int64_t cols, std::in
     });
     // Sort on the key bits in use only.
     const int end_bit = key_bits(rows, cols);
-    std::size_t tmp_bytes = w.sort_tmp_bytes;
     timed(c, "entries_radix_sort", 4.0 * 16.0 * (double)count, 0.0, [&] {
-        HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(w.sort_tmp, tmp_bytes, w.keys,
-                                                     w.keys_sorted, val, w.vals_sorted,
-                                                     (int)count, 0, end_bit, s));
+        radix_sort_pairs(s, w.keys, val, w.keys_sorted, w.vals_sorted, count, end_bit,
+                         w.sort_tmp);
     });
     timed(c, "run_sums_kernel", 28.0 * (double)count, 0.0, [&] {
         hipLaunchKernelGGL(run_sums_kernel, dim3(blocks), dim3(kCanonThreads), 0, s, count,
@@ -223,7 +214,7 @@ std::int64_t csr_from_entries(Context &c, std::int64_t rows, std::int64_t cols,
     w.flags.resize(1);
     w.scan_tmp.resize(scan_tmp_ints(count));
     w.row_tmp.resize(scan_tmp_ints(rows + 1));
-    const std::size_t tb = sort_tmp_bytes(count, key_bits(rows, cols), s);
+    const std::size_t tb = sort_tmp_bytes(count);
     w.sort_tmp.resize(std::max<std::size_t>(tb, 1));
     const CanonBuffers b{w.keys.data(), w.keys_sorted.data(), w.flags64.data(),
                          w.vals_sorted.data(), w.sums.data(), w.keep.data(),
@@ -287,7 +278,7 @@ void noalloc_setup(Context &c) {
     HIP_CHECK(hipGetSymbolAddress(&d.base, HIP_SYMBOL(scratch)));
     if (!d.base)
         throw HipError("hipGetSymbolAddress returned no address for the scratch array");
-    d.sort_bytes = sort_tmp_bytes(kNoAllocMaxEntries, 64, c.stream);
+    d.sort_bytes = sort_tmp_bytes(kNoAllocMaxEntries);
     g_noalloc[c.device] = d;
 }
 

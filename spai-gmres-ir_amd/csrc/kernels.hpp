@@ -87,6 +87,13 @@ void launch_transpose_fill(hipStream_t s, int n, const int *ptr, const int *col,
 // exclusive/inclusive scan helpers (int32), in place on counts[0..n]:
 // counts[0] must be 0; after the call counts[i] = sum of counts[<i] inputs.
 void scan_inclusive_int(hipStream_t s, int *data, std::int64_t n, int *tmp);
+// Stable LSD radix sort of (key, value) pairs on the key bits [0, end_bit)
+// (radix.hip): keys (overwritten) and vals (read only) in, keys_out and
+// vals_out out; tmp holds radix_sort_tmp_bytes(count) bytes.
+std::size_t radix_sort_tmp_bytes(std::int64_t count);
+void radix_sort_pairs(hipStream_t s, unsigned long long *keys, const double *vals,
+                      unsigned long long *keys_out, double *vals_out, std::int64_t count,
+                      int end_bit, void *tmp);
 std::int64_t scan_tmp_ints(std::int64_t n);
 
 // ------------------------------------------------------------------ tiles
