@@ -730,7 +730,7 @@ constexpr int kDppXor3 = 0x1B; // quad_perm [3, 2, 1, 0]
 constexpr int kDppXor7 = 0x141; // row_half_mirror
 constexpr int kDppXor8 = 0x128; // row_ror:8
 template <int kCtrl> __device__ inline unsigned dpp_u(unsigned v) {
-    return (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, kCtrl, 0xf, 0xf, false);
+    return (unsigned)__builtin_amdgcn_mov_dpp((int)v, kCtrl, 0xf, 0xf, true);
 }
 __device__ inline void swap16(unsigned &x, unsigned &y) {
     const auto r = __builtin_amdgcn_permlane16_swap(x, y, false, false);
@@ -1179,22 +1179,29 @@ spai_wave_kernel(const int *__restrict__ aptr, const int *__restrict__ acol,
             const float inf_norm = wave_max_f(rowsum);
             float min_pivot = INFINITY, dg = 0.f;
             int fit_status = kSpaiSuccess;
-            // One rolled loop over the steps: column j of the row is picked
-            // from r[] by compares with j, so r[] stays in registers and the
-            // code holds one step (an unrolled loop over 16 steps made the
+            // One rolled loop over the steps; column j of the row is read from
+            // r[] by its (wave-uniform) index, so r[] stays in registers and
+            // the code holds one step (an unrolled loop over 16 steps made the
             // kernel larger than the instruction cache).  Every value tested
             // below is wave-uniform (butterfly sums give every lane the same
             // bits), so the loop exits are uniform.
 #pragma unroll 1
             for (int j = 0; j < q; ++j) {
                 const bool act = lane >= j && lane < m;
-                float x = 0.f;
-#pragma unroll
-                for (int c = 0; c < kWQ; ++c)
-                    x = c == j ? r[c] : x;
-                x = act ? x : 0.f;
-                const float length = __int_as_float(__builtin_amdgcn_readfirstlane(
-                    __float_as_int((float)sqrt(wave_sum_d((double)x * x)))));
+                const float x = act ? r[j] : 0.f;
+                // The squared norm in FP32 (entries of B(I, J) are at most 1
+                // in magnitude and reflections keep column norms, so the sum
+                // cannot overflow); a sum below 2^-100, where the squares of
+                // the small entries lose bits, and a non-finite one take FP64.
+                float length;
+                {
+                    const float s2 = wave_sum_f(x * x);
+                    if (s2 >= 0x1p-100f && s2 <= 0x1p+100f)
+                        length = __builtin_amdgcn_sqrtf(s2);
+                    else
+                        length = (float)sqrt(wave_sum_d((double)x * x));
+                    length = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(length)));
+                }
                 if (!isfinite(length)) {
                     fit_status = kSpaiNumericalFailure;
                     break;
@@ -1205,17 +1212,20 @@ spai_wave_kernel(const int *__restrict__ aptr, const int *__restrict__ acol,
                 }
                 const float xj = lane_value(x, j);
                 const float diagonal = -copysignf(length, xj);
-                // reciprocals of the step's two (wave-uniform) scales by the
-                // hardware rcp (1 ulp; the FP32 build's order and rounding are
-                // its own) in place of per-lane divisions
+                // reciprocals and root of the step's (wave-uniform) scales by
+                // the hardware rcp and sqrt (1 ulp; the FP32 build's order and
+                // rounding are its own) in place of per-lane divisions
                 const float inv_len = __builtin_amdgcn_rcpf(length);
-                const float vnorm = sqrtf(2.f + 2.f * fabsf(xj) * inv_len);
+                const float vnorm = __builtin_amdgcn_sqrtf(2.f + 2.f * fabsf(xj) * inv_len);
                 float v = act ? x * inv_len : 0.f;
                 if (lane == j)
                     v += copysignf(1.f, v);
                 v = v * __builtin_amdgcn_rcpf(vnorm);
-                // v is 0 outside rows j..m-1, so the products need no mask;
-                // the sums of columns c <= j are formed but never read.
+                // v is 0 outside rows j..m-1, so neither the products nor the
+                // reflection need a mask: rows outside keep their bits (x - 0
+                // = x).  Columns c <= j are reflected too; their rows >= j are
+                // never read again, and the reflection keeps them bounded.
+                // Columns c >= q hold zeros and stay zero.
                 float p[kWQ];
 #pragma unroll
                 for (int c = 0; c < kWQ; ++c)
@@ -1224,15 +1234,9 @@ spai_wave_kernel(const int *__restrict__ aptr, const int *__restrict__ acol,
                 const float dr = wave_sum_f(v * rh);
                 const float v2 = 2.f * v;
 #pragma unroll
-                for (int c = 1; c < kWQ; ++c) {
-                    if (c > j && c < q) {
-                        const float dc = lane_value(d, 4 * c);
-                        if (act)
-                            r[c] -= v2 * dc;
-                    }
-                }
-                if (act)
-                    rh -= v2 * dr;
+                for (int c = 1; c < kWQ; ++c)
+                    r[c] -= v2 * lane_value(d, 4 * c);
+                rh -= v2 * dr;
                 if (lane == j)
                     dg = diagonal;
                 min_pivot = fminf(min_pivot, fabsf(diagonal));
@@ -1274,12 +1278,8 @@ spai_wave_kernel(const int *__restrict__ aptr, const int *__restrict__ acol,
                         coef = cj;
                     }
                     cj = lane_value(cj, j);
-                    float rj = 0.f;
-#pragma unroll
-                    for (int c = 0; c < kWQ; ++c)
-                        rj = c == j ? r[c] : rj;
                     if (lane < j)
-                        acc -= rj * cj;
+                        acc -= r[j] * cj;
                 }
                 const unsigned long long badm = __ballot(st != kSpaiSuccess);
                 if (badm != 0ull)
