@@ -654,18 +654,18 @@ template void launch_spai_columns<float>(hipStream_t, int, int, int, const int *
 // FP32 fits of the first-bucket columns (the mixed path), one wavefront per
 // column, the dense problem in registers and LDS: lane r owns row r of the
 // least-squares matrix (m <= 64).
-//   * index sets: the rows that make up a set (rows of A for the shadow,
-//     rows of B = A^T for the candidates) are loaded cooperatively, then
-//     (column, entry) keys are bitonic-sorted across the wave; adjacent
-//     compares give the sorted, duplicate-free set and every entry's
-//     position in it;
+//   * shadow: the rows of A in J are loaded cooperatively, then (column,
+//     entry) keys are bitonic-sorted across the wave; adjacent compares give
+//     the sorted, duplicate-free set I and every entry's position in it;
 //   * QR: each lane keeps its row in registers; the dot products of the
 //     Householder vector with the trailing columns come from one
 //     reduce-scatter over the wave (lane 4c ends with column c);
-//   * candidate scores: the candidate stage keeps B's values, so the terms
-//     res_i * b_ij d_j and (b_ij d_j)^2 are formed per entry and summed per
-//     candidate in sorted order (ascending i, as the lane kernel sums A's
-//     row j), with no further loads;
+//   * candidates: the columns of B's rows i in I go into an open-addressing
+//     set in LDS, whose distinct members are sorted (so the sort sees about
+//     a quarter of the entries at 128^3) and J removed;
+//   * candidate scores: one lane per candidate walks row j of A (b_ij =
+//     a_ji) in CSR order, ascending i as the lane kernel sums it, finding
+//     each column's position in I by binary search;
 //   * the beta smallest (score, index) pairs are found by argmin rounds
 //     over the wave.
 // The method and its decisions are the lane kernel's; the FP32 summation
@@ -678,19 +678,20 @@ constexpr int kWC = 128;     // distinct candidate rows including J
 constexpr int kWS = 512;     // staged entries (9-bit entry numbers in the sort keys)
 constexpr int kWKeyColBits = 23; // column bits of a 32-bit sort key (n < 2^23)
 constexpr int kWW = kWQ + 1; // LDS row stride of R
+constexpr int kWH = 2 * kWC; // candidate set slots
 
 struct WaveLds {
     // R is read last by the residual, before the candidate stage; the
-    // candidate stage's arrays are read last by the scores, before the next
-    // fit's gather.  9,344 bytes in all: 16 waves fit one CU's LDS.
+    // candidate stage's table and list are read last before the scores.
+    // 9,088 bytes in all: 16 waves fit one CU's LDS.
     union {
         float R[kWM][kWW]; // gathered B(I, J) (unchanged by the QR, which runs in registers)
         struct {
-            float sden[kWS]; // candidate stage: (b_ij d_j)^2 in sorted order
-            int skey[kWS];   // candidate stage: keys in sorted order
+            int ht[kWH];    // candidate stage: open-addressing set of columns (-1: empty)
+            int ulist[kWH]; // candidate stage: the distinct columns, unordered
         };
     };
-    float sval[kWS];     // staged values by entry; candidate stage: res_i b_ij d_j, sorted
+    float sval[kWS];     // shadow stage: staged values by entry
     signed char sc[kWS]; // entry -> staging row
     int sbeg[kWM];       // staging row r holds entries e of source index sbeg[r] + e
     float sd[kWM];       // staging row scale
@@ -698,7 +699,6 @@ struct WaveLds {
     int J[kWQ];
     int sel[kWQ];
     int cand[kWC];
-    short rbeg[kWC]; // first sorted entry of each candidate
     float score[kWC];
     float res[kWM];
 };
@@ -897,11 +897,40 @@ __device__ int wave_rows(WaveLds &W, const int *__restrict__ ptr,
     return total;
 }
 
-// Bitonic sort of 64 P keys, element e = lane * P + i held in key[i].  The
-// stage loops stay rolled (k and j are wave-uniform) so that the sorts of all
-// widths together stay small in the instruction cache; only the P keys of a
-// lane are unrolled.
+// Bitonic sort of 64 P keys, element e = lane * P + i held in key[i].  For
+// P <= 2 (the shadows and candidate sets of the stencil problems) every stage
+// is unrolled, so each cross-lane partner is a compile-time lane ^ m moved by
+// DPP or a permlane swap (a few cycles, against a round trip through the LDS
+// crossbar).  Wider sorts keep the stage loops rolled (k and j wave-uniform)
+// so that the sorts of all widths together stay small in the instruction
+// cache; only the P keys of a lane are unrolled.
 template <int P> __device__ inline void wave_bitonic(unsigned (&key)[P], int lane) {
+    if constexpr (P <= 2) {
+#pragma unroll
+        for (int k = 2; k <= kWave * P; k <<= 1) {
+#pragma unroll
+            for (int j = k >> 1; j > 0; j >>= 1) {
+                if (j >= P) { // partner lane ^ (j / P), same slot
+                    const bool lower = (lane & (j / P)) == 0;
+#pragma unroll
+                    for (int i = 0; i < P; ++i) {
+                        const unsigned o = xor_lane(key[i], j / P, lane);
+                        const bool up = ((lane * P + i) & k) == 0;
+                        const bool take_min = lower == up;
+                        const bool less = o < key[i];
+                        key[i] = take_min == less ? o : key[i];
+                    }
+                } else { // P == 2, j == 1: slots 0 and 1 of the lane
+                    const bool up = ((lane * P) & k) == 0;
+                    const unsigned x = key[0], y = key[P - 1];
+                    const bool sw = up ? x > y : x < y;
+                    key[0] = sw ? y : x;
+                    key[P - 1] = sw ? x : y;
+                }
+            }
+        }
+        return;
+    }
 #pragma unroll 1
     for (int k = 2; k <= kWave * P; k <<= 1) {
 #pragma unroll 1
@@ -939,16 +968,13 @@ template <int P> __device__ inline void wave_bitonic(unsigned (&key)[P], int lan
     }
 }
 
-// The staged entries e < total of (col, val), sorted by (col, e) and
-// numbered by distinct column.  Shadow (kCand false): I <- the sorted set and
-// R(position, staging row) <- val * sd.  Candidates (kCand true): the set
-// minus J -> cand and rbeg, with each entry's score terms stored in sorted
-// order.  Returns the set size; outputs are written only when it is within
-// the cap (kWM, or kWC - q), which the caller checks.
-template <int P, bool kCand>
+// The shadow: the staged entries e < total of (col, val), sorted by (col, e)
+// and numbered by distinct column; I <- the sorted set and R(position,
+// staging row) <- val * sd.  Returns the set size; outputs are written only
+// when it is within the cap kWM, which the caller checks.
+template <int P>
 __device__ int wave_sort_set(WaveLds &W, const int *__restrict__ col,
-                             const float *__restrict__ val, const float *__restrict__ dscale,
-                             int total, int q, int lane) {
+                             const float *__restrict__ val, int total, int lane) {
     unsigned key[P]; // (column << 9) | entry; the empty key ~0u sorts last
     {
         int src[P];
@@ -962,8 +988,8 @@ __device__ int wave_sort_set(WaveLds &W, const int *__restrict__ col,
         float v[P];
 #pragma unroll
         for (int i = 0; i < P; ++i) {
-            c[i] = col[src[i]];
-            v[i] = val[src[i]];
+            c[i] = gld(col + src[i]);
+            v[i] = gld(val + src[i]);
         }
 #pragma unroll
         for (int i = 0; i < P; ++i) {
@@ -976,7 +1002,7 @@ __device__ int wave_sort_set(WaveLds &W, const int *__restrict__ col,
     wave_bitonic<P>(key, lane);
     wave_sync(); // sval of every entry written
     int kv[P], ev[P], pos[P];
-    bool keep[P], first[P];
+    bool first[P];
 #pragma unroll
     for (int i = 0; i < P; ++i) {
         kv[i] = key[i] == ~0u ? -1 : (int)(key[i] >> 9);
@@ -990,90 +1016,184 @@ __device__ int wave_sort_set(WaveLds &W, const int *__restrict__ col,
     for (int i = 0; i < P; ++i) {
         const int pv = i ? kv[i - 1] : prev;
         first[i] = kv[i] >= 0 && kv[i] != pv;
-        bool in_j = false;
-        if (kCand && kv[i] >= 0) {
-            int lo = 0, hi = q;
-            while (lo < hi) {
-                const int mid = (lo + hi) >> 1;
-                if (W.J[mid] < kv[i])
-                    lo = mid + 1;
-                else
-                    hi = mid;
-            }
-            in_j = lo < q && W.J[lo] == kv[i];
-        }
-        keep[i] = kv[i] >= 0 && !in_j;
-        cnt += first[i] && keep[i] ? 1 : 0;
+        cnt += first[i] ? 1 : 0;
     }
     const int nset = wave_sum_int(cnt);
     {
         int run = wave_scan_excl(cnt, lane) - 1;
 #pragma unroll
         for (int i = 0; i < P; ++i) {
-            if (first[i] && keep[i])
+            if (first[i])
                 ++run;
             pos[i] = run;
         }
     }
-    if (!kCand) {
-        if (nset > kWM)
-            return nset;
-#pragma unroll
-        for (int i = 0; i < P; ++i) {
-            if (first[i])
-                W.I[pos[i]] = kv[i];
-            if (keep[i])
-                W.R[pos[i]][W.sc[ev[i]]] = W.sval[ev[i]];
-        }
-        wave_sync();
+    if (nset > kWM)
         return nset;
-    }
-    if (nset + q > kWC)
-        return nset;
-    float dj[P], sv[P], rs[P];
-#pragma unroll
-    for (int i = 0; i < P; ++i)
-        dj[i] = dscale[keep[i] ? kv[i] : 0];
 #pragma unroll
     for (int i = 0; i < P; ++i) {
-        const int e = keep[i] ? ev[i] : 0;
-        sv[i] = W.sval[e];
-        rs[i] = W.res[W.sc[e]];
-    }
-    wave_sync(); // the entry-ordered values are read before the sorted ones replace them
-#pragma unroll
-    for (int i = 0; i < P; ++i) {
-        const int s = lane * P + i;
+        if (first[i])
+            W.I[pos[i]] = kv[i];
         if (kv[i] >= 0)
-            W.skey[s] = kv[i];
-        if (keep[i]) {
-            const float v = sv[i] * dj[i];
-            W.sval[s] = rs[i] * v;
-            W.sden[s] = v * v;
-        }
-        if (first[i] && keep[i]) {
-            W.cand[pos[i]] = kv[i];
-            W.rbeg[pos[i]] = (short)s;
-        }
+            W.R[pos[i]][W.sc[ev[i]]] = W.sval[ev[i]];
     }
     wave_sync();
     return nset;
 }
 
 // (kept out of line: the sorts' registers then do not add to the kernel's)
-template <bool kCand>
-__device__ __attribute__((noinline)) int wave_set(WaveLds &W, const int *col, const float *val,
-                                                  const float *dscale, int total, int q,
-                                                  int lane) {
-    if (total <= 0) // (rows of A and B in these stages always hold an entry)
+__device__ __attribute__((noinline)) int wave_shadow(WaveLds &W, const int *col, const float *val,
+                                                     int total, int lane) {
+    if (total <= 0) // (rows of A in this stage always hold an entry)
         return 0;
     if (total <= kWave)
-        return wave_sort_set<1, kCand>(W, col, val, dscale, total, q, lane);
+        return wave_sort_set<1>(W, col, val, total, lane);
     if (total <= 2 * kWave)
-        return wave_sort_set<2, kCand>(W, col, val, dscale, total, q, lane);
+        return wave_sort_set<2>(W, col, val, total, lane);
     if (total <= 4 * kWave)
-        return wave_sort_set<4, kCand>(W, col, val, dscale, total, q, lane);
-    return wave_sort_set<8, kCand>(W, col, val, dscale, total, q, lane);
+        return wave_sort_set<4>(W, col, val, total, lane);
+    return wave_sort_set<8>(W, col, val, total, lane);
+}
+
+// The distinct columns of the list, ascending, minus J -> cand; returns
+// their count.  P keys per lane hold the n <= 64 P list entries.
+template <int P>
+__device__ int wave_sorted_candidates(WaveLds &W, int n, int q, int lane) {
+    unsigned key[P];
+#pragma unroll
+    for (int i = 0; i < P; ++i) {
+        const int e = lane * P + i;
+        key[i] = e < n ? (unsigned)W.ulist[e] : ~0u;
+    }
+    wave_bitonic<P>(key, lane);
+    bool keep[P];
+    int cnt = 0;
+#pragma unroll
+    for (int i = 0; i < P; ++i) {
+        bool in_j = false;
+        if (key[i] != ~0u) {
+            const int x = (int)key[i];
+            int lo = 0, hi = q;
+            while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if (W.J[mid] < x)
+                    lo = mid + 1;
+                else
+                    hi = mid;
+            }
+            in_j = lo < q && W.J[lo] == x;
+        }
+        keep[i] = key[i] != ~0u && !in_j;
+        cnt += keep[i] ? 1 : 0;
+    }
+    const int nc = wave_sum_int(cnt);
+    int at = wave_scan_excl(cnt, lane);
+#pragma unroll
+    for (int i = 0; i < P; ++i)
+        if (keep[i])
+            W.cand[at++] = (int)key[i];
+    wave_sync();
+    return nc;
+}
+
+// The candidates of a fit: the distinct columns of B's rows i in I (staged
+// entries e < total), minus J, ascending in cand; returns their count, or
+// kWC + 1 when the set (J included) exceeds kWC.  Each entry's column goes
+// into an open-addressing set in LDS (multiplicative hash, linear probing
+// bounded by the table size; a compare-and-swap inserts or finds it), whose
+// members are then compacted and sorted: the sort sees the distinct columns
+// only, not every entry.
+__device__ __attribute__((noinline)) int wave_candidates(WaveLds &W, const int *__restrict__ bcol,
+                                                         int total, int q, int lane) {
+#pragma unroll
+    for (int u = 0; u < kWH / kWave; ++u)
+        W.ht[lane + u * kWave] = -1;
+    wave_sync();
+    int added = 0;
+    bool full = false;
+    for (int e0 = 0; e0 < total; e0 += kWave) {
+        const int e = e0 + lane;
+        if (e < total) {
+            const int x = gld(bcol + W.sbeg[W.sc[e]] + e);
+            unsigned h = ((unsigned)x * 2654435761u) >> 24; // kWH = 256 slots
+            int probe = 0;
+            for (; probe < kWH; ++probe) {
+                const int old = atomicCAS(&W.ht[h], -1, x);
+                if (old == -1) {
+                    ++added;
+                    break;
+                }
+                if (old == x)
+                    break;
+                h = (h + 1) & (kWH - 1);
+            }
+            full = full || probe == kWH;
+        }
+    }
+    wave_sync();
+    const int nu = wave_sum_int(added);
+    if (__any(full) || nu > kWC)
+        return kWC + 1;
+    // compaction: slots lane * 4 .. lane * 4 + 3 in slot order
+    constexpr int kPer = kWH / kWave;
+    int v[kPer], cnt = 0;
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) {
+        v[u] = W.ht[lane * kPer + u];
+        cnt += v[u] >= 0 ? 1 : 0;
+    }
+    int at = wave_scan_excl(cnt, lane);
+#pragma unroll
+    for (int u = 0; u < kPer; ++u)
+        if (v[u] >= 0)
+            W.ulist[at++] = v[u];
+    wave_sync();
+    if (nu <= kWave)
+        return wave_sorted_candidates<1>(W, nu, q, lane);
+    return wave_sorted_candidates<2>(W, nu, q, lane);
+}
+
+// dot = sum res_i b_ij d_j and den = sum (b_ij d_j)^2 over the rows i in I of
+// candidate j, ascending i: row j of A in CSR order (b_ij = a_ji), each
+// column's position in the sorted I (m <= 64) found by a branch-free lower
+// bound.  Entries go in batches of 8, loaded together and searched in lock
+// step, so that the batch's LDS reads of one step are in flight together.
+__device__ inline void candidate_terms(const WaveLds &W, const int *__restrict__ aptr,
+                                       const int *__restrict__ acol,
+                                       const float *__restrict__ aval, float dj, int j, int m,
+                                       float &dot, float &den) {
+    dot = 0.f;
+    den = 0.f;
+    const int e1 = gld(aptr + j + 1);
+    for (int e0 = gld(aptr + j); e0 < e1; e0 += 8) {
+        int c[8], pos[8];
+        float a[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int e = min(e0 + u, e1 - 1);
+            c[u] = gld(acol + e);
+            a[u] = gld(aval + e);
+            pos[u] = 0;
+        }
+        // pos <- the number of entries of I below c (every read below m)
+#pragma unroll
+        for (int step = kWM; step > 0; step >>= 1) {
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int t = pos[u] + step;
+                if (t <= m && W.I[t - 1] < c[u])
+                    pos[u] = t;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            if (e0 + u < e1 && pos[u] < m && W.I[pos[u]] == c[u]) {
+                const float value = a[u] * dj;
+                dot += W.res[pos[u]] * value;
+                den += value * value;
+            }
+        }
+    }
 }
 
 // Sums a[c] (c < 16) over the wave: lane l ends with the sum of column
@@ -1115,10 +1235,10 @@ __device__ inline float wave_reduce_scatter16(float (&a)[kWQ], int lane) {
 __global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4)))
 spai_wave_kernel(const int *__restrict__ aptr, const int *__restrict__ acol,
                  const float *__restrict__ aval, const float *__restrict__ dscale,
-                 const int *__restrict__ bptr, const int *__restrict__ bcol,
-                 const float *__restrict__ bval, float eps, long long alpha, int beta,
-                 const int *__restrict__ worklist, int nwork, int *__restrict__ cursor,
-                 SpaiColumnOut out, int *__restrict__ out_j, float *__restrict__ out_c,
+                 const int *__restrict__ bptr, const int *__restrict__ bcol, float eps,
+                 long long alpha, int beta, const int *__restrict__ worklist, int nwork,
+                 int *__restrict__ cursor, SpaiColumnOut out, int *__restrict__ out_j,
+                 float *__restrict__ out_c,
                  std::int64_t out_stride) {
     __shared__ WaveLds W;
     const int lane = threadIdx.x;
@@ -1150,7 +1270,7 @@ spai_wave_kernel(const int *__restrict__ aptr, const int *__restrict__ acol,
                 break;
             }
             const int m =
-                __builtin_amdgcn_readfirstlane(wave_set<false>(W, acol, aval, nullptr, na, q, lane));
+                __builtin_amdgcn_readfirstlane(wave_shadow(W, acol, aval, na, lane));
             if (m > kWM) {
                 status = kSpaiOverflow;
                 break;
@@ -1338,7 +1458,7 @@ spai_wave_kernel(const int *__restrict__ aptr, const int *__restrict__ acol,
                 break;
             }
             const int nc =
-                __builtin_amdgcn_readfirstlane(wave_set<true>(W, bcol, bval, dscale, nb, q, lane));
+                __builtin_amdgcn_readfirstlane(wave_candidates(W, bcol, nb, q, lane));
             if (nc + q > kWC) {
                 status = kSpaiOverflow;
                 break;
@@ -1347,7 +1467,7 @@ spai_wave_kernel(const int *__restrict__ aptr, const int *__restrict__ acol,
                 status = kSpaiEmptyCandidates;
                 break;
             }
-            // ---- restricted scores, from the sorted terms
+            // ---- restricted scores
             const float rr = rnorm * rnorm;
             int first_bad = INT_MAX, bad_status = kSpaiSuccess, cl_before[kWC / kWave];
             float msum_l = 0.f; // the lane's scores, in its order
@@ -1357,13 +1477,8 @@ spai_wave_kernel(const int *__restrict__ aptr, const int *__restrict__ acol,
                 cl_before[u] = 0;
                 if (t < nc) {
                     const int j = W.cand[t];
-                    int s = W.rbeg[t];
-                    float dt = 0.f, den = 0.f;
-                    do {
-                        dt += W.sval[s];
-                        den += W.sden[s];
-                        ++s;
-                    } while (s < nb && W.skey[s] == j);
+                    float dt, den;
+                    candidate_terms(W, aptr, acol, aval, gld(dscale + j), j, m, dt, den);
                     float sv = 0.f;
                     bool cl = false;
                     const int st = score_of(rr, dt, den, sv, cl);
@@ -1478,12 +1593,12 @@ spai_wave_kernel(const int *__restrict__ aptr, const int *__restrict__ acol,
 
 void launch_spai_wave(hipStream_t s, int blocks, const int *aptr, const int *acol,
                       const float *aval, const float *dscale, const int *bptr, const int *bcol,
-                      const float *bval, float eps, long long alpha, int beta,
-                      const int *worklist, int nwork, int *cursor, SpaiColumnOut out,
-                      int *out_j, float *out_c, std::int64_t out_stride) {
+                      float eps, long long alpha, int beta, const int *worklist, int nwork,
+                      int *cursor, SpaiColumnOut out, int *out_j, float *out_c,
+                      std::int64_t out_stride) {
     hipLaunchKernelGGL(spai_wave_kernel, dim3(blocks), dim3(kWave), 0, s, aptr, acol, aval,
-                       dscale, bptr, bcol, bval, eps, alpha, beta, worklist, nwork, cursor, out,
-                       out_j, out_c, out_stride);
+                       dscale, bptr, bcol, eps, alpha, beta, worklist, nwork, cursor, out, out_j,
+                       out_c, out_stride);
     HIP_LAUNCH_CHECK();
 }
 int spai_wave_qcap() {
