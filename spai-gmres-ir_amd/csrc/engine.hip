@@ -746,8 +746,8 @@ void spai_build_typed(Context &c, const DevCsr &a, const T *aval, T *d, double e
             pass_labels.push_back("spai_wave_kernel<float>");
             timed(c, "spai_wave_kernel<float>", 0.0, 0.0, [&] {
                 launch_spai_wave(s, blocks, a.ptr.data(), a.col.data(), (const float *)aval,
-                                 (const float *)d, bptr.data(), bcol.data(), (float)eps,
-                                 alpha, beta,
+                                 (const float *)d, bptr.data(), bcol.data(),
+                                 (const float *)bval, (float)eps, alpha, beta,
                                  work.data(), nwork, counter.data(), o,
                                  out.pool_j.back().data(), out.pool_c.back().as<float>(), wq);
             }, true);

@@ -49,9 +49,9 @@ void launch_spai_columns(hipStream_t s, int blocks, int threads, int n, const in
 // (unscaled), aligned with bcol.
 void launch_spai_wave(hipStream_t s, int blocks, const int *aptr, const int *acol,
                       const float *aval, const float *dscale, const int *bptr, const int *bcol,
-                      float eps, long long alpha, int beta, const int *worklist, int nwork,
-                      int *cursor, SpaiColumnOut out, int *out_j, float *out_c,
-                      std::int64_t out_stride);
+                      const float *bval, float eps, long long alpha, int beta,
+                      const int *worklist, int nwork, int *cursor, SpaiColumnOut out,
+                      int *out_j, float *out_c, std::int64_t out_stride);
 int spai_wave_max_n();
 int spai_wave_qcap();
 // Workgroup-per-column kernel for the columns that outgrow the lane kernel.

@@ -662,10 +662,11 @@ template void launch_spai_columns<float>(hipStream_t, int, int, int, const int *
 //     reduce-scatter over the wave (lane 4c ends with column c);
 //   * candidates: the columns of B's rows i in I go into an open-addressing
 //     set in LDS, whose distinct members are sorted (so the sort sees about
-//     a quarter of the entries at 128^3) and J removed;
-//   * candidate scores: one lane per candidate walks row j of A (b_ij =
-//     a_ji) in CSR order, ascending i as the lane kernel sums it, finding
-//     each column's position in I by binary search;
+//     a quarter of the entries at 128^3) and J removed; the entries, with
+//     their values, are grouped by candidate in ascending i;
+//   * candidate scores: the terms res_i * b_ij d_j and (b_ij d_j)^2 are
+//     summed per candidate over its group, ascending i as the lane kernel
+//     sums A's row j, with no further loads;
 //   * the beta smallest (score, index) pairs are found by argmin rounds
 //     over the wave.
 // The method and its decisions are the lane kernel's; the FP32 summation
@@ -682,16 +683,22 @@ constexpr int kWH = 2 * kWC; // candidate set slots
 
 struct WaveLds {
     // R is read last by the residual, before the candidate stage; the
-    // candidate stage's table and list are read last before the scores.
-    // 9,088 bytes in all: 16 waves fit one CU's LDS.
+    // candidate stage's arrays are read last by the scores, before the next
+    // fit's gather.  9,088 bytes in all: 16 waves fit one CU's LDS.
     union {
         float R[kWM][kWW]; // gathered B(I, J) (unchanged by the QR, which runs in registers)
         struct {
-            int ht[kWH];    // candidate stage: open-addressing set of columns (-1: empty)
-            int ulist[kWH]; // candidate stage: the distinct columns, unordered
+            int ht[kWH]; // open-addressing set of candidate columns (-1: empty)
+            union {
+                int ulist[kWH];            // the set's members, (column << 8) | slot
+                unsigned short grp[kWS];   // entries grouped by slot, ascending in a group
+            };
+            int cur[kWH];                  // entries per slot, then the groups' cursors
+            unsigned char es[kWS];         // entry -> slot
+            unsigned char cslot[kWC];      // candidate -> slot
         };
     };
-    float sval[kWS];     // shadow stage: staged values by entry
+    float sval[kWS];     // staged values by entry (A's values in the shadow, B's for candidates)
     signed char sc[kWS]; // entry -> staging row
     int sbeg[kWM];       // staging row r holds entries e of source index sbeg[r] + e
     float sd[kWM];       // staging row scale
@@ -1055,8 +1062,9 @@ __device__ __attribute__((noinline)) int wave_shadow(WaveLds &W, const int *col,
     return wave_sort_set<8>(W, col, val, total, lane);
 }
 
-// The distinct columns of the list, ascending, minus J -> cand; returns
-// their count.  P keys per lane hold the n <= 64 P list entries.
+// The distinct columns of the list ((column << 8) | slot), ascending, minus
+// J -> cand and cslot; returns their count.  P keys per lane hold the
+// n <= 64 P list entries.
 template <int P>
 __device__ int wave_sorted_candidates(WaveLds &W, int n, int q, int lane) {
     unsigned key[P];
@@ -1072,7 +1080,7 @@ __device__ int wave_sorted_candidates(WaveLds &W, int n, int q, int lane) {
     for (int i = 0; i < P; ++i) {
         bool in_j = false;
         if (key[i] != ~0u) {
-            const int x = (int)key[i];
+            const int x = (int)(key[i] >> 8);
             int lo = 0, hi = q;
             while (lo < hi) {
                 const int mid = (lo + hi) >> 1;
@@ -1090,31 +1098,46 @@ __device__ int wave_sorted_candidates(WaveLds &W, int n, int q, int lane) {
     int at = wave_scan_excl(cnt, lane);
 #pragma unroll
     for (int i = 0; i < P; ++i)
-        if (keep[i])
-            W.cand[at++] = (int)key[i];
+        if (keep[i]) {
+            W.cand[at] = (int)(key[i] >> 8);
+            W.cslot[at] = (unsigned char)(key[i] & 255u);
+            ++at;
+        }
     wave_sync();
     return nc;
 }
 
 // The candidates of a fit: the distinct columns of B's rows i in I (staged
 // entries e < total), minus J, ascending in cand; returns their count, or
-// kWC + 1 when the set (J included) exceeds kWC.  Each entry's column goes
-// into an open-addressing set in LDS (multiplicative hash, linear probing
-// bounded by the table size; a compare-and-swap inserts or finds it), whose
-// members are then compacted and sorted: the sort sees the distinct columns
-// only, not every entry.
+// kWC + 1 when the set (J included) exceeds kWC.
+//   * Each entry's column goes into an open-addressing set in LDS
+//     (multiplicative hash, linear probing bounded by the table size; a
+//     compare-and-swap inserts or finds it); the entry keeps its slot and
+//     its value b_ij, and the slot counts its entries.
+//   * The set's members are compacted and sorted (the sort sees the distinct
+//     columns only, not every entry), J removed.
+//   * The entries are grouped by slot, each group in ascending entry order
+//     (a stable counting sort: group starts by a scan of the counts, ranks
+//     within a 64-entry round from eight ballots on the slot bits), so that
+//     candidate j finds its terms in ascending i with no further loads.
 __device__ __attribute__((noinline)) int wave_candidates(WaveLds &W, const int *__restrict__ bcol,
+                                                         const float *__restrict__ bval,
                                                          int total, int q, int lane) {
+    constexpr int kPer = kWH / kWave;
 #pragma unroll
-    for (int u = 0; u < kWH / kWave; ++u)
+    for (int u = 0; u < kPer; ++u) {
         W.ht[lane + u * kWave] = -1;
+        W.cur[lane + u * kWave] = 0;
+    }
     wave_sync();
     int added = 0;
     bool full = false;
     for (int e0 = 0; e0 < total; e0 += kWave) {
         const int e = e0 + lane;
         if (e < total) {
-            const int x = gld(bcol + W.sbeg[W.sc[e]] + e);
+            const int src = W.sbeg[W.sc[e]] + e;
+            const int x = gld(bcol + src);
+            W.sval[e] = gld(bval + src);
             unsigned h = ((unsigned)x * 2654435761u) >> 24; // kWH = 256 slots
             int probe = 0;
             for (; probe < kWH; ++probe) {
@@ -1128,71 +1151,88 @@ __device__ __attribute__((noinline)) int wave_candidates(WaveLds &W, const int *
                 h = (h + 1) & (kWH - 1);
             }
             full = full || probe == kWH;
+            W.es[e] = (unsigned char)h;
+            atomicAdd(&W.cur[h], 1);
         }
     }
     wave_sync();
     const int nu = wave_sum_int(added);
     if (__any(full) || nu > kWC)
         return kWC + 1;
-    // compaction: slots lane * 4 .. lane * 4 + 3 in slot order
-    constexpr int kPer = kWH / kWave;
-    int v[kPer], cnt = 0;
+    // compaction: slots lane * kPer .. in slot order, (column << 8) | slot
+    {
+        int v[kPer], cnt = 0;
 #pragma unroll
-    for (int u = 0; u < kPer; ++u) {
-        v[u] = W.ht[lane * kPer + u];
-        cnt += v[u] >= 0 ? 1 : 0;
+        for (int u = 0; u < kPer; ++u) {
+            v[u] = W.ht[lane * kPer + u];
+            cnt += v[u] >= 0 ? 1 : 0;
+        }
+        int at = wave_scan_excl(cnt, lane);
+#pragma unroll
+        for (int u = 0; u < kPer; ++u)
+            if (v[u] >= 0)
+                W.ulist[at++] = (v[u] << 8) | (lane * kPer + u);
     }
-    int at = wave_scan_excl(cnt, lane);
-#pragma unroll
-    for (int u = 0; u < kPer; ++u)
-        if (v[u] >= 0)
-            W.ulist[at++] = v[u];
     wave_sync();
-    if (nu <= kWave)
-        return wave_sorted_candidates<1>(W, nu, q, lane);
-    return wave_sorted_candidates<2>(W, nu, q, lane);
+    const int nc = nu <= kWave ? wave_sorted_candidates<1>(W, nu, q, lane)
+                               : wave_sorted_candidates<2>(W, nu, q, lane);
+    // group starts: exclusive scan of the slot counts, in slot order
+    {
+        int c[kPer], sum = 0;
+#pragma unroll
+        for (int u = 0; u < kPer; ++u) {
+            c[u] = W.cur[lane * kPer + u];
+            sum += c[u];
+        }
+        int at = wave_scan_excl(sum, lane);
+#pragma unroll
+        for (int u = 0; u < kPer; ++u) {
+            W.cur[lane * kPer + u] = at;
+            at += c[u];
+        }
+    }
+    wave_sync(); // (the sort has read the list: its space now takes grp)
+    const unsigned long long below = (1ull << lane) - 1ull;
+    for (int e0 = 0; e0 < total; e0 += kWave) {
+        const int e = e0 + lane;
+        const bool valid = e < total;
+        const int sl = valid ? (int)W.es[e] : 0;
+        unsigned long long peers = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+            const bool bit = (sl >> b) & 1;
+            const unsigned long long m = __ballot(bit);
+            peers &= bit ? m : ~m;
+        }
+        const int rank = __popcll(peers & below);
+        const int pos = valid ? W.cur[sl] + rank : 0;
+        wave_sync(); // every lane has read its group's cursor
+        if (valid) {
+            W.grp[pos] = (unsigned short)e;
+            if (rank == 0)
+                W.cur[sl] += __popcll(peers);
+        }
+        wave_sync();
+    }
+    // each cursor now ends its group; a group starts where the previous
+    // slot's ends
+    return nc;
 }
 
-// dot = sum res_i b_ij d_j and den = sum (b_ij d_j)^2 over the rows i in I of
-// candidate j, ascending i: row j of A in CSR order (b_ij = a_ji), each
-// column's position in the sorted I (m <= 64) found by a branch-free lower
-// bound.  Entries go in batches of 8, loaded together and searched in lock
-// step, so that the batch's LDS reads of one step are in flight together.
-__device__ inline void candidate_terms(const WaveLds &W, const int *__restrict__ aptr,
-                                       const int *__restrict__ acol,
-                                       const float *__restrict__ aval, float dj, int j, int m,
-                                       float &dot, float &den) {
+// dot = sum res_i b_ij d_j and den = sum (b_ij d_j)^2 over candidate t's
+// entries (one per row i of I holding column j), ascending i: the group of
+// its slot, in entry order.
+__device__ inline void candidate_terms(const WaveLds &W, int t, float dj, float &dot,
+                                       float &den) {
+    const int sl = W.cslot[t];
+    const int p1 = W.cur[sl], p0 = sl > 0 ? W.cur[sl - 1] : 0;
     dot = 0.f;
     den = 0.f;
-    const int e1 = gld(aptr + j + 1);
-    for (int e0 = gld(aptr + j); e0 < e1; e0 += 8) {
-        int c[8], pos[8];
-        float a[8];
-#pragma unroll
-        for (int u = 0; u < 8; ++u) {
-            const int e = min(e0 + u, e1 - 1);
-            c[u] = gld(acol + e);
-            a[u] = gld(aval + e);
-            pos[u] = 0;
-        }
-        // pos <- the number of entries of I below c (every read below m)
-#pragma unroll
-        for (int step = kWM; step > 0; step >>= 1) {
-#pragma unroll
-            for (int u = 0; u < 8; ++u) {
-                const int t = pos[u] + step;
-                if (t <= m && W.I[t - 1] < c[u])
-                    pos[u] = t;
-            }
-        }
-#pragma unroll
-        for (int u = 0; u < 8; ++u) {
-            if (e0 + u < e1 && pos[u] < m && W.I[pos[u]] == c[u]) {
-                const float value = a[u] * dj;
-                dot += W.res[pos[u]] * value;
-                den += value * value;
-            }
-        }
+    for (int p = p0; p < p1; ++p) {
+        const int e = W.grp[p];
+        const float value = W.sval[e] * dj;
+        dot += W.res[W.sc[e]] * value;
+        den += value * value;
     }
 }
 
@@ -1235,10 +1275,10 @@ __device__ inline float wave_reduce_scatter16(float (&a)[kWQ], int lane) {
 __global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4)))
 spai_wave_kernel(const int *__restrict__ aptr, const int *__restrict__ acol,
                  const float *__restrict__ aval, const float *__restrict__ dscale,
-                 const int *__restrict__ bptr, const int *__restrict__ bcol, float eps,
-                 long long alpha, int beta, const int *__restrict__ worklist, int nwork,
-                 int *__restrict__ cursor, SpaiColumnOut out, int *__restrict__ out_j,
-                 float *__restrict__ out_c,
+                 const int *__restrict__ bptr, const int *__restrict__ bcol,
+                 const float *__restrict__ bval, float eps, long long alpha, int beta,
+                 const int *__restrict__ worklist, int nwork, int *__restrict__ cursor,
+                 SpaiColumnOut out, int *__restrict__ out_j, float *__restrict__ out_c,
                  std::int64_t out_stride) {
     __shared__ WaveLds W;
     const int lane = threadIdx.x;
@@ -1458,7 +1498,7 @@ spai_wave_kernel(const int *__restrict__ aptr, const int *__restrict__ acol,
                 break;
             }
             const int nc =
-                __builtin_amdgcn_readfirstlane(wave_candidates(W, bcol, nb, q, lane));
+                __builtin_amdgcn_readfirstlane(wave_candidates(W, bcol, bval, nb, q, lane));
             if (nc + q > kWC) {
                 status = kSpaiOverflow;
                 break;
@@ -1471,14 +1511,19 @@ spai_wave_kernel(const int *__restrict__ aptr, const int *__restrict__ acol,
             const float rr = rnorm * rnorm;
             int first_bad = INT_MAX, bad_status = kSpaiSuccess, cl_before[kWC / kWave];
             float msum_l = 0.f; // the lane's scores, in its order
+            float cdj[kWC / kWave];
+#pragma unroll
+            for (int u = 0; u < kWC / kWave; ++u) {
+                const int t = lane + u * kWave;
+                cdj[u] = t < nc ? gld(dscale + W.cand[t]) : 0.f;
+            }
 #pragma unroll
             for (int u = 0; u < kWC / kWave; ++u) {
                 const int t = lane + u * kWave;
                 cl_before[u] = 0;
                 if (t < nc) {
-                    const int j = W.cand[t];
                     float dt, den;
-                    candidate_terms(W, aptr, acol, aval, gld(dscale + j), j, m, dt, den);
+                    candidate_terms(W, t, cdj[u], dt, den);
                     float sv = 0.f;
                     bool cl = false;
                     const int st = score_of(rr, dt, den, sv, cl);
@@ -1593,12 +1638,12 @@ spai_wave_kernel(const int *__restrict__ aptr, const int *__restrict__ acol,
 
 void launch_spai_wave(hipStream_t s, int blocks, const int *aptr, const int *acol,
                       const float *aval, const float *dscale, const int *bptr, const int *bcol,
-                      float eps, long long alpha, int beta, const int *worklist, int nwork,
-                      int *cursor, SpaiColumnOut out, int *out_j, float *out_c,
-                      std::int64_t out_stride) {
+                      const float *bval, float eps, long long alpha, int beta,
+                      const int *worklist, int nwork, int *cursor, SpaiColumnOut out,
+                      int *out_j, float *out_c, std::int64_t out_stride) {
     hipLaunchKernelGGL(spai_wave_kernel, dim3(blocks), dim3(kWave), 0, s, aptr, acol, aval,
-                       dscale, bptr, bcol, eps, alpha, beta, worklist, nwork, cursor, out, out_j,
-                       out_c, out_stride);
+                       dscale, bptr, bcol, bval, eps, alpha, beta, worklist, nwork, cursor, out,
+                       out_j, out_c, out_stride);
     HIP_LAUNCH_CHECK();
 }
 int spai_wave_qcap() {
