@@ -714,14 +714,19 @@ __device__ inline void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
     __builtin_amdgcn_wave_barrier();
 }
+// Exclusive prefix sum over the wave by DPP (no LDS crossbar): shifts by 1,
+// 2, 4 and 8 lanes within each row of 16 (lanes without a source add the
+// old value 0), then row 0's total into row 1 and row 2's into row 3 (row
+// broadcast of lane 15), then lane 31's total into rows 2 and 3.
 __device__ inline int wave_scan_excl(int v, int lane) {
+    (void)lane;
     int x = v;
-#pragma unroll
-    for (int off = 1; off < kWave; off <<= 1) {
-        const int y = __shfl_up(x, off, kWave);
-        if (lane >= off)
-            x += y;
-    }
+    x += __builtin_amdgcn_update_dpp(0, x, 0x111, 0xf, 0xf, false); // row_shr:1
+    x += __builtin_amdgcn_update_dpp(0, x, 0x112, 0xf, 0xf, false); // row_shr:2
+    x += __builtin_amdgcn_update_dpp(0, x, 0x114, 0xf, 0xf, false); // row_shr:4
+    x += __builtin_amdgcn_update_dpp(0, x, 0x118, 0xf, 0xf, false); // row_shr:8
+    x += __builtin_amdgcn_update_dpp(0, x, 0x142, 0xa, 0xf, false); // row_bcast:15
+    x += __builtin_amdgcn_update_dpp(0, x, 0x143, 0xc, 0xf, false); // row_bcast:31
     return x - v;
 }
 // Cross-lane moves without the LDS crossbar (gfx950; every lane active):
@@ -893,7 +898,7 @@ __device__ int wave_rows(WaveLds &W, const int *__restrict__ ptr,
     const float d = has && dscale ? dscale[row] : 1.f;
     const int len = e1 - e0;
     const int off = wave_scan_excl(len, lane);
-    const int total = __shfl(off + len, kWave - 1, kWave);
+    const int total = __builtin_amdgcn_readlane(off + len, kWave - 1);
     if (total > kWS)
         return -1;
     W.sbeg[lane] = e0 - off;
@@ -1015,9 +1020,8 @@ __device__ int wave_sort_set(WaveLds &W, const int *__restrict__ col,
         kv[i] = key[i] == ~0u ? -1 : (int)(key[i] >> 9);
         ev[i] = (int)(key[i] & 511u);
     }
-    int prev = __shfl_up(kv[P - 1], 1, kWave);
-    if (lane == 0)
-        prev = -2;
+    // the previous lane's last key (DPP wave_shr:1; lane 0 keeps -2)
+    const int prev = __builtin_amdgcn_update_dpp(-2, kv[P - 1], 0x138, 0xf, 0xf, false);
     int cnt = 0;
 #pragma unroll
     for (int i = 0; i < P; ++i) {
@@ -1286,7 +1290,7 @@ spai_wave_kernel(const int *__restrict__ aptr, const int *__restrict__ acol,
         int slot = 0;
         if (lane == 0)
             slot = atomicAdd(cursor, 1);
-        slot = __shfl(slot, 0, kWave);
+        slot = __builtin_amdgcn_readfirstlane(slot);
         if (slot >= nwork)
             break;
         const int k = worklist[slot];
@@ -1612,7 +1616,7 @@ spai_wave_kernel(const int *__restrict__ aptr, const int *__restrict__ acol,
                     mine = W.sel[lane - q];
                 int rank = 0;
                 for (int t = 0; t < nq; ++t)
-                    rank += __shfl(mine, t, kWave) < mine ? 1 : 0;
+                    rank += __builtin_amdgcn_readlane(mine, t) < mine ? 1 : 0;
                 wave_sync();
                 if (lane < nq)
                     W.J[rank] = mine;
