@@ -1113,7 +1113,6 @@ void GmresEngine<T>::solve(const DevCsr &m1, const DevCsr &m2, GmresMode mode, c
     const std::string lproj = std::string("project_kernel<") + tname + ">";
     const std::string lred = std::string("gram_reduce_kernel<") + tname + ">";
     const std::string lsol = std::string("gram_solve_kernel<") + tname + ">";
-    const std::string lsol_lds = std::string("gram_solve_lds_kernel<") + tname + ">";
     const std::string lupd = std::string("gram_update_kernel<") + tname + ">";
     const std::string lnrm = std::string("normalize_givens_kernel<") + tname + ">";
     const double sz = sizeof(T), nn = (double)L_.n;
@@ -1206,10 +1205,11 @@ void GmresEngine<T>::solve(const DevCsr &m1, const DevCsr &m2, GmresMode mode, c
               [&] { gmres_project<T>(s, tl, a, k, kmax); });
         timed(c_, lred.c_str(), 2.0 * (2 * k + 1) * tl.ntiles * 8.0, 0.0,
               [&] { gmres_gram_reduce<T>(s, tl, a, k, kmax); });
-        timed(c_, (kmax <= kGramSolveLdsMaxK ? lsol_lds : lsol).c_str(), 0.0, 0.0,
-              [&] { gmres_gram_solve<T>(s, tl, a, k, kmax); });
+        // (for kmax <= kGramSolveLdsMaxK the update pass also solves for h)
+        if (!gmres_gram_solve_fused(kmax))
+            timed(c_, lsol.c_str(), 0.0, 0.0, [&] { gmres_gram_solve<T>(s, tl, a, k, kmax); });
         timed(c_, lupd.c_str(), (k + 3) * nn * sz, 2.0 * (k + 1) * nn,
-              [&] { gmres_gram_update<T>(s, tl, a, k); });
+              [&] { gmres_gram_update<T>(s, tl, a, k, kmax); });
         timed(c_, lnrm.c_str(), 2.0 * nn * sz, 0.0,
               [&] { gmres_normalize_givens<T>(s, tl, a, k); });
         if (verify_now)

@@ -487,16 +487,20 @@ __global__ void __launch_bounds__(kWave) gram_solve_kernel(Tiles tiles, KrylovAr
     }
 }
 
-// The same solve for k < 512 with a 1024-thread block: all waves stage
-// column chunks of L in LDS (every load of a chunk in flight at once), then
-// wave 0 runs the substitution from LDS, unknown j in lane j % 64, slot
-// j / 64, each pivot broadcast by readlane.  Every h_j again
+// The same solve for k < 512, fused into the update pass (gram_update_kernel
+// with S > 0): every update block solves its system's (I + L) h = q itself
+// before it streams the basis, so no launch separates the two passes.  All
+// waves stage column chunks of L in LDS (every load of a chunk in flight at
+// once), then wave 0 runs the substitution from LDS, unknown j in lane
+// j % 64, slot j / 64, each pivot broadcast by readlane.  Every h_j again
 // sees its terms in the order i = 0, 1, ..., j-1 (bit-identical to the
-// kernel above); the rotations follow from LDS.
-constexpr int kSolveThreads = 1024;
-// S unknowns per lane (k < 64 S): 4 for k < 256 with 48 KiB chunks of L, 8
-// up to k < 512 with 40 KiB chunks (the larger slot arrays share the LDS).
-template <int S> constexpr int kSolveChunk = S == 4 ? 6144 : 5120;
+// kernel above), so every block of a system holds the same h.  The system's
+// first block then applies the older rotations from LDS and writes the
+// column of H.
+// S unknowns per lane (k < 64 S): 4 for k < 256 with 24 KiB chunks of L, 8
+// up to k < 512 with 18 KiB chunks: with h and the pads, 30 KiB of LDS, so
+// that five update blocks (their register budget) still share a CU.
+template <int S> constexpr int kSolveChunk = S == 4 ? 3072 : 2304;
 static_assert(kGramSolveLdsMaxK == 8 * kWave - 1, "LDS Gram solve capacity");
 
 __device__ inline double readlane_f64(double v, int l) {
@@ -561,35 +565,22 @@ __device__ inline void gram_steps(double (&acc)[S], const double *Lc, int &i, in
     }
 }
 
-template <typename T, int S>
-__global__ void __launch_bounds__(kSolveThreads)
-gram_solve_lds_kernel(Tiles tiles, KrylovArgs<T> a, int k) {
-    k = step_index(a, k);
-    if (k >= a.cap) // a replayed graph's steps beyond the cap do nothing
-        return;
+// (I + L) h = q for system s at step k < 64 S by a block of NT threads (the
+// chunk loop and substitution described above); h_0..h_k end in hs, visible
+// to the whole block.
+template <typename T, int S, int NT>
+__device__ void gram_substitute(const KrylovArgs<T> &a, int s, int k, double *Lc, double *hs) {
     constexpr int kSolvePad = S * kWave, kChunk = kSolveChunk<S>;
-    __shared__ double Lc[kSolvePad + kChunk + kSolvePad];
-    __shared__ double hs[S * kWave], lcs[S * kWave], lsn[S * kWave];
-    const int s = blockIdx.x;
-    if (!a.st[s].active || !a.st[s].run[k & 1])
-        return;
+    static_assert(kChunk % NT == 0, "chunk entries per thread");
     const int tid = threadIdx.x, wave = tid / kWave, lane = tid % kWave, cap = a.cap;
     const double *q = a.q + (std::int64_t)s * (cap + 1);
     const double *L = a.gram + (std::int64_t)s * a.gstride;
-    double *h = a.h + (std::int64_t)s * (cap + 1);
-    double *col = a.H + (std::int64_t)s * a.hstride + hcol_offset(k);
-    const double *cs = a.cs + (std::int64_t)s * cap, *sn = a.sn + (std::int64_t)s * cap;
-    // The rotations (k <= 511 < kSolveThreads: one each) and q are loaded
-    // with the first chunk of L, so the three are one round of latency.
-    const bool rot = tid < k;
-    const double ccs = rot ? cs[tid] : 0.0, csn = rot ? sn[tid] : 0.0;
     double acc[S];
 #pragma unroll
     for (int u = 0; u < S; ++u) {
         const int j = lane + u * kWave;
         acc[u] = wave == 0 && j <= k ? q[j] : 0.0;
     }
-    bool staged = false;
     // Steps i = 0..k-1 (column i of L holds rows j = i+1..k), in chunks of
     // whole columns; column i of a chunk starting at c0 begins at entry
     // sum_{c0 <= i' < i} (k - i').
@@ -606,13 +597,13 @@ gram_solve_lds_kernel(Tiles tiles, KrylovArgs<T> a, int k) {
         // (2k+1)^2 - 8e < 2^24 is exact in FP32, whose rounded root is within
         // the one-step correction.
         {
-            constexpr int R = kChunk / kSolveThreads;
+            constexpr int R = kChunk / NT;
             const long long base = (long long)c0 * (2 * k + 1 - c0) / 2;
             const float b2 = 2.0f * (float)k + 1.0f;
             int src[R];
 #pragma unroll
             for (int u = 0; u < R; ++u) {
-                const long long e = base + min(tid + u * kSolveThreads, used - 1);
+                const long long e = base + min(tid + u * NT, used - 1);
                 int i = (int)floorf((b2 - sqrtf(b2 * b2 - 8.0f * (float)e)) * 0.5f);
                 i = max(c0, min(i, c1 - 1));
                 if ((long long)i * (2 * k + 1 - i) / 2 > e)
@@ -628,13 +619,8 @@ gram_solve_lds_kernel(Tiles tiles, KrylovArgs<T> a, int k) {
                 v[u] = L[src[u]];
 #pragma unroll
             for (int u = 0; u < R; ++u)
-                if (tid + u * kSolveThreads < used)
-                    Lc[kSolvePad + tid + u * kSolveThreads] = v[u];
-            if (!staged && rot) {
-                lcs[tid] = ccs;
-                lsn[tid] = csn;
-            }
-            staged = true;
+                if (tid + u * NT < used)
+                    Lc[kSolvePad + tid + u * NT] = v[u];
         }
         __syncthreads();
         if (wave == 0) {
@@ -653,22 +639,35 @@ gram_solve_lds_kernel(Tiles tiles, KrylovArgs<T> a, int k) {
         __syncthreads();
         c0 = c1;
     }
-    if (wave != 0)
-        return;
+    if (wave == 0) {
 #pragma unroll
-    for (int u = 0; u < S; ++u) {
-        const int j = lane + u * kWave;
-        if (j <= k) {
-            h[j] = acc[u];
-            hs[j] = acc[u];
+        for (int u = 0; u < S; ++u) {
+            const int j = lane + u * kWave;
+            if (j <= k)
+                hs[j] = acc[u];
         }
     }
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-    // Rotations 0..k-1 on rows 0..k (src/gmres.cpp:183-189), lane 0.  A
-    // non-finite intermediate carried row stays non-finite to the end, so
-    // finiteness is checked once over the rotated column afterwards.
+    __syncthreads();
+}
+
+// Rotations 0..k-1 on rows 0..k of h (src/gmres.cpp:183-189) by lane 0 of
+// wave 0, cs and sn staged in buf (2 x 512 entries) by the block; the
+// rotated column goes to H and its finiteness to rot_bad.  A non-finite
+// intermediate carried row stays non-finite to the end, so finiteness is
+// checked once over the rotated column afterwards.  hs is overwritten.
+template <typename T, int NT>
+__device__ void rotate_column(const KrylovArgs<T> &a, int s, int k, double *hs, double *buf) {
+    const int tid = threadIdx.x, wave = tid / kWave, lane = tid % kWave, cap = a.cap;
+    const double *cs = a.cs + (std::int64_t)s * cap, *sn = a.sn + (std::int64_t)s * cap;
+    __syncthreads(); // every read of hs and buf by the block is done
+    for (int r = tid; r < k; r += NT) {
+        buf[r] = cs[r];
+        buf[kGramSolveLdsMaxK + 1 + r] = sn[r];
+    }
+    __syncthreads();
+    if (wave != 0)
+        return;
+    const double *lcs = buf, *lsn = buf + kGramSolveLdsMaxK + 1;
     if (lane == 0) {
         double up = hs[0];
         constexpr int B = 8; // LDS reads batched ahead of the rotation chain
@@ -695,6 +694,7 @@ gram_solve_lds_kernel(Tiles tiles, KrylovArgs<T> a, int k) {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    double *col = a.H + (std::int64_t)s * a.hstride + hcol_offset(k);
     bool bad = false;
     for (int r = lane; r <= k; r += kWave) {
         const double v = hs[r];
@@ -743,7 +743,10 @@ __device__ inline void upd_apply(T (&x)[G][E], const typename Vec16<T>::type (&b
     }
 }
 
-template <typename T>
+// S = 0: h from the Gram solve kernel; S = 4 or 8: the block solves for h
+// first (gram_substitute, k < 64 S), and the system's first block applies
+// the rotations after its update.
+template <typename T, int S>
 __global__ void __launch_bounds__(kUpdateThreads)
 gram_update_kernel(Tiles tiles, KrylovArgs<T> a, int k) {
     k = step_index(a, k);
@@ -756,7 +759,18 @@ gram_update_kernel(Tiles tiles, KrylovArgs<T> a, int k) {
     BSPAI_SYS_TILES
     if (!a.st[s].active || !a.st[s].run[k & 1])
         return;
-    const double *h = a.h + (std::int64_t)s * (a.cap + 1);
+    const double *h;
+    [[maybe_unused]] double *hsolved = nullptr, *lbuf = nullptr;
+    if constexpr (S > 0) {
+        __shared__ double Lc[2 * S * kWave + kSolveChunk<S>];
+        __shared__ double hs[S * kWave];
+        gram_substitute<T, S, kUpdateThreads>(a, s, k, Lc, hs);
+        h = hs;
+        hsolved = hs;
+        lbuf = Lc;
+    } else {
+        h = a.h + (std::int64_t)s * (a.cap + 1);
+    }
     const int r0 = tiles.row0[t], nr = tiles.row1[t] - r0;
     const int tid = threadIdx.x;
     T *__restrict__ w = a.w + r0;
@@ -811,6 +825,10 @@ gram_update_kernel(Tiles tiles, KrylovArgs<T> a, int k) {
         for (int e = 0; e < E; ++e)
             vals[g * E + e] = (double)x[g][e];
     store_norm_partial(vals, G * E, slot_ptr(a.part, tiles.ntiles, kSlotNorm), t, lds);
+    if constexpr (S > 0) {
+        if (t == t0)
+            rotate_column<T, kUpdateThreads>(a, s, k, hsolved, lbuf);
+    }
 }
 
 // h_{k+1,k} = ||w||; V_{k+1} = w / h (unless breakdown); the writer runs the
@@ -1160,22 +1178,28 @@ void gmres_gram_reduce(hipStream_t s, const Tiles &tl, const KrylovArgs<T> &a, i
                        dim3(kReduceWaves * kWave), 0, s, tl, a, k);
     HIP_LAUNCH_CHECK();
 }
+bool gmres_gram_solve_fused(int kmax) {
+    return kmax <= kGramSolveLdsMaxK;
+}
 template <typename T>
 void gmres_gram_solve(hipStream_t s, const Tiles &tl, const KrylovArgs<T> &a, int k, int kmax) {
-    // the variant that holds the largest step the launch may see
-    if (kmax < 4 * kWave)
-        hipLaunchKernelGGL((gram_solve_lds_kernel<T, 4>), dim3(tl.nsys), dim3(kSolveThreads), 0,
-                           s, tl, a, k);
-    else if (kmax <= kGramSolveLdsMaxK)
-        hipLaunchKernelGGL((gram_solve_lds_kernel<T, 8>), dim3(tl.nsys), dim3(kSolveThreads), 0,
-                           s, tl, a, k);
-    else
-        hipLaunchKernelGGL(gram_solve_kernel<T>, dim3(tl.nsys), dim3(kWave), 0, s, tl, a, k);
+    if (gmres_gram_solve_fused(kmax)) // the update pass solves
+        return;
+    hipLaunchKernelGGL(gram_solve_kernel<T>, dim3(tl.nsys), dim3(kWave), 0, s, tl, a, k);
     HIP_LAUNCH_CHECK();
 }
-template <typename T> void gmres_gram_update(hipStream_t s, const Tiles &tl, const KrylovArgs<T> &a, int k) {
-    hipLaunchKernelGGL(gram_update_kernel<T>, dim3(tl.ntiles), dim3(kUpdateThreads), 0, s, tl, a,
-                       k);
+template <typename T>
+void gmres_gram_update(hipStream_t s, const Tiles &tl, const KrylovArgs<T> &a, int k, int kmax) {
+    // the variant that holds the largest step the launch may see
+    if (kmax < 4 * kWave)
+        hipLaunchKernelGGL((gram_update_kernel<T, 4>), dim3(tl.ntiles), dim3(kUpdateThreads), 0,
+                           s, tl, a, k);
+    else if (gmres_gram_solve_fused(kmax))
+        hipLaunchKernelGGL((gram_update_kernel<T, 8>), dim3(tl.ntiles), dim3(kUpdateThreads), 0,
+                           s, tl, a, k);
+    else
+        hipLaunchKernelGGL((gram_update_kernel<T, 0>), dim3(tl.ntiles), dim3(kUpdateThreads), 0,
+                           s, tl, a, k);
     HIP_LAUNCH_CHECK();
 }
 template <typename T>
@@ -1231,7 +1255,8 @@ void gmres_diff_partial(hipStream_t s, const Tiles &tl, SysState *st, double *pa
                                        int);                                                      \
     template void gmres_gram_solve<T>(hipStream_t, const Tiles &, const KrylovArgs<T> &, int,     \
                                       int);                                                       \
-    template void gmres_gram_update<T>(hipStream_t, const Tiles &, const KrylovArgs<T> &, int);   \
+    template void gmres_gram_update<T>(hipStream_t, const Tiles &, const KrylovArgs<T> &, int,    \
+                                       int);                                                      \
     template void gmres_normalize_givens<T>(hipStream_t, const Tiles &, const KrylovArgs<T> &,    \
                                             int);                                                 \
     template void gmres_candidate<T>(hipStream_t, const Tiles &, const KrylovArgs<T> &);          \

@@ -154,7 +154,11 @@ template <typename T>
 void gmres_gram_reduce(hipStream_t s, const Tiles &tl, const KrylovArgs<T> &a, int k, int kmax);
 template <typename T>
 void gmres_gram_solve(hipStream_t s, const Tiles &tl, const KrylovArgs<T> &a, int k, int kmax);
-template <typename T> void gmres_gram_update(hipStream_t s, const Tiles &tl, const KrylovArgs<T> &a, int k);
+// Gram solves for launches that may see steps up to kmax <= kGramSolveLdsMaxK
+// run inside the update pass (gmres_gram_solve then launches nothing).
+bool gmres_gram_solve_fused(int kmax);
+template <typename T>
+void gmres_gram_update(hipStream_t s, const Tiles &tl, const KrylovArgs<T> &a, int k, int kmax);
 template <typename T>
 void gmres_normalize_givens(hipStream_t s, const Tiles &tl, const KrylovArgs<T> &a, int k);
 // *kdev = k (one thread): the step a replayed graph starts at.
