@@ -903,21 +903,74 @@ normalize_givens_kernel(Tiles tiles, KrylovArgs<T> a, int k) {
 }
 
 // Candidate x = x0 + sum_{j<count} y_j V_j (FP64, j ascending, src/gmres.cpp:60-63).
+// Each thread owns the update pass's rows (G 16-byte groups) and streams the
+// basis the same way, two batches of loads in flight; every row still sees
+// x0 + y_0 V_0 + y_1 V_1 + ... in that order.
 template <typename T>
-__global__ void __launch_bounds__(kTileThreads)
+__global__ void __launch_bounds__(kUpdateThreads)
 candidate_kernel(Tiles tiles, KrylovArgs<T> a) {
     const int t = blockIdx.x, s = tiles.sys[t];
     const SysState &st = a.st[s];
     if (!st.active || !st.triggered)
         return;
+    using V16 = typename Vec16<T>::type;
+    constexpr int E = 16 / sizeof(T);
+    constexpr int G = kTileRows / (kUpdateThreads * E);
+    constexpr int B = kUpdBatch<T>;
     const double *y = a.y + (std::int64_t)s * a.cap;
     const int count = st.ccount;
-    for (int r = tiles.row0[t] + threadIdx.x; r < tiles.row1[t]; r += kTileThreads) {
-        double x = a.x0 ? a.x0[r] : 0.0;
-        for (int j = 0; j < count; ++j)
-            x = x + (double)a.V[j][r] * y[j];
-        a.x[r] = x;
+    const int r0 = tiles.row0[t], nr = tiles.row1[t] - r0;
+    const int tid = threadIdx.x;
+    double x[G][E];
+#pragma unroll
+    for (int g = 0; g < G; ++g)
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            const int r = (g * kUpdateThreads + tid) * E + e;
+            x[g][e] = a.x0 && r < nr ? a.x0[r0 + r] : 0.0;
+        }
+    auto apply = [&](const V16(&buf)[B][G], int base) {
+#pragma unroll
+        for (int u = 0; u < B; ++u) {
+            if (base + u < count) {
+                const double yu = y[base + u];
+#pragma unroll
+                for (int g = 0; g < G; ++g)
+#pragma unroll
+                    for (int e = 0; e < E; ++e)
+                        x[g][e] = x[g][e] + (double)lane_of<T>(buf[u][g], e) * yu;
+            }
+        }
+    };
+    auto groups = [&](auto aligned, auto checked) {
+        constexpr bool A = decltype(aligned)::value, C = decltype(checked)::value;
+        V16 b0[B][G], b1[B][G];
+        upd_load<T, G, A, C>(b0, a, 0, count - 1, r0, tid, nr);
+        for (int base = 0; base < count; base += 2 * B) {
+            upd_load<T, G, A, C>(b1, a, base + B, count - 1, r0, tid, nr);
+            apply(b0, base);
+            if (base + B >= count)
+                break;
+            upd_load<T, G, A, C>(b0, a, base + 2 * B, count - 1, r0, tid, nr);
+            apply(b1, base + B);
+        }
+    };
+    if (count > 0) {
+        if (nr == kTileRows && r0 % E == 0)
+            groups(std::true_type{}, std::false_type{});
+        else if (nr == kTileRows)
+            groups(std::false_type{}, std::false_type{});
+        else
+            groups(std::false_type{}, std::true_type{});
     }
+#pragma unroll
+    for (int g = 0; g < G; ++g)
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            const int r = (g * kUpdateThreads + tid) * E + e;
+            if (r < nr)
+                a.x[r0 + r] = x[g][e];
+        }
 }
 
 // y = M x on the tile's rows (sequential row order), fused with the norm
@@ -1208,7 +1261,7 @@ void gmres_normalize_givens(hipStream_t s, const Tiles &tl, const KrylovArgs<T> 
     HIP_LAUNCH_CHECK();
 }
 template <typename T> void gmres_candidate(hipStream_t s, const Tiles &tl, const KrylovArgs<T> &a) {
-    hipLaunchKernelGGL(candidate_kernel<T>, BSPAI_GRID, tl, a);
+    hipLaunchKernelGGL(candidate_kernel<T>, dim3(tl.ntiles), dim3(kUpdateThreads), 0, s, tl, a);
     HIP_LAUNCH_CHECK();
 }
 template <typename T>
