@@ -853,35 +853,23 @@ __device__ inline float wave_max_f(float v) {
     swap32(x, y);
     return __uint_as_float(x) > __uint_as_float(y) ? __uint_as_float(x) : __uint_as_float(y);
 }
-// Minimum over the wave of 64-bit keys (every lane ends with the same key):
-// the stages of wave_sum_f on both halves, the smaller key kept at each.
-template <int kCtrl> __device__ inline unsigned long long dpp_u64(unsigned long long v) {
-    return ((unsigned long long)dpp_u<kCtrl>((unsigned)(v >> 32)) << 32) | dpp_u<kCtrl>((unsigned)v);
-}
-__device__ inline unsigned long long wave_min_u64(unsigned long long v) {
-    unsigned long long o = dpp_u64<kDppXor1>(v);
+// Minimum over the wave of 32-bit keys (every lane ends with the same key).
+__device__ inline unsigned wave_min_u32(unsigned v) {
+    unsigned o = dpp_u<kDppXor1>(v);
     v = o < v ? o : v;
-    o = dpp_u64<kDppXor2>(v);
+    o = dpp_u<kDppXor2>(v);
     v = o < v ? o : v;
-    o = dpp_u64<kDppXor7>(v);
+    o = dpp_u<kDppXor7>(v);
     v = o < v ? o : v;
-    o = dpp_u64<kDppXor8>(v);
+    o = dpp_u<kDppXor8>(v);
     v = o < v ? o : v;
-#pragma unroll
-    for (int st = 0; st < 2; ++st) {
-        unsigned xl = (unsigned)v, yl = xl, xh = (unsigned)(v >> 32), yh = xh;
-        if (st == 0) {
-            swap16(xl, yl);
-            swap16(xh, yh);
-        } else {
-            swap32(xl, yl);
-            swap32(xh, yh);
-        }
-        const unsigned long long a = ((unsigned long long)xh << 32) | xl,
-                                 b = ((unsigned long long)yh << 32) | yl;
-        v = a < b ? a : b;
-    }
-    return v;
+    unsigned x = v, y = x;
+    swap16(x, y);
+    v = x < y ? x : y;
+    x = v;
+    y = x;
+    swap32(x, y);
+    return x < y ? x : y;
 }
 __device__ inline float lane_value(float v, int src) {
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), src));
@@ -1565,19 +1553,20 @@ spai_wave_kernel(const int *__restrict__ aptr, const int *__restrict__ acol,
                 break;
             }
             // ---- the beta smallest (score, index) with score <= mean, by
-            // argmin rounds over 64-bit keys (score bits, candidate): scores
-            // are finite and >= 0, so their bits order as their values
-            unsigned long long skey[kWC / kWave];
+            // argmin rounds: scores are finite and >= 0, so their bits order
+            // as their values; candidates ascend with t, so among equal
+            // scores the smallest index is the smallest t (slot 0's lanes
+            // before slot 1's)
+            unsigned skey[kWC / kWave];
             int elig = 0;
 #pragma unroll
             for (int u = 0; u < kWC / kWave; ++u) {
                 const int t = lane + u * kWave;
-                skey[u] = ~0ull;
+                skey[u] = ~0u;
                 if (t < nc) {
                     const float st_ = W.score[t];
                     if (st_ <= mean) {
-                        const unsigned bits = st_ == 0.f ? 0u : __float_as_uint(st_);
-                        skey[u] = ((unsigned long long)bits << 32) | (unsigned)W.cand[t];
+                        skey[u] = st_ == 0.f ? 0u : __float_as_uint(st_);
                         ++elig;
                     }
                 }
@@ -1592,18 +1581,21 @@ spai_wave_kernel(const int *__restrict__ aptr, const int *__restrict__ acol,
                 status = kSpaiOverflow;
                 break;
             }
+            static_assert(kWC / kWave == 2, "two candidate slots per lane");
             for (int r = 0; r < nsel; ++r) {
-                unsigned long long lo = skey[0];
-#pragma unroll
-                for (int u = 1; u < kWC / kWave; ++u)
-                    lo = skey[u] < lo ? skey[u] : lo;
-                lo = wave_min_u64(lo);
-#pragma unroll
-                for (int u = 0; u < kWC / kWave; ++u)
-                    if (skey[u] == lo)
-                        skey[u] = ~0ull;
+                const unsigned lo = wave_min_u32(skey[0] < skey[1] ? skey[0] : skey[1]);
+                const unsigned long long m0 = __ballot(skey[0] == lo);
+                const unsigned long long m1 = __ballot(skey[1] == lo);
+                const int u = m0 != 0ull ? 0 : 1;
+                const int l = __ffsll((long long)(u == 0 ? m0 : m1)) - 1;
+                if (lane == l) {
+                    if (u == 0)
+                        skey[0] = ~0u;
+                    else
+                        skey[1] = ~0u;
+                }
                 if (lane == 0)
-                    W.sel[r] = (int)(unsigned)(lo & 0xffffffffull);
+                    W.sel[r] = W.cand[l + u * kWave];
             }
             wave_sync();
             // J <- sorted(J u selection)
