@@ -211,8 +211,8 @@ spmv_slice_kernel(Tiles tiles, const int *__restrict__ flag, int flag_stride, Sl
     const int row = row0 + lane;
     if (row >= row1)
         return;
-    const int s0 = sv.soff[sl], s1 = sv.soff[sl + 1];
-    const int len = sv.slen[row];
+    const int s0 = gld(sv.soff + sl), s1 = gld(sv.soff + sl + 1);
+    const int len = gld(sv.slen + row);
     const int width = (s1 - s0) / kSliceRows; // the slice's longest row
     const int base = s0 + lane;
     if (xs.tab)
@@ -229,10 +229,10 @@ spmv_slice_kernel(Tiles tiles, const int *__restrict__ flag, int flag_stride, Sl
         for (int u = 0; u < U; ++u) {
             const int q = base + min(j0 + u, width - 1) * kSliceRows;
             if constexpr (C16)
-                cj[u] = row + (int)sv.scol16[q];
+                cj[u] = row + (int)gld(sv.scol16 + q);
             else
-                cj[u] = sv.scol[q];
-            vv[u] = sval[q];
+                cj[u] = gld(sv.scol + q);
+            vv[u] = gld(sval + q);
         }
         TX xv[U];
 #pragma unroll
