@@ -32,6 +32,8 @@
 //
 // Precision: T is the Krylov vector/matrix type (double for the plain path,
 // float for the mixed path); Hessenberg, rotations, y and candidates are FP64.
+#include <cstdlib>
+#include <string>
 #include <type_traits>
 #include "common.hpp"
 #include "krylov.hpp"
@@ -1232,7 +1234,10 @@ void gmres_gram_reduce(hipStream_t s, const Tiles &tl, const KrylovArgs<T> &a, i
     HIP_LAUNCH_CHECK();
 }
 bool gmres_gram_solve_fused(int kmax) {
-    return kmax <= kGramSolveLdsMaxK;
+    // BSPAI_GRAM_FUSED=0 keeps the separate one-wave solve kernel at every
+    // step (the same bits; for comparisons)
+    const char *env = std::getenv("BSPAI_GRAM_FUSED");
+    return kmax <= kGramSolveLdsMaxK && !(env && std::string(env) == "0");
 }
 template <typename T>
 void gmres_gram_solve(hipStream_t s, const Tiles &tl, const KrylovArgs<T> &a, int k, int kmax) {

@@ -565,6 +565,49 @@ def test_graph_replay_same_bits(monkeypatch, case):
     assert out["1"][2][1] > 0  # graphs were replayed
 
 
+@pytest.mark.parametrize("case", ["c1_fp64", "cube32_mixed", "long_fp64", "batched"])
+def test_gram_solve_in_update_same_bits(monkeypatch, case):
+    """The Gram solve inside the update pass (S = 4 for launches up to step
+    255, 8 up to 511) against the separate one-wave solve kernel at every
+    step: identical statuses, counts and solution bits.  The long case (a
+    2-D system of 288^2 rows with the first fit of every SPAI column only,
+    epsilon 2, to 1e-10) runs past step 511, where both take the separate
+    kernel."""
+    import bspai_hip
+
+    P = _problems()
+    if case == "batched":
+        sb = P.many_systems(count=12, lmax=80)
+        a, b, systems = sb.matrix, sb.rhs, sb.offsets
+    elif case.startswith("long"):
+        a = P.convection_diffusion_2d(288)
+        b, systems = P.equal_rhs(a.n), None
+    else:
+        a = P.convection_diffusion_3d(32) if case.startswith("cube") else P.convection_diffusion_2d()
+        b, systems = P.equal_rhs(a.n), None
+    prec = "mixed" if case.endswith("mixed") else "fp64"
+    kw = dict(precision=prec,
+              tolerance=1e-10 if case.startswith("long") else 1e-8 if prec == "fp64" else 1e-4,
+              max_iterations=1500 if case.startswith("long") else 200,
+              relres_target=1e-12 if prec == "mixed" else 0.0,
+              epsilon=2.0 if case.startswith("long") else 0.3)
+    out = {}
+    for mode in ("1", "0"):
+        monkeypatch.setenv("BSPAI_GRAM_FUSED", mode)
+        ctx = bspai_hip.Context(0)
+        m = bspai_hip.Matrix(a, ctx, systems=systems)
+        r = bspai_hip.spai_gmres_ir(m, b, **kw)
+        rs = r if isinstance(r, list) else [r]
+        out[mode] = ([(x.status, x.outer_steps, x.inner_iterations) for x in rs],
+                     [bits(np.asarray(x.solution)) for x in rs])
+    assert out["0"][0] == out["1"][0]
+    assert all(s == "success" for s, _, _ in out["1"][0])
+    for x0, x1 in zip(out["0"][1], out["1"][1]):
+        assert np.array_equal(x0, x1)
+    if case.startswith("long"):
+        assert max(out["1"][0][0][2]) > 511
+
+
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_spai_bucket_autotune_keeps_bits(monkeypatch, precision):
     """The capacity-bucket autotuner times launch shapes of the workgroup SPAI
