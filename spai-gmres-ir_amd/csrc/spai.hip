@@ -684,7 +684,7 @@ constexpr int kWH = 2 * kWC; // candidate set slots
 struct WaveLds {
     // R is read last by the residual, before the candidate stage; the
     // candidate stage's arrays are read last by the scores, before the next
-    // fit's gather.  9,088 bytes in all: 16 waves fit one CU's LDS.
+    // fit's gather.  9,152 bytes in all: 16 waves fit one CU's LDS.
     union {
         float R[kWM][kWW]; // gathered B(I, J) (unchanged by the QR, which runs in registers)
         struct {
@@ -698,6 +698,7 @@ struct WaveLds {
             unsigned char cslot[kWC];      // candidate -> slot
         };
     };
+    alignas(16) float bq[kWQ]; // wave-uniform values of a step, broadcast by LDS reads
     float sval[kWS];     // staged values by entry (A's values in the shadow, B's for candidates)
     signed char sc[kWS]; // entry -> staging row
     int sbeg[kWM];       // staging row r holds entries e of source index sbeg[r] + e
@@ -1383,12 +1384,30 @@ spai_wave_kernel(const int *__restrict__ aptr, const int *__restrict__ acol,
                 for (int c = 0; c < kWQ; ++c)
                     p[c] = v * r[c];
                 const float d = wave_reduce_scatter16(p, lane);
+                // column c's sum (lanes 4c..4c+3) to every lane through LDS:
+                // four broadcast reads instead of 15 readlanes (the kernel is
+                // bound by VALU issue; LDS reads are not VALU instructions)
+                if ((lane & 3) == 0)
+                    W.bq[lane >> 2] = d;
+                wave_sync();
+                float dq[kWQ];
+#pragma unroll
+                for (int c4 = 0; c4 < kWQ / 4; ++c4) {
+                    const float4 t4 = reinterpret_cast<const float4 *>(W.bq)[c4];
+                    dq[4 * c4] = t4.x;
+                    dq[4 * c4 + 1] = t4.y;
+                    dq[4 * c4 + 2] = t4.z;
+                    dq[4 * c4 + 3] = t4.w;
+                }
                 const float dr = wave_sum_f(v * rh);
                 const float v2 = 2.f * v;
+                // fused multiply-adds (one rounding, one VALU instruction
+                // each; the FP32 build's rounding is its own)
 #pragma unroll
                 for (int c = 1; c < kWQ; ++c)
-                    r[c] -= v2 * lane_value(d, 4 * c);
-                rh -= v2 * dr;
+                    r[c] = __builtin_fmaf(-v2, dq[c], r[c]);
+                wave_sync(); // every lane has read bq before the next step writes it
+                rh = __builtin_fmaf(-v2, dr, rh);
                 if (lane == j)
                     dg = diagonal;
                 min_pivot = fminf(min_pivot, fabsf(diagonal));
@@ -1431,7 +1450,7 @@ spai_wave_kernel(const int *__restrict__ aptr, const int *__restrict__ acol,
                     }
                     cj = lane_value(cj, j);
                     if (lane < j)
-                        acc -= r[j] * cj;
+                        acc = __builtin_fmaf(-r[j], cj, acc);
                 }
                 const unsigned long long badm = __ballot(st != kSpaiSuccess);
                 if (badm != 0ull)
@@ -1443,11 +1462,20 @@ spai_wave_kernel(const int *__restrict__ aptr, const int *__restrict__ acol,
             }
             const float pivot_ratio = inf_norm == 0.f ? 0.f : min_pivot / inf_norm;
             // ---- residual s = B(I, J) m - e_k(I), from the gathered rows
+            // (the coefficients broadcast through LDS, as the step sums are)
+            if (lane < kWQ)
+                W.bq[lane] = coef;
+            wave_sync();
             float sres = 0.f;
 #pragma unroll
-            for (int c = 0; c < kWQ; ++c)
-                if (c < q)
-                    sres += W.R[lane][c] * lane_value(coef, c);
+            for (int c4 = 0; c4 < kWQ / 4; ++c4) {
+                const float4 t4 = reinterpret_cast<const float4 *>(W.bq)[c4];
+                const float cq[4] = {t4.x, t4.y, t4.z, t4.w};
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+                    if (4 * c4 + u < q)
+                        sres = __builtin_fmaf(W.R[lane][4 * c4 + u], cq[u], sres);
+            }
             if (lane == pk)
                 sres -= 1.f;
             if (lane >= m)
