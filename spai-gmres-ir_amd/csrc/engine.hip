@@ -428,43 +428,73 @@ struct PoolTable {
     const int *j[8];
     const void *c[8];
 };
-template <typename T>
-__global__ void p_count_pools_kernel(int n, const int *q, const int *pass,
-                                     const std::int64_t *off, PoolTable pt, const T *d, int *counts,
-                                     int *asm_fail) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= n)
-        return;
-    const int *pj = pt.j[pass[k]] + off[k];
-    const T *pc = static_cast<const T *>(pt.c[pass[k]]) + off[k];
+// P = N^T D from the column pools: row k of P is column k of N, entries
+// t = 0..q_k-1 in pattern order, value c_t d_{j_t}, zeros dropped.  One group
+// of 16 lanes per column takes 16 entries at a time, so that a wavefront's
+// loads cover four columns' consecutive pool slots (the first pass stores
+// column k at slot k) and its stores four consecutive rows of P; an entry's
+// place in its row is the count of kept entries before it (a ballot within
+// the group), so the row keeps pattern order.
+constexpr int kPoolGroup = 16;
+template <typename T, bool kFill>
+__device__ inline void pool_row(int k, const int *q, const int *pass, const std::int64_t *off,
+                                const PoolTable &pt, const T *d, int *counts, int *asm_fail,
+                                const int *pptr, int *pcol, T *pval) {
+    const int g = (int)(threadIdx.x % kWave) / kPoolGroup; // group within the wave
+    const int l = (int)(threadIdx.x % kPoolGroup);
+    const bool has = k >= 0;
+    const int qk = has ? q[k] : 0;
+    const int *pj = has ? pt.j[pass[k]] + off[k] : nullptr;
+    const T *pc = has ? static_cast<const T *>(pt.c[pass[k]]) + off[k] : nullptr;
+    int w = kFill && has ? pptr[k] : 0;
     int cnt = 0;
-    for (int t = 0; t < q[k]; ++t) {
-        const T cf = pc[t];
-        const T v = cf * d[pj[t]];
-        if (!isfinite(v) || (cf != 0 && v == 0))
-            atomicMin(asm_fail, k);
-        cnt += v != 0;
+    const unsigned long long below = (1ull << (g * kPoolGroup + l)) - 1ull;
+    const unsigned long long gmask = 0xffffull << (g * kPoolGroup);
+    // every lane runs the largest group's rounds of its wave (ballots)
+    int rounds = (qk + kPoolGroup - 1) / kPoolGroup;
+    for (int o = 1; o < kWave; o <<= 1)
+        rounds = max(rounds, __shfl_xor(rounds, o, kWave));
+    for (int i = 0; i < rounds; ++i) {
+        const int t = i * kPoolGroup + l;
+        const bool in = t < qk;
+        T v = 0;
+        int j = 0;
+        if (in) {
+            j = pj[t];
+            const T cf = pc[t];
+            v = cf * d[j];
+            if (!kFill && (!isfinite(v) || (cf != 0 && v == 0)))
+                atomicMin(asm_fail, k);
+        }
+        const bool keep = in && v != 0;
+        const unsigned long long m = __ballot(keep) & gmask;
+        if (kFill && keep) {
+            const int at = w + __popcll(m & below);
+            pcol[at] = j;
+            pval[at] = v;
+        }
+        const int kept = __popcll(m);
+        w += kept;
+        cnt += kept;
     }
-    counts[k + 1] = cnt;
+    if (!kFill && has && l == 0)
+        counts[k + 1] = cnt;
 }
 template <typename T>
-__global__ void p_fill_pools_kernel(int n, const int *q, const int *pass, const std::int64_t *off,
-                                    PoolTable pt, const T *d, const int *pptr, int *pcol,
-                                    T *pval) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= n)
-        return;
-    const int *pj = pt.j[pass[k]] + off[k];
-    const T *pc = static_cast<const T *>(pt.c[pass[k]]) + off[k];
-    int w = pptr[k];
-    for (int t = 0; t < q[k]; ++t) {
-        const T v = pc[t] * d[pj[t]];
-        if (v != 0) {
-            pcol[w] = pj[t];
-            pval[w] = v;
-            ++w;
-        }
-    }
+__global__ void __launch_bounds__(256)
+p_count_pools_kernel(int n, const int *q, const int *pass, const std::int64_t *off, PoolTable pt,
+                     const T *d, int *counts, int *asm_fail) {
+    const std::int64_t k = ((std::int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kPoolGroup;
+    pool_row<T, false>(k < n ? (int)k : -1, q, pass, off, pt, d, counts, asm_fail, nullptr,
+                       nullptr, nullptr);
+}
+template <typename T>
+__global__ void __launch_bounds__(256)
+p_fill_pools_kernel(int n, const int *q, const int *pass, const std::int64_t *off, PoolTable pt,
+                    const T *d, const int *pptr, int *pcol, T *pval) {
+    const std::int64_t k = ((std::int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kPoolGroup;
+    pool_row<T, true>(k < n ? (int)k : -1, q, pass, off, pt, d, nullptr, nullptr, pptr, pcol,
+                      pval);
 }
 template <typename T>
 __global__ void patterns_kernel(int n, const int *q, const int *pass, const std::int64_t *off,
@@ -826,7 +856,8 @@ void spai_build_typed(Context &c, const DevCsr &a, const T *aval, T *d, double e
     out.p.n = n;
     out.p.ptr.resize(n + 1);
     HIP_CHECK(hipMemsetAsync(out.p.ptr.data(), 0, sizeof(int), s));
-    hipLaunchKernelGGL(p_count_pools_kernel<T>, dim3(ceil_div(n, 256)), dim3(256), 0, s, n,
+    const unsigned pool_blocks = (unsigned)ceil_div((std::int64_t)n * kPoolGroup, 256);
+    hipLaunchKernelGGL(p_count_pools_kernel<T>, dim3(pool_blocks), dim3(256), 0, s, n,
                        out.c_q.data(), out.c_pass.data(), out.c_off.data(), pt, (const T *)d,
                        out.p.ptr.data(), ff.data() + 1);
     scan_inclusive_int(s, out.p.ptr.data(), n + 1, scan_tmp.data());
@@ -861,7 +892,7 @@ void spai_build_typed(Context &c, const DevCsr &a, const T *aval, T *d, double e
         out.p.has64 = false;
     }
     T *pval = sizeof(T) == 8 ? (T *)(void *)out.p.val.data() : (T *)(void *)out.p.val32.data();
-    hipLaunchKernelGGL(p_fill_pools_kernel<T>, dim3(ceil_div(n, 256)), dim3(256), 0, s, n,
+    hipLaunchKernelGGL(p_fill_pools_kernel<T>, dim3(pool_blocks), dim3(256), 0, s, n,
                        out.c_q.data(), out.c_pass.data(), out.c_off.data(), pt, (const T *)d,
                        out.p.ptr.data(), out.p.col.data(), pval);
     out.p.lanes = choose_lanes(n, pnnz);
