@@ -403,22 +403,34 @@ __global__ void column_summary_kernel(int n, const int *status, const int *fits,
                                       const int *clamps, const int *q, const double *flops,
                                       const int *pass, int npass, int *first_fail,
                                       unsigned long long *tot, double *ftot) {
-    // Block sums first (exact: the counts stay far below 2^53), one atomic
-    // per block and counter.
+    // Each thread sums a strided share of the columns, then block sums
+    // (exact: the counts stay far below 2^53), one atomic per block and
+    // counter.
     __shared__ double lds[256 / kWave];
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool in = k < n;
-    if (in && status[k] != kSpaiSuccess)
-        atomicMin(first_fail, k);
-    const int *src[4] = {fits, growths, clamps, q};
+    double cnt[4] = {0, 0, 0, 0}, pf[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    int fail = INT_MAX;
+    for (std::int64_t k = (std::int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n;
+         k += (std::int64_t)gridDim.x * blockDim.x) {
+        if (status[k] != kSpaiSuccess)
+            fail = min(fail, (int)k);
+        cnt[0] += fits[k];
+        cnt[1] += growths[k];
+        cnt[2] += clamps[k];
+        cnt[3] += q[k];
+        const int p = pass[k];
+        for (int u = 0; u < 8; ++u)
+            pf[u] += u == p ? flops[k] : 0.0;
+    }
+    if (fail != INT_MAX)
+        atomicMin(first_fail, fail);
     for (int c = 0; c < 4; ++c) {
-        const double v = block_sum(in ? (double)src[c][k] : 0.0, lds);
+        const double v = block_sum(cnt[c], lds);
         if (threadIdx.x == 0)
             atomicAdd(tot + c, (unsigned long long)v);
     }
     // QR flops per capacity pass (telemetry only: summation order does not matter)
     for (int p = 0; p < npass; ++p) {
-        const double f = block_sum(in && pass[k] == p ? flops[k] : 0.0, lds);
+        const double f = block_sum(pf[p], lds);
         if (threadIdx.x == 0)
             atomicAdd(ftot + p, f);
     }
@@ -844,7 +856,8 @@ void spai_build_typed(Context &c, const DevCsr &a, const T *aval, T *d, double e
         HIP_CHECK(hipMemsetAsync(tot.data(), 0, 4 * sizeof(unsigned long long), s));
         HIP_CHECK(hipMemsetAsync(ftot.data(), 0, 8 * sizeof(double), s));
     }
-    hipLaunchKernelGGL(column_summary_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, s, n,
+    hipLaunchKernelGGL(column_summary_kernel, dim3(std::min<unsigned>(ceil_div(n, 256), 1024)),
+                       dim3(256), 0, s, n,
                        out.c_status.data(), out.c_fits.data(), out.c_growths.data(),
                        out.c_clamps.data(), out.c_q.data(), out.c_flops.data(),
                        out.c_pass.data(), out.passes, ff.data(), tot.data(), ftot.data());
