@@ -1296,14 +1296,29 @@ spai_wave_kernel(const int *__restrict__ aptr, const int *__restrict__ acol,
 #pragma unroll
             for (int c = 0; c < kWW; ++c)
                 W.R[lane][c] = 0.f;
-            const int na = __builtin_amdgcn_readfirstlane(
-                wave_rows(W, aptr, dscale, lane < q ? W.J[lane] : 0, q, lane));
-            if (na < 0) {
-                status = kSpaiOverflow;
-                break;
+            int m;
+            if (q == 1) {
+                // the first fit: I is row k of A itself (a canonical CSR row
+                // is sorted and free of duplicates), R(r, 0) = a_k,I[r] d_k
+                const int e0 = gld(aptr + k), len = gld(aptr + k + 1) - e0;
+                m = len;
+                if (len <= kWM) {
+                    const float dk = gld(dscale + k);
+                    if (lane < len) {
+                        W.I[lane] = gld(acol + e0 + lane);
+                        W.R[lane][0] = gld(aval + e0 + lane) * dk;
+                    }
+                    wave_sync();
+                }
+            } else {
+                const int na = __builtin_amdgcn_readfirstlane(
+                    wave_rows(W, aptr, dscale, lane < q ? W.J[lane] : 0, q, lane));
+                if (na < 0) {
+                    status = kSpaiOverflow;
+                    break;
+                }
+                m = __builtin_amdgcn_readfirstlane(wave_shadow(W, acol, aval, na, lane));
             }
-            const int m =
-                __builtin_amdgcn_readfirstlane(wave_shadow(W, acol, aval, na, lane));
             if (m > kWM) {
                 status = kSpaiOverflow;
                 break;
