@@ -1079,6 +1079,9 @@ template <typename T> void GmresEngine<T>::ensure_basis(int count) {
     // update 116 -> 109 us per launch; FP64 projection 280 -> 247 us.
     constexpr std::size_t kBasisSkew = 576;
     const std::size_t ld = (std::size_t)((L_.n + 63) / 64 * 64) + kBasisSkew;
+    // (the table's earlier upload may still be queued behind a running
+    // graph: let it finish before the host vector can move)
+    sync(c_);
     while ((int)vptrs_.size() < count) {
         const int take = std::min(chunk, cap_ + 1 - (int)vptrs_.size());
         chunks_.emplace_back((std::size_t)take * ld);

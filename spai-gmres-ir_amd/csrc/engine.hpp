@@ -313,7 +313,11 @@ template <typename T> class GmresEngine {
     std::vector<GmresOutcome> outcomes();
     KrylovArgs<T> args() const { return args_; }
     int cap() const { return cap_; }
-    int sync_every = 4;
+    // Host check points every sync_every steps: two replayed graphs queued
+    // per check, so the card is not idle while the host reads the state.
+    // Growth of the workspace synchronizes before it moves buffers, and
+    // steps queued behind a triggered or finished one do nothing.
+    int sync_every = 8;
     // Arnoldi steps per replayed graph (BSPAI_GMRES_GRAPH=0 launches every
     // kernel on the stream instead); graphs_replayed counts the replays of
     // the last solve.
