@@ -1269,8 +1269,14 @@ void GmresEngine<T>::solve(const DevCsr &m1, const DevCsr &m2, GmresMode mode, c
     // from 64 steps), and a growth moves H, the Gram triangle and the
     // partials, so the graph is captured again for the larger capacity.
     const char *genv = std::getenv("BSPAI_GMRES_GRAPH");
-    const bool graphs = !(genv && std::string(genv) == "0") && !c_.timer.on &&
-                        cap_ >= kGraphSteps && sync_every % kGraphSteps == 0;
+    // With the step kernels timed (not only the SPAI build's), every kernel
+    // is launched on the stream with its events, and the host checks after
+    // every step, so that no held step (which exits at once) is launched
+    // and counted with a full step's bytes.
+    const bool timed_steps = c_.timer.on && !c_.timer.spai_only;
+    const int check_every = timed_steps ? 1 : sync_every;
+    const bool graphs = !(genv && std::string(genv) == "0") && !timed_steps &&
+                        cap_ >= kGraphSteps && check_every % kGraphSteps == 0;
     graphs_replayed = 0;
     if (graphs && kdev_.size() == 0)
         kdev_.resize(1);
@@ -1349,7 +1355,7 @@ void GmresEngine<T>::solve(const DevCsr &m1, const DevCsr &m2, GmresMode mode, c
             ++k;
         }
         (void)k0;
-        if (k % sync_every == 0 || k == cap_) {
+        if (k % check_every == 0 || k == cap_) {
             download(c_, host.data(), st_.data(), L_.nsys);
             sync(c_);
             if (defer && host[0].active && host[0].triggered) {
